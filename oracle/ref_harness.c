@@ -1,0 +1,402 @@
+/*
+ * Test-infrastructure driver around the compiled reference library
+ * (oracle/_ref/libgamera_ref.so). It builds a gcs_problem_config problem
+ * with the reference's own setup functions and advances it with the
+ * reference's own step, following the serial Earth driver
+ * (cubedsphere/tests/test_cubedsphere_dipole_background.c:330-699).
+ *
+ * It is compiled against the reference headers only; the problem config is
+ * restated here field by field (same layout as include/gamera_cs_problem.h,
+ * checked by a size probe from Python) so that this file needs no header
+ * from the rest of the repository.
+ *
+ * Only tests and the CPU-baseline leg of bench.py load this library.
+ */
+#include "cubedsphere_earth_boundary.h"
+#include "cubedsphere_earth_startup.h"
+#include "cubedsphere_halo.h"
+#include "cubedsphere_mi_dipole.h"
+#include "cubedsphere_step.h"
+#include "nonorthogonal_background.h"
+#include "nonorthogonal_initialization.h"
+#include "nonorthogonal_radial_map.h"
+#include "nonorthogonal_state.h"
+#include "nonorthogonal_step.h"
+#include "solar_wind.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+enum { REFH_MAX_WIND = 8 };
+
+typedef struct {
+  size_t radial_cells, angular_cells, ghost;
+  int radial_map_version;
+  double inner_radius, outer_radius, radial_stretch;
+  gamera_cs_earth_startup startup;
+  double perturbation;
+  uint64_t seed;
+  int use_boris, strict_inner_gas_wall;
+  double dt_fraction, cfl;
+  size_t wind_count;
+  int wind_linear;
+  double wind_time[REFH_MAX_WIND];
+  double wind_bz[REFH_MAX_WIND];
+  double wind_reference_x;
+  int mi_enabled;
+  double hall_siemens;
+  gamera_cs_mi_dipole_config mi;
+} refh_config;
+
+typedef struct {
+  refh_config cfg;
+  size_t active[3], ghost, extent[3];
+  gamera_no_grid grid[GAMERA_CS_FACE_COUNT];
+  gamera_no_storage storage[GAMERA_CS_FACE_COUNT];
+  gamera_no_background_data owned[GAMERA_CS_FACE_COUNT];
+  gamera_no_background_field background[GAMERA_CS_FACE_COUNT];
+  gamera_solar_wind_series wind;
+  gamera_cs_earth_boundary boundary;
+  gamera_no_advance_options options;
+  gamera_no_timestep_options timestep;
+  gamera_cs_halo_plan *halo_plan;
+  gamera_cs_mi_dipole *mi;
+  double time, previous_dt, dt;
+  long steps;
+  int mi_updates;
+} refh_problem;
+
+size_t refh_config_size(void) { return sizeof(refh_config); }
+
+/* splitmix64 finaliser; the perturbation hash of gcs_problem_config. */
+static uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+static double hashed_unit(uint64_t seed, int face, size_t i, size_t j,
+                          size_t k, int variable) {
+  uint64_t x = mix64(seed + (uint64_t)face);
+  x = mix64(x + (uint64_t)i);
+  x = mix64(x + (uint64_t)j);
+  x = mix64(x + (uint64_t)k);
+  x = mix64(x + (uint64_t)variable);
+  return 2.0 * ((double)(x >> 11) * 0x1.0p-53) - 1.0;
+}
+
+static int perturb(refh_problem *p, int face) {
+  const double amp = p->cfg.perturbation;
+  if (amp == 0.0) return 0;
+  const double gamma = p->cfg.startup.gamma;
+  for (size_t i = 0; i < p->extent[0]; ++i)
+    for (size_t j = 0; j < p->extent[1]; ++j)
+      for (size_t k = 0; k < p->extent[2]; ++k) {
+        const size_t cell = gamera_no_index3(p->extent, i, j, k);
+        double *u = &p->storage[face].conserved[cell * GAMERA_NO_FLUX_COUNT];
+        gamera_no_primitive w;
+        if (gamera_no_conserved_to_primitive(u, gamma, 1e-6, 1e-8, &w) != 0)
+          return -1;
+        w.density *= 1.0 + amp * hashed_unit(p->cfg.seed, face, i, j, k, 0);
+        w.pressure *= 1.0 + amp * hashed_unit(p->cfg.seed, face, i, j, k, 1);
+        if (gamera_no_primitive_to_conserved(&w, gamma, 1e-6, 1e-8, u) != 0)
+          return -1;
+      }
+  return 0;
+}
+
+static double cfl_limit(refh_problem *p) {
+  const size_t lower[3] = {p->ghost, p->ghost, p->ghost};
+  const size_t upper[3] = {p->ghost + p->active[0], p->ghost + p->active[1],
+                           p->ghost + p->active[2]};
+  double limit = INFINITY;
+  for (int face = 0; face < GAMERA_CS_FACE_COUNT; ++face) {
+    double local;
+    if (gamera_no_local_timestep(&p->grid[face], p->storage[face].conserved,
+                                 p->storage[face].cell_magnetic, lower, upper,
+                                 &p->timestep, p->owned[face].cell_magnetic,
+                                 &local) != 0)
+      return NAN;
+    limit = fmin(limit, local);
+  }
+  return limit;
+}
+
+void refh_destroy(refh_problem *p) {
+  if (p == NULL) return;
+  gamera_cs_halo_plan_destroy(p->halo_plan);
+  gamera_cs_mi_dipole_destroy(p->mi);
+  for (int face = 0; face < GAMERA_CS_FACE_COUNT; ++face) {
+    gamera_no_background_destroy(&p->owned[face]);
+    gamera_no_storage_destroy(&p->storage[face]);
+    gamera_no_grid_destroy(&p->grid[face]);
+  }
+  gamera_solar_wind_destroy(&p->wind);
+  free(p);
+}
+
+refh_problem *refh_create(const refh_config *cfg) {
+  if (cfg == NULL || cfg->ghost < 4 || cfg->wind_count < 1 ||
+      cfg->wind_count > REFH_MAX_WIND)
+    return NULL;
+  refh_problem *p = calloc(1, sizeof(*p));
+  if (p == NULL) return NULL;
+  p->cfg = *cfg;
+  p->ghost = cfg->ghost;
+  p->active[0] = cfg->radial_cells;
+  p->active[1] = p->active[2] = cfg->angular_cells;
+  for (int d = 0; d < 3; ++d) p->extent[d] = p->active[d] + 2 * p->ghost;
+  gamera_solar_wind_init(&p->wind);
+  const gamera_cs_earth_startup *startup = &p->cfg.startup;
+  const double gamma = startup->gamma;
+
+  gamera_no_radial_map map;
+  if (gamera_no_radial_map_init(&map, cfg->radial_map_version,
+                                cfg->inner_radius, cfg->outer_radius,
+                                cfg->radial_stretch) != 0)
+    goto fail;
+  const size_t vertex_count = (p->extent[0] + 1) * (p->extent[1] + 1) *
+                              (p->extent[2] + 1);
+  for (int face = 0; face < GAMERA_CS_FACE_COUNT; ++face) {
+    gamera_no_vec3 *vertices = calloc(vertex_count, sizeof(*vertices));
+    const int ok = vertices != NULL &&
+        gamera_cs_generate_vertices_radial_map(face, p->active, p->ghost,
+                                               &map, vertices) == 0 &&
+        gamera_no_grid_create(p->extent, vertices, &p->grid[face]) == 0 &&
+        gamera_cs_calibrate_face_frames(&p->grid[face]) == 0 &&
+        gamera_no_storage_create(p->extent, &p->storage[face]) == 0 &&
+        gamera_no_background_create(&p->grid[face],
+                                    gamera_cs_earth_startup_background,
+                                    (void *)startup, &p->owned[face]) == 0;
+    free(vertices);
+    if (!ok) goto fail;
+    p->background[face] = p->owned[face].field;
+    if (gamera_no_initialize_primitives(&p->grid[face], &p->storage[face],
+                                        gamera_cs_earth_startup_primitive,
+                                        (void *)startup, false, gamma, 1e-6,
+                                        1e-8) != 0 ||
+        perturb(p, face) != 0 ||
+        gamera_no_initialize_flux_from_vector_potential(
+            &p->grid[face], &p->storage[face],
+            gamera_cs_earth_startup_potential, (void *)startup) != 0 ||
+        gamera_no_save_current_as_old(&p->storage[face]) != 0 ||
+        gamera_cs_earth_background_adjust(&p->grid[face], &p->owned[face],
+                                          startup) != 0 ||
+        gamera_cs_earth_total_to_residual(&p->grid[face], &p->storage[face],
+                                          &p->owned[face]) != 0)
+      goto fail;
+  }
+
+  p->timestep = (gamera_no_timestep_options){
+      gamma, 1e-6, 1e-8, cfg->cfl, 10.0, true, cfg->use_boris != 0, true};
+  p->options = (gamera_no_advance_options){
+      .stress = {gamma, 1e-6, 1e-8, 1.0, 0.25, 0.25, 10.0, true, true,
+                 cfg->use_boris != 0, true},
+      .emf = {1e-6, 1.0, 0.5, 10.0, cfg->cfl, 0.0, cfg->use_boris != 0, true},
+      .update = {gamma, 1e-6, 1e-8, 10.0, true, cfg->use_boris != 0, true}};
+
+  gamera_solar_wind_state states[REFH_MAX_WIND];
+  for (size_t n = 0; n < cfg->wind_count; ++n) {
+    states[n] = (gamera_solar_wind_state){
+        .density = startup->density,
+        .pressure = startup->density * startup->speed * startup->speed /
+                    (startup->gamma * startup->mach * startup->mach),
+        .velocity = {startup->speed, 0.0, 0.0},
+        .magnetic = {startup->imf.value[0], startup->imf.value[1],
+                     startup->imf.value[2]}};
+    if (cfg->wind_count > 1) states[n].magnetic[GAMERA_WIND_Z] =
+        cfg->wind_bz[n];
+  }
+  p->wind.linear_interpolation = cfg->wind_count > 1 ? cfg->wind_linear : 1;
+  if (gamera_solar_wind_set(&p->wind, cfg->wind_count, cfg->wind_time,
+                            states) != 0)
+    goto fail;
+  p->wind.reference[0] = cfg->wind_reference_x;
+  p->boundary = (gamera_cs_earth_boundary){
+      .startup = startup, .wind = &p->wind, .background = p->background,
+      .density_floor = 1e-6, .pressure_floor = 1e-8, .cfl = cfg->cfl,
+      .strict_inner_gas_wall = cfg->strict_inner_gas_wall};
+  p->options.fluid_flux_sync = gamera_cs_earth_inner_fluid_flux;
+  p->options.edge_emf_sync = gamera_cs_earth_no_mi_emf;
+  p->options.edge_emf_context = &p->boundary;
+  if (cfg->mi_enabled) {
+    if (cfg->hall_siemens != 0.0 ||
+        gamera_cs_mi_dipole_create(p->grid, p->storage, p->background,
+                                   startup, p->active, p->ghost, &cfg->mi,
+                                   &p->mi) != 0)
+      goto fail;
+    p->boundary.mi = p->mi;
+  }
+  if (gamera_cs_halo_plan_create(p->grid, p->active, p->ghost,
+                                 &p->halo_plan) != 0)
+    goto fail;
+  const double limit = cfl_limit(p);
+  if (!(limit > 0.0)) goto fail;
+  p->dt = cfg->dt_fraction * limit;
+  return p;
+fail:
+  refh_destroy(p);
+  return NULL;
+}
+
+/* One driver iteration: dt rule, MI cadence, one six-face step. */
+static int advance_one(refh_problem *p) {
+  if (p->steps > 0) {
+    const double limit = cfl_limit(p);
+    if (!(limit > 0.0)) return -1;
+    p->dt = fmin(p->cfg.dt_fraction * limit, 1.25 * p->previous_dt);
+  }
+  if (p->mi != NULL) {
+    const int updated = gamera_cs_mi_dipole_maybe_update(p->mi, p->time);
+    if (updated < 0) return -1;
+    p->mi_updates += updated;
+  }
+  p->boundary.step_dt = p->dt;
+  p->boundary.step_time = p->time;
+  const double ratio = p->steps == 0 ? 0.0 : 0.5 * p->dt / p->previous_dt;
+  if (gamera_cs_advance_six_with_background_planned(
+          p->grid, p->storage, p->active, p->ghost, ratio, p->dt, &p->options,
+          p->background, p->halo_plan, gamera_cs_earth_fill_radial_ghosts,
+          p->time, p->steps == 0 ? p->time : p->time - p->previous_dt,
+          &p->boundary) != 0)
+    return -1;
+  p->time += p->dt;
+  p->previous_dt = p->dt;
+  ++p->steps;
+  return 0;
+}
+
+int refh_advance(refh_problem *p, long count) {
+  if (p == NULL || count < 0) return -1;
+  for (long n = 0; n < count; ++n)
+    if (advance_one(p) != 0) return -1;
+  return 0;
+}
+
+/* One step with caller-chosen dt and predictor ratio (no dt rule, no MI). */
+int refh_step_explicit(refh_problem *p, double ratio, double dt,
+                       double time, double previous_time) {
+  if (p == NULL) return -1;
+  p->boundary.step_dt = dt;
+  p->boundary.step_time = time;
+  return gamera_cs_advance_six_with_background_planned(
+      p->grid, p->storage, p->active, p->ghost, ratio, dt, &p->options,
+      p->background, p->halo_plan, gamera_cs_earth_fill_radial_ghosts, time,
+      previous_time, &p->boundary);
+}
+
+void refh_clock(const refh_problem *p, double out[4]) {
+  out[0] = p->time;
+  out[1] = p->dt;
+  out[2] = p->previous_dt;
+  out[3] = (double)p->steps;
+}
+
+double refh_cfl_limit(refh_problem *p) { return cfl_limit(p); }
+
+/* Pointer to one array of one face and its length in doubles. */
+int refh_field(refh_problem *p, int face, int field, double **data,
+               size_t *count) {
+  if (p == NULL || face < 0 || face >= GAMERA_CS_FACE_COUNT || data == NULL ||
+      count == NULL)
+    return -1;
+  gamera_no_grid *g = &p->grid[face];
+  gamera_no_storage *s = &p->storage[face];
+  gamera_no_background_data *b = &p->owned[face];
+  const size_t cells = gamera_no_element_count3(g->cell_extent);
+  size_t faces[3], edges[3];
+  for (int d = 0; d < 3; ++d) {
+    faces[d] = gamera_no_element_count3(g->face[d].extent);
+    edges[d] = gamera_no_element_count3(g->edge[d].extent);
+  }
+  /* Field ids follow include/gamera_cs_problem.h. */
+  if (field >= 0 && field <= 5) {
+    double *arrays[6] = {s->conserved, s->old_conserved,
+                         s->predicted_conserved,
+                         s->cell_magnetic->value, s->old_cell_magnetic->value,
+                         s->predicted_cell_magnetic->value};
+    *data = arrays[field];
+    *count = cells * (field < 3 ? 5 : 3);
+    return 0;
+  }
+  if (field >= 6 && field < 24) {
+    const int group = (field - 6) / 3, d = (field - 6) % 3;
+    switch (group) {
+      case 0: *data = s->face_flux[d]; *count = faces[d]; return 0;
+      case 1: *data = s->old_face_flux[d]; *count = faces[d]; return 0;
+      case 2: *data = s->predicted_face_flux[d]; *count = faces[d]; return 0;
+      case 3: *data = s->edge_emf[d]; *count = edges[d]; return 0;
+      case 4: *data = s->fluid_face_flux[d]; *count = 5 * faces[d]; return 0;
+      default: *data = s->maxwell_face_flux[d]->value; *count = 3 * faces[d];
+               return 0;
+    }
+  }
+  switch (field) {
+    case 24: *data = s->hydro_rate; *count = 5 * cells; return 0;
+    case 25: *data = s->maxwell_rate->value; *count = 3 * cells; return 0;
+    case 26: *data = g->vertex->value;
+             *count = 3 * gamera_no_element_count3(g->vertex_extent); return 0;
+    case 27: *data = &g->cell->volume;
+             *count = cells * (sizeof(gamera_no_cell_geometry) / 8); return 0;
+    default: break;
+  }
+  if (field >= 28 && field < 31) {
+    const int d = field - 28;
+    *data = &g->face[d].value->area;
+    *count = faces[d] * (sizeof(gamera_no_face_geometry) / 8);
+    return 0;
+  }
+  if (field >= 31 && field < 34) {
+    const int d = field - 31;
+    *data = &g->edge[d].value->length;
+    *count = edges[d] * (sizeof(gamera_no_edge_geometry) / 8);
+    return 0;
+  }
+  if (field == 34) { *data = b->cell_magnetic->value; *count = 3 * cells; return 0; }
+  if (field == 35) { *data = b->cell_force->value; *count = 3 * cells; return 0; }
+  if (field >= 36 && field < 39) {
+    *data = b->face_magnetic[field - 36]->value;
+    *count = 3 * faces[field - 36];
+    return 0;
+  }
+  if (field >= 39 && field < 42) {
+    *data = b->face_flux[field - 39];
+    *count = faces[field - 39];
+    return 0;
+  }
+  if (field >= 42 && field < 45) {
+    *data = b->edge_magnetic[field - 42]->value;
+    *count = 3 * edges[field - 42];
+    return 0;
+  }
+  return -1;
+}
+
+int refh_edge_valid(refh_problem *p, int face, int direction,
+                    unsigned char **data, size_t *count) {
+  if (p == NULL || face < 0 || face >= GAMERA_CS_FACE_COUNT || direction < 0 ||
+      direction > 2)
+    return -1;
+  *data = p->grid[face].edge[direction].valid;
+  *count = gamera_no_element_count3(p->grid[face].edge[direction].extent);
+  return 0;
+}
+
+/* Diagnostic counters written by the step into storage. */
+void refh_counters(const refh_problem *p, int face, double out[2]) {
+  out[0] = (double)p->storage[face].emergency_diffusion_face_count;
+  out[1] = p->storage[face].emergency_diffusion_max_interface_speed;
+}
+
+int refh_mi_updates(const refh_problem *p) { return p->mi_updates; }
+
+/* Both hemispheres' potential [hemi][colatitude][azimuth] in volts. */
+int refh_mi_potential(const refh_problem *p, double *out, size_t count) {
+  if (p == NULL || p->mi == NULL) return -1;
+  double epoch, next;
+  return gamera_cs_mi_dipole_export_potential(p->mi, out, count, &epoch,
+                                              &next);
+}
