@@ -1,0 +1,365 @@
+// C ABI of the HIP library: the problem API (create / advance / inspect a
+// device-resident six-face Earth problem). Declared in
+// include/gamera_cs_step.h.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gcs_device.hpp"
+
+namespace gcs {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string &message) { g_error = message; }
+const char *last_error() { return g_error.c_str(); }
+
+namespace {
+
+int check_config(const gcs_problem_config &c) {
+  if (c.ghost != 4) return set_error("ghost must be 4"), -1;
+  if (c.radial_cells < 2 || c.angular_cells < 9)
+    return set_error("grid needs nr >= 2 and n >= 9"), -1;
+  if (c.wind_count < 1 || c.wind_count > GCS_MAX_WIND_SAMPLES)
+    return set_error("wind_count out of range"), -1;
+  if (!(c.startup.gamma > 1.0) || !(c.dt_fraction > 0.0) || !(c.cfl > 0.0))
+    return set_error("invalid gamma, dt fraction or CFL"), -1;
+  if (c.mi_enabled) return set_error("MI coupling is not built yet"), -1;
+  const long cells = (long)(c.radial_cells + 9) * (c.angular_cells + 9) *
+                     ((c.angular_cells + 9 + 7) / 8 * 8) * 6;
+  if (cells >= (1L << 31)) return set_error("grid too large for 32-bit indices"), -1;
+  return 0;
+}
+
+// Wind series of the config (solar_wind.c:142-171 with the startup state).
+void make_wind(Problem &P) {
+  const gcs_problem_config &c = P.cfg;
+  const gamera_cs_earth_startup &s = c.startup;
+  WindSeries &w = P.wind_series;
+  std::memset(&w, 0, sizeof(w));
+  w.count = (int)c.wind_count;
+  w.linear = c.wind_count > 1 ? c.wind_linear : 1;
+  for (size_t n = 0; n < c.wind_count; ++n) {
+    w.time[n] = c.wind_time[n];
+    double *st = w.state[n];
+    st[0] = s.density;
+    st[1] = s.density * s.speed * s.speed / (s.gamma * s.mach * s.mach);
+    st[2] = s.speed;
+    st[3] = st[4] = 0.0;
+    for (int d = 0; d < 3; ++d) st[5 + d] = s.imf.value[d];
+    if (c.wind_count > 1) st[7] = c.wind_bz[n];
+  }
+  w.reference[0] = c.wind_reference_x;
+}
+
+void free_problem(Problem *P) {
+  if (P == nullptr) return;
+  if (P->graph) (void)hipGraphExecDestroy(P->graph);
+  (void)hipFree(P->planes);
+  (void)hipFree(P->bytes);
+  (void)hipFree(P->scalars);
+  (void)hipFree(P->halo_cells);
+  (void)hipFree(P->halo_faces);
+  (void)hipFree(P->seams);
+  (void)hipFree(P->edge_members);
+  (void)hipFree(P->edge_groups);
+  (void)hipFree(P->seam_cells);
+  if (P->ev_start) (void)hipEventDestroy(P->ev_start);
+  if (P->ev_stop) (void)hipEventDestroy(P->ev_stop);
+  if (P->stream) (void)hipStreamDestroy(P->stream);
+  delete P;
+}
+
+int init_scalars(Problem &P) {
+  StepScalars s;
+  std::memset(&s, 0, sizeof(s));
+  const double big = DBL_MAX;
+  std::memcpy(&s.cfl_min_bits, &big, sizeof(big));
+  GCS_HIP(hipMemcpy(P.scalars, &s, sizeof(s), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int read_scalars(Problem &P, StepScalars &s) {
+  GCS_HIP(hipStreamSynchronize(P.stream));
+  GCS_HIP(hipMemcpy(&s, P.scalars, sizeof(s), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int check_failure(Problem &P) {
+  StepScalars s;
+  if (read_scalars(P, s) != 0) return -1;
+  if (s.failed) {
+    char msg[160];
+    const long per = P.L.face_stride;
+    const long at = s.fail_index;
+    const int f = (int)(at / per);
+    const long r = at - f * per;
+    std::snprintf(msg, sizeof(msg),
+                  "MHD_FAILURE stage=%u face=%d local=%ld,%ld,%ld", s.fail_stage, f,
+                  r / P.L.di(), (r % P.L.di()) / P.L.SK, r % P.L.SK);
+    set_error(msg);
+    return -1;
+  }
+  P.time = s.time;
+  P.dt = s.dt;
+  P.previous_dt = s.previous_dt;
+  P.steps = (long)s.steps;
+  return 0;
+}
+
+// Copy one component plane of one cube face to the host in the
+// reference's k-fastest order for extent e.
+int fetch(Problem &P, int plane, int face, const size_t e[3], int ncomp,
+          int comp, std::vector<double> &out) {
+  const Layout &L = P.L;
+  std::vector<double> buf(L.face_stride);
+  GCS_HIP(hipMemcpy(buf.data(), P.plane(plane) + face * L.face_stride,
+                    L.face_stride * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < e[0]; ++i)
+    for (size_t j = 0; j < e[1]; ++j)
+      for (size_t k = 0; k < e[2]; ++k)
+        out[((i * e[1] + j) * e[2] + k) * ncomp + comp] =
+            buf[(i * L.SJ + j) * L.SK + k];
+  return 0;
+}
+
+}  // namespace
+
+int download_field(Problem &P, int face, int field, std::vector<double> &out) {
+  const Layout &L = P.L;
+  const int cur = P.level, old = 1 - P.level;
+  size_t ce[3] = {(size_t)L.NI, (size_t)L.NJ, (size_t)L.NK};
+  size_t ve[3] = {(size_t)L.NI + 1, (size_t)L.NJ + 1, (size_t)L.NK + 1};
+  const size_t ncell = ce[0] * ce[1] * ce[2];
+  auto planes = [&](int first, int ncomp, const size_t e[3]) {
+    out.assign(e[0] * e[1] * e[2] * ncomp, 0.0);
+    for (int q = 0; q < ncomp; ++q)
+      if (fetch(P, first + q, face, e, ncomp, q, out) != 0) return -1;
+    return 0;
+  };
+  if (field == GCS_FIELD_CONSERVED) return planes(plane_u(cur, 0), 5, ce);
+  if (field == GCS_FIELD_OLD_CONSERVED) return planes(plane_u(old, 0), 5, ce);
+  if (field == GCS_FIELD_PREDICTED_CONSERVED) return planes(P_UPRED, 5, ce);
+  if (field == GCS_FIELD_CELL_MAGNETIC) return planes(plane_b(cur, 0), 3, ce);
+  if (field == GCS_FIELD_OLD_CELL_MAGNETIC) return planes(plane_b(old, 0), 3, ce);
+  if (field == GCS_FIELD_PREDICTED_CELL_MAGNETIC) return planes(P_BPRED, 3, ce);
+  if (field >= GCS_FIELD_FACE_FLUX && field < GCS_FIELD_HYDRO_RATE) {
+    const int group = (field - GCS_FIELD_FACE_FLUX) / 3;
+    const int d = (field - GCS_FIELD_FACE_FLUX) % 3;
+    size_t fe[3], ee[3];
+    face_extent(L, d, fe);
+    edge_extent(L, d, ee);
+    switch (group) {
+      case 0: return planes(plane_phi(cur, d), 1, fe);
+      case 1: return planes(plane_phi(old, d), 1, fe);
+      case 2: return planes(P_PHIPRED + d, 1, fe);
+      case 3: return planes(P_EMF + d, 1, ee);
+      case 4: return planes(P_FLUID + 5 * d, 5, fe);
+      default: return planes(P_MAXWELL + 3 * d, 3, fe);
+    }
+  }
+  if (field == GCS_FIELD_HYDRO_RATE) {  // fused into the update, not stored
+    out.assign(ncell * 5, 0.0);
+    return 0;
+  }
+  if (field == GCS_FIELD_MAXWELL_RATE) {
+    out.assign(ncell * 3, 0.0);
+    return 0;
+  }
+  if (field == GCS_FIELD_VERTEX) return planes(P_VERT, 3, ve);
+  if (field == GCS_FIELD_CELL_GEOMETRY) {
+    // gamera_no_cell_geometry: volume, centroid, face[3][2], cfl_length[3].
+    const int per = sizeof(gamera_no_cell_geometry) / sizeof(double);
+    out.assign(ncell * per, 0.0);
+    std::vector<double> tmp(ncell);
+    auto put = [&](int plane, int slot, int di, int dj, int dk) {
+      const Layout &LL = P.L;
+      std::vector<double> buf(LL.face_stride);
+      if (hipMemcpy(buf.data(), P.plane(plane) + face * LL.face_stride,
+                    LL.face_stride * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
+      for (size_t i = 0; i < ce[0]; ++i)
+        for (size_t j = 0; j < ce[1]; ++j)
+          for (size_t k = 0; k < ce[2]; ++k)
+            out[((i * ce[1] + j) * ce[2] + k) * per + slot] =
+                buf[((i + di) * LL.SJ + j + dj) * LL.SK + k + dk];
+      return 0;
+    };
+    if (put(P_VOL, 0, 0, 0, 0) != 0) return -1;
+    for (int e = 0; e < 3; ++e)
+      if (put(P_CEN + e, 1 + e, 0, 0, 0) != 0) return -1;
+    for (int d = 0; d < 3; ++d)
+      for (int s = 0; s < 2; ++s)
+        for (int q = 0; q < FACE_PLANES; ++q)
+          if (put(P_FACE + FACE_PLANES * d + q, 4 + (2 * d + s) * FACE_PLANES + q,
+                  s * (d == 0), s * (d == 1), s * (d == 2)) != 0)
+            return -1;
+    for (int d = 0; d < 3; ++d)
+      if (put(P_CFL + d, 4 + 6 * FACE_PLANES + d, 0, 0, 0) != 0) return -1;
+    return 0;
+  }
+  if (field >= GCS_FIELD_FACE_GEOMETRY && field < GCS_FIELD_FACE_GEOMETRY + 3) {
+    const int d = field - GCS_FIELD_FACE_GEOMETRY;
+    size_t fe[3];
+    face_extent(L, d, fe);
+    return planes(P_FACE + FACE_PLANES * d, FACE_PLANES, fe);
+  }
+  if (field >= GCS_FIELD_EDGE_GEOMETRY && field < GCS_FIELD_EDGE_GEOMETRY + 3) {
+    const int d = field - GCS_FIELD_EDGE_GEOMETRY;
+    size_t ee[3];
+    edge_extent(L, d, ee);
+    return planes(P_EDGE + EDGE_PLANES * d, 10, ee);
+  }
+  if (field == GCS_FIELD_BG_CELL_MAGNETIC) return planes(P_B0C, 3, ce);
+  if (field == GCS_FIELD_BG_CELL_FORCE) return planes(P_FORCE, 3, ce);
+  if (field >= GCS_FIELD_BG_FACE_MAGNETIC && field < GCS_FIELD_COUNT) {
+    const int group = (field - GCS_FIELD_BG_FACE_MAGNETIC) / 3;
+    const int d = (field - GCS_FIELD_BG_FACE_MAGNETIC) % 3;
+    size_t fe[3], ee[3];
+    face_extent(L, d, fe);
+    edge_extent(L, d, ee);
+    if (group == 0) return planes(P_B0F + 3 * d, 3, fe);
+    if (group == 1) return planes(P_B0FLUX + d, 1, fe);
+    return planes(P_B0E + 3 * d, 3, ee);
+  }
+  set_error("unknown field id");
+  return -1;
+}
+
+int download_valid(Problem &P, int face, int dir, std::vector<uint8_t> &out) {
+  const Layout &L = P.L;
+  size_t ee[3];
+  edge_extent(L, dir, ee);
+  std::vector<uint8_t> buf(L.face_stride);
+  GCS_HIP(hipMemcpy(buf.data(), P.byte_plane(Q_VALID + dir) + face * L.face_stride,
+                    L.face_stride, hipMemcpyDeviceToHost));
+  out.assign(ee[0] * ee[1] * ee[2], 0);
+  for (size_t i = 0; i < ee[0]; ++i)
+    for (size_t j = 0; j < ee[1]; ++j)
+      for (size_t k = 0; k < ee[2]; ++k)
+        out[(i * ee[1] + j) * ee[2] + k] = buf[(i * L.SJ + j) * L.SK + k];
+  return 0;
+}
+
+}  // namespace gcs
+
+using gcs::Problem;
+
+extern "C" {
+
+const char *gcs_last_error(void) { return gcs::last_error(); }
+
+size_t gcs_config_size(void) { return sizeof(gcs_problem_config); }
+
+Problem *gcs_problem_create(const gcs_problem_config *cfg) {
+  if (cfg == nullptr || gcs::check_config(*cfg) != 0) return nullptr;
+  Problem *P = new Problem();
+  P->cfg = *cfg;
+  P->L = gcs::make_layout((int)cfg->radial_cells, (int)cfg->angular_cells,
+                          (int)cfg->ghost);
+  const size_t nbytes = (size_t)gcs::PLANES * P->L.total * sizeof(double);
+  const size_t nflags = (size_t)gcs::BYTE_PLANES * P->L.total;
+  if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&P->planes, nbytes) != hipSuccess ||
+      hipMalloc(&P->bytes, nflags) != hipSuccess ||
+      hipMalloc(&P->scalars, sizeof(gcs::StepScalars)) != hipSuccess ||
+      hipMemset(P->planes, 0, nbytes) != hipSuccess ||
+      hipMemset(P->bytes, 0, nflags) != hipSuccess ||
+      hipEventCreate(&P->ev_start) != hipSuccess ||
+      hipEventCreate(&P->ev_stop) != hipSuccess) {
+    gcs::set_error("device allocation failed");
+    gcs::free_problem(P);
+    return nullptr;
+  }
+  gcs::make_wind(*P);
+  if (gcs::init_scalars(*P) != 0 || gcs::setup_problem(*P) != 0 ||
+      gcs::build_plans(*P) != 0) {
+    gcs::free_problem(P);
+    return nullptr;
+  }
+  return P;
+}
+
+void gcs_problem_destroy(Problem *P) { gcs::free_problem(P); }
+
+int gcs_problem_advance(Problem *P, long steps) {
+  if (P == nullptr || steps < 0) return -1;
+  for (long n = 0; n < steps; ++n)
+    if (gcs::enqueue_step(*P, false) != 0) return -1;
+  return gcs::check_failure(*P);
+}
+
+// One step with caller-chosen dt, predictor ratio and boundary times; the
+// clock is left unchanged (like refh_step_explicit / orc_step_explicit).
+int gcs_problem_step_explicit(Problem *P, double ratio, double dt, double time,
+                              double previous_time) {
+  if (P == nullptr || !(dt > 0.0) || !std::isfinite(ratio)) return -1;
+  gcs::StepScalars s;
+  if (gcs::read_scalars(*P, s) != 0) return -1;
+  s.dt = dt;
+  s.ratio = ratio;
+  s.time = time;
+  s.previous_time = previous_time;
+  if (hipMemcpy(P->scalars, &s, sizeof(s), hipMemcpyHostToDevice) != hipSuccess) return -1;
+  if (gcs::enqueue_step(*P, true) != 0) return -1;
+  return gcs::check_failure(*P);
+}
+
+void gcs_problem_clock(Problem *P, double out[4]) {
+  gcs::StepScalars s;
+  if (P == nullptr || gcs::read_scalars(*P, s) != 0) return;
+  out[0] = s.time;
+  out[1] = s.dt;
+  out[2] = s.previous_dt;
+  out[3] = (double)s.steps;
+}
+
+int gcs_problem_field(Problem *P, int face, int field, double **data,
+                      size_t *count) {
+  if (P == nullptr || face < 0 || face >= 6 || field < 0 ||
+      field >= GCS_FIELD_COUNT || data == nullptr || count == nullptr)
+    return -1;
+  if (hipStreamSynchronize(P->stream) != hipSuccess) return -1;
+  if (P->host_dirty) {
+    P->host_fields.assign(6 * GCS_FIELD_COUNT, {});
+    P->host_valid.assign(18, {});
+    P->host_dirty = false;
+  }
+  std::vector<double> &v = P->host_fields[face * GCS_FIELD_COUNT + field];
+  if (v.empty() && gcs::download_field(*P, face, field, v) != 0) return -1;
+  *data = v.data();
+  *count = v.size();
+  return 0;
+}
+
+int gcs_problem_edge_valid(Problem *P, int face, int dir, unsigned char **data,
+                           size_t *count) {
+  if (P == nullptr || face < 0 || face >= 6 || dir < 0 || dir > 2) return -1;
+  if (P->host_dirty) {
+    P->host_fields.assign(6 * GCS_FIELD_COUNT, {});
+    P->host_valid.assign(18, {});
+    P->host_dirty = false;
+  }
+  std::vector<uint8_t> &v = P->host_valid[face * 3 + dir];
+  if (v.empty() && gcs::download_valid(*P, face, dir, v) != 0) return -1;
+  *data = v.data();
+  *count = v.size();
+  return 0;
+}
+
+// Emergency-diffusion counters of one cube face (count, max speed).
+void gcs_problem_counters(Problem *P, int face, double out[2]) {
+  gcs::StepScalars s;
+  if (P == nullptr || gcs::read_scalars(*P, s) != 0) return;
+  out[0] = (double)s.emergency_count[face];
+  double speed;
+  std::memcpy(&speed, &s.emergency_speed_bits[face], sizeof(speed));
+  out[1] = speed;
+}
+
+}  // extern "C"

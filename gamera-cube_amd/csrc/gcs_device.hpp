@@ -1,0 +1,128 @@
+// Host-side owner of one device-resident six-face problem.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gamera_cs_problem.h"
+#include "gcs_layout.hpp"
+#include "gcs_planes.hpp"
+
+namespace gcs {
+
+#define GCS_HIP(call)                                                      \
+  do {                                                                     \
+    hipError_t err_ = (call);                                              \
+    if (err_ != hipSuccess) {                                              \
+      gcs::set_error(std::string(#call) + ": " + hipGetErrorString(err_)); \
+      return -1;                                                           \
+    }                                                                      \
+  } while (0)
+
+void set_error(const std::string &message);
+const char *last_error();
+
+// Scalars the step kernels read from device memory, so that a captured step
+// graph replays with the dt the previous step's CFL reduction chose.
+struct StepScalars {
+  double dt, ratio, time, previous_time, previous_dt;
+  unsigned long long cfl_min_bits; // min local CFL dt (IEEE bits, > 0)
+  long long steps;
+  unsigned int failed;     // nonzero: a kernel hit a reference failure path
+  unsigned int fail_stage; // first failing stage (kernel id)
+  int fail_index;          // flat index of the first failing element
+  int pad;
+  unsigned long long emergency_count[6];
+  unsigned long long emergency_speed_bits[6];  // max, as IEEE bits (>= 0)
+};
+
+// Solar-wind series for device sampling (solar_wind.h:24-35).
+struct WindSeries {
+  int count, linear;
+  double time[GCS_MAX_WIND_SAMPLES];
+  double state[GCS_MAX_WIND_SAMPLES][8];  // rho, p, v[3], b[3]
+  double reference[3];
+  double time_offset, by, bz;
+};
+
+// Halo plans on the device (built on the host from the cube geometry).
+struct HaloCell {        // tangential ghost cell: 2 x 2 donor block
+  int target;            // flat index (layout.at) of the receiving cell
+  int donor;             // flat index of donor (p, q) = (0, 0) of the block
+  int nearest;           // flat index of the nearest-donor fallback
+  int pad;
+  double w[4];           // wx[p] * wy[q] for (0,0), (0,1), (1,0), (1,1)
+};
+struct HaloFace {        // ghost magnetic face
+  int target;            // flat index of the face
+  int donor;             // flat index of the donor block corner cell
+  int dir;
+  int pad;
+  double w[4];
+};
+struct SeamPair {        // one shared physical face seen from two cube faces
+  int a, b;              // flat face indices
+  int dir_a, dir_b;
+  double sign;           // orientation of b relative to a
+};
+struct EdgeMember {
+  int index;             // flat edge index
+  int dir;
+  double sign;
+};
+
+struct Problem {
+  gcs_problem_config cfg;
+  Layout L;
+  double *planes = nullptr;        // PLANES * L.total doubles
+  uint8_t *bytes = nullptr;        // BYTE_PLANES * L.total
+  StepScalars *scalars = nullptr;  // device
+  int level = 0;                   // which U/B/PHI level is current
+  long steps = 0;
+  double time = 0.0, dt = 0.0, previous_dt = 0.0;
+  // plans
+  HaloCell *halo_cells = nullptr;
+  int n_halo_cells = 0;
+  HaloFace *halo_faces = nullptr;
+  int n_halo_faces = 0;
+  SeamPair *seams = nullptr;
+  int n_seams = 0;
+  EdgeMember *edge_members = nullptr;
+  int *edge_groups = nullptr;      // CSR offsets, n_edge_groups + 1
+  int n_edge_groups = 0;
+  int n_edge_members = 0;
+  // host mirrors of reference-layout arrays (allocated on first request)
+  std::vector<std::vector<double>> host_fields;
+  std::vector<std::vector<uint8_t>> host_valid;
+  bool host_dirty = true;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  std::vector<double> kernel_ms;   // per-stage timing (profiling mode)
+  int *seam_cells = nullptr;       // active cells with a seam face
+  int n_seam_cells = 0;
+  WindSeries wind_series;
+  bool earth = true;               // Earth radial boundary active
+  bool fidelity = true;            // reproduce ghost values of the storage
+
+  double *plane(int p) const { return planes + (long)p * L.total; }
+  uint8_t *byte_plane(int q) const { return bytes + (long)q * L.total; }
+};
+
+// Setup: geometry, background, initial state and plans.
+int setup_problem(Problem &P);
+int build_plans(Problem &P);
+// One step of the driver loop (dt rule + six-face step + clock) on
+// P.stream; with explicit_dt the step uses the dt, ratio and times already
+// in the device scalars and leaves the clock alone.
+int enqueue_step(Problem &P, bool explicit_dt);
+// Copy one reference-layout field of one cube face to the host.
+int download_field(Problem &P, int face, int field, std::vector<double> &out);
+int download_valid(Problem &P, int face, int dir, std::vector<uint8_t> &out);
+int upload_field(Problem &P, int face, int field, const double *data,
+                 size_t count);
+
+}  // namespace gcs

@@ -1,0 +1,1155 @@
+// The six-face time step on the device.
+//
+// One step is the reference's gamera_cs_advance_six_with_background_planned
+// (cubedsphere/cubedsphere_step.c:13-154) with the Earth callbacks
+// (cubedsphere_earth_boundary.c:212-231, :259-266, :446-451), driven by the
+// serial driver's dt rule (test_cubedsphere_dipole_background.c:593-699).
+// Its stages run as a fixed sequence of kernels on one stream; dt, the
+// predictor ratio and the clock live in device memory, so the sequence can be
+// captured once as a graph and replayed.
+//
+// Differences in structure from the reference, none of which changes a
+// computed value:
+//  * the two time levels of U, cell B and face flux swap roles every step
+//    instead of copying current into old (nonorthogonal_step.c:361-377);
+//  * the update fuses stress divergence, Reynolds update, CT, B recovery and
+//    the Boris correction per active cell; a face shared by two cells gets
+//    the same CT value from both, and only one writes it;
+//  * after the final seam-face average only cells touching a seam face are
+//    recovered again: every other cell's six fluxes are unchanged;
+//  * halo stencils carry only their 2 x 2 nonzero-weight donors, summed in
+//    the reference's p-major / q-minor order (the other 60 terms are exact
+//    zeros for finite donors, cubedsphere_halo_fastpath.h:1-24).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+#include "gcs_device.hpp"
+#include "gcs_math.hpp"
+
+namespace gcs {
+
+namespace {
+
+struct Options {  // validated on the host (cubedsphere_step.c:23-47)
+  double gamma, rho_floor, p_floor, pdm, hogs_h, hogs_m, c;
+  double emf_rho_floor, emf_pdm, emf_diff, emf_cfl;
+  int boris, hogs;
+  // Earth boundary
+  double wall_rho_floor, wall_p_floor, wall_cfl;
+  int strict_wall;
+};
+
+struct K {
+  Layout L;
+  double *pl;
+  uint8_t *by;
+  StepScalars *sc;
+  int cur, old;  // time-level slots of this step
+  Options o;
+  __device__ double *p(int q) const { return pl + (long)q * L.total; }
+  __device__ const uint8_t *b(int q) const { return by + (long)q * L.total; }
+  __device__ void fail(unsigned stage, long index) const {
+    if (atomicOr(&sc->failed, 1u) == 0u) {
+      sc->fail_stage = stage;
+      sc->fail_index = (int)index;
+    }
+  }
+};
+
+// (f, i, j, k) of thread t in the box lo <= (i, j, k) < hi on every face.
+GCS_DEV bool box(long t, const int lo[3], const int hi[3], int &f, int &i,
+                 int &j, int &k) {
+  const int e0 = hi[0] - lo[0], e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
+  const long per = (long)e0 * e1 * e2;
+  if (t >= 6 * per) return false;
+  f = (int)(t / per);
+  long r = t - f * per;
+  i = (int)(r / ((long)e1 * e2));
+  r -= (long)i * e1 * e2;
+  j = (int)(r / e2);
+  k = (int)(r - (long)j * e2);
+  i += lo[0];
+  j += lo[1];
+  k += lo[2];
+  return true;
+}
+
+GCS_DEV long tid() { return blockIdx.x * (long)blockDim.x + threadIdx.x; }
+
+// ---- dt rule ---------------------------------------------------------------
+
+// Local CFL dt of every active cell, min-reduced (nonorthogonal_step.c:
+// 449-525). Positive doubles order like their bit patterns.
+__global__ void k_cfl(K c) {
+  const Layout &L = c.L;
+  const int lo[3] = {L.ghost, L.ghost, L.ghost};
+  const int hi[3] = {L.ghost + L.nr, L.ghost + L.n, L.ghost + L.n};
+  int f, i, j, k;
+  double best = DBL_MAX;
+  if (box(tid(), lo, hi, f, i, j, k)) {
+    const long at = L.at(f, i, j, k);
+    double u[5];
+    for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
+    Prim w;
+    if (!cons_to_prim(u, c.o.gamma, c.o.rho_floor, c.o.p_floor, w)) {
+      c.fail(1, at);
+    } else {
+      const double cs = sqrt(c.o.gamma * w.p / w.rho);
+      double m[3];
+      for (int d = 0; d < 3; ++d) m[d] = c.p(plane_b(c.cur, d))[at] + c.p(P_B0C + d)[at];
+      double va = sqrt(dot3(m, m)) / sqrt(w.rho);
+      if (c.o.boris) va = c.o.c * va / hypot(c.o.c, va);
+      const double speed = sqrt(dot3(w.v, w.v)) + hypot(cs, va);
+      double len = c.p(P_CFL)[at];
+      len = fmin(len, c.p(P_CFL + 1)[at]);
+      len = fmin(len, c.p(P_CFL + 2)[at]);
+      const double cand = c.o.emf_cfl * len / speed;
+      if (!isfinite(cand) || cand <= 0.0) c.fail(2, at);
+      else best = cand;
+    }
+  }
+  // wavefront min, then one atomic per wave
+  unsigned long long bits = __double_as_longlong(best);
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long other = __shfl_down(bits, off, 64);
+    bits = other < bits ? other : bits;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMin(&c.sc->cfl_min_bits, bits);
+}
+
+// Driver dt rule (test_cubedsphere_dipole_background.c:593-616, :669-670).
+__global__ void k_dt(K c, double fraction) {
+  StepScalars *s = c.sc;
+  const double limit = __longlong_as_double(s->cfl_min_bits);
+  s->cfl_min_bits = __double_as_longlong(DBL_MAX);
+  if (s->steps == 0) {
+    s->dt = fraction * limit;
+    s->ratio = 0.0;
+    s->previous_time = s->time;
+  } else {
+    s->dt = fmin(fraction * limit, 1.25 * s->previous_dt);
+    s->ratio = 0.5 * s->dt / s->previous_dt;
+    s->previous_time = s->time - s->previous_dt;
+  }
+}
+
+__global__ void k_clock(K c) {
+  StepScalars *s = c.sc;
+  s->time += s->dt;
+  s->previous_dt = s->dt;
+  s->steps += 1;
+}
+
+// ---- seams -----------------------------------------------------------------
+
+// cubedsphere_interface.c:7-81 for the face flux of the given levels.
+__global__ void k_sync_phi(K c, const SeamPair *pairs, int n, int both) {
+  const long t = tid();
+  if (t >= n) return;
+  const SeamPair sp = pairs[t];
+  for (int lv = 0; lv < (both ? 2 : 1); ++lv) {
+    const int level = lv == 0 ? c.cur : c.old;
+    double *a = c.p(plane_phi(level, sp.dir_a)) + sp.a;
+    double *b = c.p(plane_phi(level, sp.dir_b)) + sp.b;
+    const double common = 0.5 * (*a + sp.sign * *b);
+    *a = common;
+    *b = sp.sign * common;
+  }
+}
+
+// Same reconciliation of the fluid (5) and Maxwell (3) face fluxes.
+__global__ void k_sync_flux(K c, const SeamPair *pairs, int n) {
+  const long t = tid();
+  if (t >= n) return;
+  const SeamPair sp = pairs[t];
+  for (int q = 0; q < 8; ++q) {
+    double *a = q < 5 ? c.p(P_FLUID + 5 * sp.dir_a + q) + sp.a
+                      : c.p(P_MAXWELL + 3 * sp.dir_a + q - 5) + sp.a;
+    double *b = q < 5 ? c.p(P_FLUID + 5 * sp.dir_b + q) + sp.b
+                      : c.p(P_MAXWELL + 3 * sp.dir_b + q - 5) + sp.b;
+    const double common = 0.5 * (*a + sp.sign * *b);
+    *a = common;
+    *b = sp.sign * common;
+  }
+}
+
+// cubedsphere_interface.c:217-242: signed mean over each coincident group.
+__global__ void k_sync_edges(K c, const int *groups, const EdgeMember *m,
+                             int n) {
+  const long g = tid();
+  if (g >= n) return;
+  const int a = groups[g], b = groups[g + 1];
+  double common = 0.0;
+  for (int t = a; t < b; ++t) common += m[t].sign * c.p(P_EMF + m[t].dir)[m[t].index];
+  common /= (double)(b - a);
+  for (int t = a; t < b; ++t) c.p(P_EMF + m[t].dir)[m[t].index] = m[t].sign * common;
+}
+
+// ---- ghost fill --------------------------------------------------------------
+
+// cubedsphere_halo.c:103-115
+GCS_DEV bool halo_valid(const double u[5]) {
+  const double rho = u[0];
+  if (!isfinite(rho) || rho <= 0.0) return false;
+  double m2 = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    if (!isfinite(u[1 + d])) return false;
+    m2 += u[1 + d] * u[1 + d];
+  }
+  return isfinite(u[4]) && u[4] > 0.5 * m2 / rho;
+}
+
+// 2 x 2 donor block sum in the reference's p-major / q-minor order.
+GCS_DEV double block_sum(const double *plane, int donor, int sk,
+                         const double w[4]) {
+  double s = 0.0;
+  s += w[0] * plane[donor];
+  s += w[1] * plane[donor + 1];
+  s += w[2] * plane[donor + sk];
+  s += w[3] * plane[donor + sk + 1];
+  return s;
+}
+
+// cubedsphere_halo.c:582-632: tangential ghost cells, both levels.
+__global__ void k_halo_cells(K c, const HaloCell *e, int n) {
+  const long t = tid();
+  if (t >= n) return;
+  const HaloCell h = e[t];
+  const int sk = c.L.SK;
+  double cu[5], ou[5], cb[3], ob[3];
+  for (int q = 0; q < 5; ++q) {
+    cu[q] = block_sum(c.p(plane_u(c.cur, q)), h.donor, sk, h.w);
+    ou[q] = block_sum(c.p(plane_u(c.old, q)), h.donor, sk, h.w);
+  }
+  for (int d = 0; d < 3; ++d) {
+    cb[d] = block_sum(c.p(plane_b(c.cur, d)), h.donor, sk, h.w);
+    ob[d] = block_sum(c.p(plane_b(c.old, d)), h.donor, sk, h.w);
+  }
+  int src = -1;
+  if (!halo_valid(cu) || !halo_valid(ou)) src = h.nearest;
+  for (int q = 0; q < 5; ++q) {
+    c.p(plane_u(c.cur, q))[h.target] = src < 0 ? cu[q] : c.p(plane_u(c.cur, q))[src];
+    c.p(plane_u(c.old, q))[h.target] = src < 0 ? ou[q] : c.p(plane_u(c.old, q))[src];
+  }
+  for (int d = 0; d < 3; ++d) {
+    c.p(plane_b(c.cur, d))[h.target] = src < 0 ? cb[d] : c.p(plane_b(c.cur, d))[src];
+    c.p(plane_b(c.old, d))[h.target] = src < 0 ? ob[d] : c.p(plane_b(c.old, d))[src];
+  }
+}
+
+// cubedsphere_halo.c:651-686: ghost face flux = interpolated donor B . area.
+__global__ void k_halo_faces(K c, const HaloFace *e, int n) {
+  const long t = tid();
+  if (t >= n) return;
+  const HaloFace h = e[t];
+  const int sk = c.L.SK;
+  const int base = P_FACE + FACE_PLANES * h.dir + FO_AV;
+  double cf = 0.0, of = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const double av = c.p(base + d)[h.target];
+    cf += block_sum(c.p(plane_b(c.cur, d)), h.donor, sk, h.w) * av;
+    of += block_sum(c.p(plane_b(c.old, d)), h.donor, sk, h.w) * av;
+  }
+  c.p(plane_phi(c.cur, h.dir))[h.target] = cf;
+  c.p(plane_phi(c.old, h.dir))[h.target] = of;
+}
+
+// cubedsphere_halo.c:217-245 (plain problems without the Earth boundary).
+__global__ void k_radial_outflow(K c) {
+  const Layout &L = c.L;
+  const int lo[3] = {0, 0, 0};
+  const int hi[3] = {2 * L.ghost, L.NJ, L.NK};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const int ti = i < L.ghost ? i : L.ghost + L.nr + (i - L.ghost);
+  const int si = i < L.ghost ? L.ghost : L.ghost + L.nr - 1;
+  const long t = L.at(f, ti, j, k), s = L.at(f, si, j, k);
+  for (int lv = 0; lv < 2; ++lv) {
+    const int level = lv == 0 ? c.cur : c.old;
+    for (int q = 0; q < 5; ++q) c.p(plane_u(level, q))[t] = c.p(plane_u(level, q))[s];
+    for (int d = 0; d < 3; ++d) c.p(plane_b(level, d))[t] = c.p(plane_b(level, d))[s];
+  }
+}
+
+// ---- solar wind (nonorthogonal/src/solar_wind.c) -------------------------------
+
+GCS_DEV bool wind_ok(const double s[8]) {
+  if (!isfinite(s[0]) || !isfinite(s[1]) || s[0] <= 0.0 || s[1] <= 0.0) return false;
+  for (int q = 2; q < 8; ++q)
+    if (!isfinite(s[q])) return false;
+  return fabs(s[2]) > DBL_MIN;
+}
+
+// :206-232 (with :173-204)
+GCS_DEV bool wind_at(const WindSeries &w, double t, double out[8]) {
+  if (!isfinite(t)) return false;
+  if (w.count == 1 || t <= w.time[0]) {
+    for (int q = 0; q < 8; ++q) out[q] = w.state[0][q];
+    return true;
+  }
+  if (t >= w.time[w.count - 1]) {
+    for (int q = 0; q < 8; ++q) out[q] = w.state[w.count - 1][q];
+    return true;
+  }
+  int lo = 0, hi = w.count - 1;
+  while (hi - lo > 1) {
+    const int mid = lo + (hi - lo) / 2;
+    if (w.time[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  if (!w.linear) {
+    for (int q = 0; q < 8; ++q) out[q] = w.state[lo][q];
+    return true;
+  }
+  const double fr = (t - w.time[lo]) / (w.time[lo + 1] - w.time[lo]);
+  for (int q = 0; q < 8; ++q)
+    out[q] = w.state[lo][q] + fr * (w.state[lo + 1][q] - w.state[lo][q]);
+  return wind_ok(out);
+}
+
+// :234-275: planar-front ballistic delay.
+GCS_DEV bool wind_sample(const WindSeries &w, const double x[3], double t,
+                         double out[8]) {
+  const double mt = t + w.time_offset;
+  double cur[8];
+  if (!wind_at(w, mt, cur)) return false;
+  const double c2 = w.by * w.by + w.bz * w.bz;
+  const double factor = cur[2] / (1.0 + c2);
+  const double fv[3] = {factor, -factor * w.by, -factor * w.bz};
+  double s2 = 0.0, pr = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    if (!isfinite(x[d])) return false;
+    s2 += fv[d] * fv[d];
+    pr += (x[d] - w.reference[d]) * fv[d];
+  }
+  if (!(s2 > DBL_MIN) || !isfinite(s2)) return false;
+  return wind_at(w, mt - pr / s2, out);
+}
+
+// :277-311: upstream-to-tail weight.
+GCS_DEV bool wind_weight(const WindSeries &w, const double x[3], double t,
+                         double &weight) {
+  double s[8];
+  if (!wind_sample(w, x, t, s)) return false;
+  double r2 = 0.0, v2 = 0.0, cosine = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    r2 += x[d] * x[d];
+    v2 += s[2 + d] * s[2 + d];
+    cosine -= x[d] * s[2 + d];
+  }
+  if (!(r2 > DBL_MIN) || !(v2 > DBL_MIN)) return false;
+  cosine /= sqrt(r2 * v2);
+  cosine = fmax(-1.0, fmin(1.0, cosine));
+  const double angle = acos(cosine);
+  const double half_pi = 0.5 * acos(-1.0);
+  if (angle <= half_pi) weight = 1.0;
+  else if (angle >= 2.0 * half_pi) weight = 0.0;
+  else {
+    const double s1 = (angle - half_pi) / half_pi;
+    weight = exp(1.0 - 1.0 / (1.0 - s1 * s1));
+  }
+  return true;
+}
+
+GCS_DEV bool sample_wind(const WindSeries &w, const double x[3], double t,
+                         double s[8], double &weight) {
+  return wind_sample(w, x, t, s) && wind_weight(w, x, t, weight);
+}
+
+// ---- Earth radial boundary -------------------------------------------------
+
+// cubedsphere_earth_boundary.c:27-112: radial ghost cells of both levels.
+__global__ void k_earth_cells(K c, WindSeries w) {
+  const Layout &L = c.L;
+  const int lo[3] = {0, 0, 0};
+  const int hi[3] = {2 * L.ghost, L.NJ, L.NK};
+  int f, r, j, k;
+  if (!box(tid(), lo, hi, f, r, j, k)) return;
+  const bool inner = r < L.ghost;
+  const int outer = L.ghost + L.nr;
+  const int i = inner ? r : outer + (r - L.ghost);
+  const int si = inner ? 2 * L.ghost - 1 - i : outer - 1;
+  const long t = L.at(f, i, j, k), s = L.at(f, si, j, k);
+  const double x[3] = {c.p(P_CEN)[t], c.p(P_CEN + 1)[t], c.p(P_CEN + 2)[t]};
+  const double rad = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  for (int lv = 0; lv < 2; ++lv) {
+    const int level = lv == 0 ? c.cur : c.old;
+    const double time = lv == 0 ? c.sc->time : c.sc->previous_time;
+    double u[5];
+    for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(level, q))[s];
+    Prim pw;
+    if (!cons_to_prim(u, c.o.gamma, c.o.wall_rho_floor, c.o.wall_p_floor, pw) ||
+        !(rad > 0.0)) {
+      c.fail(11, t);
+      return;
+    }
+    if (inner) {
+      if (c.o.strict_wall) {
+        pw.v[0] = pw.v[1] = pw.v[2] = 0.0;
+      } else {
+        double vn = 0.0;
+        for (int d = 0; d < 3; ++d) vn += pw.v[d] * x[d] / rad;
+        for (int d = 0; d < 3; ++d) pw.v[d] -= 2.0 * vn * x[d] / rad;
+      }
+    } else {
+      double sw[8], wt;
+      if (!sample_wind(w, x, time, sw, wt)) {
+        c.fail(12, t);
+        return;
+      }
+      double vn = 0.0;
+      for (int d = 0; d < 3; ++d) vn += pw.v[d] * x[d] / rad;
+      const double cs = sqrt(c.o.gamma * pw.p / pw.rho);
+      if (vn < cs)
+        for (int d = 0; d < 3; ++d) pw.v[d] += (cs - vn) * x[d] / rad;
+      pw.rho = wt * sw[0] + (1.0 - wt) * pw.rho;
+      pw.p = wt * sw[1] + (1.0 - wt) * pw.p;
+      for (int d = 0; d < 3; ++d) pw.v[d] = wt * sw[2 + d] + (1.0 - wt) * pw.v[d];
+    }
+    if (!prim_to_cons(pw, c.o.gamma, c.o.wall_rho_floor, c.o.wall_p_floor, u)) {
+      c.fail(13, t);
+      return;
+    }
+    for (int q = 0; q < 5; ++q) c.p(plane_u(level, q))[t] = u[q];
+  }
+}
+
+// cubedsphere_earth_boundary.c:114-176: radial ghost faces, both levels.
+// Thread box: radial slot r < ghost is inner face r; otherwise faces beyond
+// the outer wall (r - ghost of them).
+__global__ void k_earth_faces(K c, int d, WindSeries w) {
+  const Layout &L = c.L;
+  const int outer = L.ghost + L.nr;
+  const int n_out = (L.NI + (d == 0)) - (outer + (d == 0));
+  const int lo[3] = {0, 0, 0};
+  const int hi[3] = {L.ghost + n_out, L.NJ + (d == 1), L.NK + (d == 2)};
+  int f, r, j, k;
+  if (!box(tid(), lo, hi, f, r, j, k)) return;
+  const bool inner = r < L.ghost;
+  const int i = inner ? r : outer + (d == 0) + (r - L.ghost);
+  const long t = L.at(f, i, j, k);
+  const int fb = P_FACE + FACE_PLANES * d;
+  for (int lv = 0; lv < 2; ++lv) {
+    const int level = lv == 0 ? c.cur : c.old;
+    const double time = lv == 0 ? c.sc->time : c.sc->previous_time;
+    double *flux = c.p(plane_phi(level, d));
+    if (inner) {
+      const long s = L.at(f, L.ghost, j, k);
+      const double sa = c.p(fb + FO_AREA)[s];
+      if (!(sa > 0.0)) {
+        c.fail(14, t);
+        return;
+      }
+      flux[t] = flux[s] * c.p(fb + FO_AREA)[t] / sa;
+    } else {
+      const double x[3] = {c.p(fb + FO_CEN)[t], c.p(fb + FO_CEN + 1)[t],
+                           c.p(fb + FO_CEN + 2)[t]};
+      double sw[8], wt;
+      if (!sample_wind(w, x, time, sw, wt)) {
+        c.fail(15, t);
+        return;
+      }
+      double wf = -c.p(P_B0FLUX + d)[t];
+      for (int e = 0; e < 3; ++e) wf += sw[5 + e] * c.p(fb + FO_AV + e)[t];
+      const long s = L.at(f, d == 0 ? outer : outer - 1, j, k);
+      flux[t] = wt * wf + (1.0 - wt) * flux[s];
+    }
+  }
+}
+
+// ---- recovery ----------------------------------------------------------------
+
+GCS_DEV bool recover_at(const K &c, long at, int phi_level_plane,
+                        double out[3]) {
+  const Layout &L = c.L;
+  const int piv = c.b(Q_PIV)[at];
+  if (piv == 0xff) return false;
+  const long step[3] = {L.di(), (long)L.SK, 1};
+  double flux[6], fc[6][3], cen[3], rec[9];
+  for (int d = 0; d < 3; ++d) {
+    const double *ph = c.p(phi_level_plane + d);
+    flux[2 * d] = ph[at];
+    flux[2 * d + 1] = ph[at + step[d]];
+    for (int e = 0; e < 3; ++e) {
+      const double *fcp = c.p(P_FACE + FACE_PLANES * d + FO_CEN + e);
+      fc[2 * d][e] = fcp[at];
+      fc[2 * d + 1][e] = fcp[at + step[d]];
+    }
+  }
+  for (int e = 0; e < 3; ++e) cen[e] = c.p(P_CEN + e)[at];
+  for (int r = 0; r < 9; ++r) rec[r] = c.p(P_REC + r)[at];
+  return recover_cell(flux, fc, cen, c.p(P_VOL)[at], rec, piv, out);
+}
+
+// Recover every cell of a level (nonorthogonal_step.c:310-359); with
+// inner_copy the cells inside the wall take the first shell's value
+// (cubedsphere_earth_boundary.c:235-245).
+__global__ void k_recover(K c, int phi_plane, int b_plane, int inner_copy,
+                          int ghost_only) {
+  const Layout &L = c.L;
+  const int lo[3] = {0, 0, 0};
+  const int hi[3] = {L.NI, L.NJ, L.NK};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  if (ghost_only && i >= L.ghost && i < L.ghost + L.nr && j >= L.ghost &&
+      j < L.ghost + L.n && k >= L.ghost && k < L.ghost + L.n)
+    return;
+  const long at = L.at(f, i, j, k);
+  double b[3];
+  if (!recover_at(c, at, phi_plane, b)) {
+    c.fail(21, at);
+    return;
+  }
+  if (inner_copy && i < L.ghost && !recover_at(c, L.at(f, L.ghost, j, k), phi_plane, b)) {
+    c.fail(22, at);
+    return;
+  }
+  for (int d = 0; d < 3; ++d) c.p(b_plane + d)[at] = b[d];
+}
+
+// Recover a listed set of cells (active cells with a seam face).
+__global__ void k_recover_list(K c, const int *cells, int n, int phi_plane,
+                               int b_plane) {
+  const long t = tid();
+  if (t >= n) return;
+  const long at = cells[t];
+  double b[3];
+  if (!recover_at(c, at, phi_plane, b)) {
+    c.fail(23, at);
+    return;
+  }
+  for (int d = 0; d < 3; ++d) c.p(b_plane + d)[at] = b[d];
+}
+
+// ---- predictor -----------------------------------------------------------------
+
+// nonorthogonal_step.c:379-419: primitive AB predictor of every cell,
+// face-flux predictor of every face and B of the predicted fluxes.
+__global__ void k_predict(K c) {
+  const Layout &L = c.L;
+  const int lo[3] = {0, 0, 0};
+  const int hi[3] = {L.NI + 1, L.NJ + 1, L.NK + 1};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  const double r = c.sc->ratio;
+  const long step[3] = {L.di(), (long)L.SK, 1};
+  const bool in[3] = {i < L.NI, j < L.NJ, k < L.NK};
+  for (int d = 0; d < 3; ++d) {
+    const bool face = (d == 0 || in[0]) && (d == 1 || in[1]) && (d == 2 || in[2]);
+    if (!face) continue;
+    const double cf = c.p(plane_phi(c.cur, d))[at];
+    const double of = c.p(plane_phi(c.old, d))[at];
+    c.p(P_PHIPRED + d)[at] = cf + r * (cf - of);
+  }
+  if (!(in[0] && in[1] && in[2])) return;
+  double uo[5], uc[5], up[5];
+  for (int q = 0; q < 5; ++q) {
+    uo[q] = c.p(plane_u(c.old, q))[at];
+    uc[q] = c.p(plane_u(c.cur, q))[at];
+  }
+  if (!predict_cell(uo, uc, r, c.o.gamma, c.o.rho_floor, c.o.p_floor, up)) {
+    c.fail(31, at);
+    return;
+  }
+  for (int q = 0; q < 5; ++q) c.p(P_UPRED + q)[at] = up[q];
+  // B of the predicted fluxes; the upper faces' predictions are recomputed
+  // here with the same expression rather than read back.
+  const int piv = c.b(Q_PIV)[at];
+  double flux[6], fc[6][3], cen[3], rec[9], b[3];
+  for (int d = 0; d < 3; ++d) {
+    const double *cp = c.p(plane_phi(c.cur, d)), *op = c.p(plane_phi(c.old, d));
+    for (int s = 0; s < 2; ++s) {
+      const long fa = at + s * step[d];
+      flux[2 * d + s] = cp[fa] + r * (cp[fa] - op[fa]);
+      for (int e = 0; e < 3; ++e)
+        fc[2 * d + s][e] = c.p(P_FACE + FACE_PLANES * d + FO_CEN + e)[fa];
+    }
+  }
+  for (int e = 0; e < 3; ++e) cen[e] = c.p(P_CEN + e)[at];
+  for (int q = 0; q < 9; ++q) rec[q] = c.p(P_REC + q)[at];
+  if (piv == 0xff || !recover_cell(flux, fc, cen, c.p(P_VOL)[at], rec, piv, b)) {
+    c.fail(32, at);
+    return;
+  }
+  for (int d = 0; d < 3; ++d) c.p(P_BPRED + d)[at] = b[d];
+}
+
+// ---- face fluxes -----------------------------------------------------------------
+
+// nonorthogonal_sweep.c:408-548 for faces of direction d bounding the
+// active box: 8-cell Up7/PDM of the predicted state, kinetic fluid and
+// Maxwell fluxes, HOGS, emergency diffusion, area scaling.
+__global__ void __launch_bounds__(128) k_face_flux(K c, int d) {
+  const Layout &L = c.L;
+  const int lo[3] = {L.ghost, L.ghost, L.ghost};
+  const int hi[3] = {L.ghost + L.nr + (d == 0), L.ghost + L.n + (d == 1),
+                     L.ghost + L.n + (d == 2)};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
+  const Options &o = c.o;
+  double vol[8];
+  Prim ws[8];
+  for (int n = 0; n < 8; ++n) {
+    const long cell = at + (n - 4) * st;
+    vol[n] = c.p(P_VOL)[cell];
+    double u[5];
+    for (int q = 0; q < 5; ++q) u[q] = c.p(P_UPRED + q)[cell];
+    if (!cons_to_prim(u, o.gamma, o.rho_floor, o.p_floor, ws[n])) {
+      c.fail(41, at);
+      return;
+    }
+  }
+  const double iv = central8(vol);
+  if (!isfinite(iv) || fabs(iv) <= DBL_MIN) {
+    c.fail(42, at);
+    return;
+  }
+  Prim w[2];
+  {
+    double q8[8];
+    for (int n = 0; n < 8; ++n) q8[n] = ws[n].rho;
+    up7_pdm(vol, iv, q8, o.pdm, w[0].rho, w[1].rho);
+    for (int e = 0; e < 3; ++e) {
+      for (int n = 0; n < 8; ++n) q8[n] = ws[n].v[e];
+      up7_pdm(vol, iv, q8, o.pdm, w[0].v[e], w[1].v[e]);
+    }
+    for (int n = 0; n < 8; ++n) q8[n] = ws[n].p;
+    up7_pdm(vol, iv, q8, o.pdm, w[0].p, w[1].p);
+  }
+  double bb[2][3];
+  for (int e = 0; e < 3; ++e) {
+    double q8[8];
+    for (int n = 0; n < 8; ++n) q8[n] = c.p(P_BPRED + e)[at + (n - 4) * st];
+    up7_pdm(vol, iv, q8, o.pdm, bb[0][e], bb[1][e]);
+  }
+  if (!valid_interface(w[0].rho, w[0].v, w[0].p) ||
+      !valid_interface(w[1].rho, w[1].v, w[1].p)) {
+    c.fail(43, at);
+    return;
+  }
+  const int fb = P_FACE + FACE_PLANES * d;
+  const double area = c.p(fb + FO_AREA)[at];
+  FaceFrame fr;
+  for (int e = 0; e < 3; ++e) {
+    fr.n[e] = c.p(fb + FO_N + e)[at];
+    fr.t1[e] = c.p(fb + FO_T1 + e)[at];
+    fr.t2[e] = c.p(fb + FO_T2 + e)[at];
+  }
+  FluidFlux fl;
+  fluid_flux(w, o.gamma, fr, fl);
+  const double bn = c.p(P_PHIPRED + d)[at] / area;
+  const double b0[3] = {c.p(P_B0F + 3 * d)[at], c.p(P_B0F + 3 * d + 1)[at],
+                        c.p(P_B0F + 3 * d + 2)[at]};
+  MaxwellFlux mx;
+  if (!maxwell_flux(w, bb, bn, fr.n, b0, c.p(P_B0N + d)[at], o.boris, o.c,
+                    fl.un, mx)) {
+    c.fail(44, at);
+    return;
+  }
+  if (o.hogs) hogs(fl, mx, o.hogs_h, o.hogs_m, o.boris, o.c);
+  if (o.hogs && o.boris) {
+    // nonorthogonal_flux.c:206-241
+    double s2 = 0.0;
+    s2 += dot3(w[0].v, w[0].v);
+    s2 += dot3(w[1].v, w[1].v);
+    if (sqrt(0.5 * s2) >= 1.5 * o.c) {
+      const long lc = at - st;
+      for (int q = 0; q < 5; ++q) {
+        const double lo_u = c.p(P_UPRED + q)[lc], up_u = c.p(P_UPRED + q)[at];
+        if (!isfinite(lo_u) || !isfinite(up_u)) {
+          c.fail(45, at);
+          return;
+        }
+        fl.f[q] -= o.c * (up_u - lo_u);
+      }
+      double t2 = 0.0;
+      for (int side = 0; side < 2; ++side)
+        for (int e = 0; e < 3; ++e) t2 += w[side].v[e] * w[side].v[e];
+      atomicAdd(&c.sc->emergency_count[f], 1ull);
+      atomicMax(&c.sc->emergency_speed_bits[f],
+                (unsigned long long)__double_as_longlong(sqrt(0.5 * t2)));
+    }
+  }
+  for (int q = 0; q < 5; ++q) c.p(P_FLUID + 5 * d + q)[at] = area * fl.f[q];
+  for (int e = 0; e < 3; ++e) c.p(P_MAXWELL + 3 * d + e)[at] = area * mx.m[e];
+}
+
+// nonorthogonal_step.c:35-71 on the inner wall face.
+__global__ void k_inner_trap(K c) {
+  const Layout &L = c.L;
+  const int lo[3] = {L.ghost, L.ghost, L.ghost};
+  const int hi[3] = {L.ghost + 1, L.ghost + L.n, L.ghost + L.n};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  double *m = c.p(P_FLUID + 0) + at;
+  double *e = c.p(P_FLUID + 4) + at;
+  if (*m > 0.0) {
+    *m = 0.0;
+    if (*e > 0.0) *e = 0.0;
+  }
+}
+
+// ---- edge EMF --------------------------------------------------------------------
+
+// nonorthogonal_sweep.c:223-259: central8 cell velocity at faces of
+// direction d, over the box the EMF sweep reads. A failure leaves NaN,
+// which the EMF's finiteness test turns into the reference's error.
+__global__ void k_face_velocity(K c, int d) {
+  const Layout &L = c.L;
+  int lo[3], hi[3];
+  const int ext[3] = {L.NI, L.NJ, L.NK};
+  const int act[3] = {L.nr, L.n, L.n};
+  for (int a = 0; a < 3; ++a) {
+    if (a == d) {
+      lo[a] = L.ghost;
+      hi[a] = L.ghost + act[a] + 1;
+    } else {
+      lo[a] = L.ghost - 4;
+      hi[a] = min(L.ghost + act[a] + 4, ext[a]);
+    }
+  }
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
+  double vol[8], wv[3][8];
+  for (int n = 0; n < 8; ++n) {
+    const long cell = at + (n - 4) * st;
+    vol[n] = c.p(P_VOL)[cell];
+    const double rho = fmax(c.p(P_UPRED)[cell], c.o.emf_rho_floor);
+    for (int e = 0; e < 3; ++e) wv[e][n] = vol[n] * c.p(P_UPRED + 1 + e)[cell] / rho;
+  }
+  const double iv = central8(vol);
+  const bool ok = isfinite(iv) && fabs(iv) > DBL_MIN;
+  for (int e = 0; e < 3; ++e)
+    c.p(P_FVEL + 3 * d + e)[at] = ok ? central8(wv[e]) / iv : NAN;
+}
+
+// nonorthogonal_sweep.c:624-752 for edges of direction e bounding the
+// active box.
+__global__ void __launch_bounds__(128) k_edge_emf(K c, int e) {
+  const Layout &L = c.L;
+  const int t1 = (e + 1) % 3, t2 = (e + 2) % 3;
+  const int lo[3] = {L.ghost, L.ghost, L.ghost};
+  const int hi[3] = {L.ghost + L.nr + (e != 0), L.ghost + L.n + (e != 1),
+                     L.ghost + L.n + (e != 2)};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  if (!c.b(Q_VALID + e)[at] || !c.b(Q_EBOK + e)[at]) {
+    c.fail(51, at);
+    return;
+  }
+  const long step[3] = {L.di(), (long)L.SK, 1};
+  const Options &o = c.o;
+  const int eb = P_EDGE + EDGE_PLANES * e;
+  // Edge velocity: area-weighted central8 of t1-face velocities along t2.
+  double a1[8], v[3];
+  {
+    double wv[3][8];
+    for (int n = 0; n < 8; ++n) {
+      const long fa = at + (n - 4) * step[t2];
+      a1[n] = c.p(P_FACE + FACE_PLANES * t1 + FO_AREA)[fa];
+      for (int m = 0; m < 3; ++m) wv[m][n] = a1[n] * c.p(P_FVEL + 3 * t1 + m)[fa];
+    }
+    const double ea = central8(a1);
+    if (!isfinite(ea) || fabs(ea) <= DBL_MIN) {
+      c.fail(52, at);
+      return;
+    }
+    for (int m = 0; m < 3; ++m) v[m] = central8(wv[m]) / ea;
+  }
+  double et1[3], et2[3];
+  for (int m = 0; m < 3; ++m) {
+    et1[m] = c.p(eb + EO_T1 + m)[at];
+    et2[m] = c.p(eb + EO_T2 + m)[at];
+  }
+  const double v1 = dot3(v, et1), v2 = dot3(v, et2);
+  // Transverse face-normal B: Up7/PDM of flux / area along the other axis.
+  double fl, fr, sl, sr;
+  {
+    double fld[8];
+    const double *ph = c.p(P_PHIPRED + t1);
+    for (int n = 0; n < 8; ++n) fld[n] = ph[at + (n - 4) * step[t2]] / a1[n];
+    up7_pdm(a1, central8(a1), fld, o.emf_pdm, fl, fr);
+  }
+  {
+    double a2[8], fld[8];
+    const double *ph = c.p(P_PHIPRED + t2);
+    for (int n = 0; n < 8; ++n) {
+      const long fa = at + (n - 4) * step[t1];
+      a2[n] = c.p(P_FACE + FACE_PLANES * t2 + FO_AREA)[fa];
+      fld[n] = ph[fa] / a2[n];
+    }
+    const double ia = central8(a2);
+    if (!isfinite(ia) || fabs(ia) <= DBL_MIN) {
+      c.fail(53, at);
+      return;
+    }
+    up7_pdm(a2, ia, fld, o.emf_pdm, sl, sr);
+  }
+  double solve[5], bt[2];
+  for (int m = 0; m < 5; ++m) solve[m] = c.p(eb + EO_SOLVE + m)[at];
+  edge_field(solve, 0.5 * (fl + fr), 0.5 * (sl + sr), bt);
+  bt[0] += c.p(P_B0ET + 2 * e)[at];
+  bt[1] += c.p(P_B0ET + 2 * e + 1)[at];
+  const double jump = (sr - sl) - (fr - fl);
+  double rho = 0.0;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) rho += 0.25 * c.p(P_UPRED)[at - a * step[t1] - b * step[t2]];
+  if (!isfinite(rho) || rho <= DBL_MIN) {
+    c.fail(54, at);
+    return;
+  }
+  double emf;
+  if (!edge_emf(v1, v2, bt[0], bt[1], jump, rho, c.p(eb + EO_LEN)[at], o.emf_diff,
+                o.boris, o.c, o.emf_cfl, c.sc->dt, emf)) {
+    c.fail(55, at);
+    return;
+  }
+  c.p(P_EMF + e)[at] = emf;
+}
+
+// cubedsphere_earth_boundary.c:445-489: outer shells blend toward the wind
+// line integral -(V x B) . l.
+__global__ void k_earth_emf_outer(K c, int d, WindSeries w) {
+  const Layout &L = c.L;
+  const int outer = L.ghost + L.nr;
+  const int lo[3] = {outer - 3, 0, 0};
+  const int hi[3] = {min(outer + 1, L.NI + (d != 0)), L.NJ + (d != 1), L.NK + (d != 2)};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  if (!c.b(Q_VALID + d)[at]) return;
+  const long up = at + (d == 0 ? L.di() : d == 1 ? (long)L.SK : 1);
+  double x[3];
+  for (int m = 0; m < 3; ++m) x[m] = 0.5 * (c.p(P_VERT + m)[at] + c.p(P_VERT + m)[up]);
+  double sw[8], wt;
+  if (!sample_wind(w, x, c.sc->time + 0.5 * c.sc->dt, sw, wt)) {
+    c.fail(61, at);
+    return;
+  }
+  if (i < outer) wt /= 1.0 + (double)(outer - i);
+  const double *vv = sw + 2, *bb = sw + 5;
+  const double el[3] = {-(vv[1] * bb[2] - vv[2] * bb[1]), -(vv[2] * bb[0] - vv[0] * bb[2]),
+                        -(vv[0] * bb[1] - vv[1] * bb[0])};
+  const int eb = P_EDGE + EDGE_PLANES * d;
+  const double len = c.p(eb + EO_LEN)[at];
+  double line = 0.0;
+  for (int m = 0; m < 3; ++m) line += el[m] * c.p(eb + EO_N + m)[at] * len;
+  double *emf = c.p(P_EMF + d) + at;
+  *emf = (1.0 - wt) * *emf + wt * line;
+}
+
+// cubedsphere_earth_boundary.c:362-443: diffusive wind EMF on the outer wall.
+__global__ void k_earth_emf_diffusive(K c, WindSeries w) {
+  const Layout &L = c.L;
+  const int i = L.ghost + L.nr;
+  const int lo[3] = {i, L.ghost, L.ghost};
+  const int hi[3] = {i + 1, L.ghost + L.n + 1, L.ghost + L.n + 1};
+  int f, ii, j, k;
+  if (!box(tid(), lo, hi, f, ii, j, k)) return;
+  // Edge extents: J (NI+1, NJ, NK+1), K (NI+1, NJ+1, NK).
+  if (j == 0 || k == 0 || j >= L.NJ || k >= L.NK) return;
+  const long at = L.at(f, i, j, k);
+  if (!c.b(Q_VALID + 1)[at] || !c.b(Q_VALID + 2)[at]) return;
+  const long si = L.di(), sj = L.SK;
+  auto dens = [&](int dir, long fa, double &out) {
+    const double area = c.p(P_FACE + FACE_PLANES * dir + FO_AREA)[fa];
+    if (!(area > 0.0)) return false;
+    out = c.p(P_PHIPRED + dir)[fa] / area;
+    return (bool)isfinite(out);
+  };
+  double bikp, bikm, bkip, bkim, bjip, bjim, bijp, bijm;
+  if (!dens(0, at, bikp) || !dens(0, at - 1, bikm) || !dens(2, at, bkip) ||
+      !dens(2, at - si, bkim) || !dens(1, at, bjip) || !dens(1, at - si, bjim) ||
+      !dens(0, at, bijp) || !dens(0, at - sj, bijm)) {
+    c.fail(62, at);
+    return;
+  }
+  const double cj = (bikp - bikm) - (bkip - bkim);
+  const double ck = (bjip - bjim) - (bijp - bijm);
+  const double dl = sqrt(c.p(P_EDGE + EDGE_PLANES * 1 + EO_LEN)[at] *
+                         c.p(P_EDGE + EDGE_PLANES * 2 + EO_LEN)[at]);
+  double xj[3], xk[3];
+  for (int m = 0; m < 3; ++m) {
+    xj[m] = 0.5 * (c.p(P_VERT + m)[at] + c.p(P_VERT + m)[at + sj]);
+    xk[m] = 0.5 * (c.p(P_VERT + m)[at] + c.p(P_VERT + m)[at + 1]);
+  }
+  const double t = c.sc->time + 0.5 * c.sc->dt;
+  double wj[8], wk[8], gj, gk;
+  if (!(dl > 0.0) || !sample_wind(w, xj, t, wj, gj) || !sample_wind(w, xk, t, wk, gk)) {
+    c.fail(63, at);
+    return;
+  }
+  const double c2 = w.by * w.by + w.bz * w.bz;
+  const double front = fmax(fabs(wj[2]), fabs(wk[2])) / sqrt(1.0 + c2);
+  const double speed = fmin(front, c.o.wall_cfl * dl / c.sc->dt);
+  c.p(P_EMF + 1)[at] += gj * speed * cj * dl;
+  c.p(P_EMF + 2)[at] += gk * speed * ck * dl;
+}
+
+// cubedsphere_earth_boundary.c:493-503: E_t = 0 on the inner wall shell.
+__global__ void k_earth_emf_inner(K c) {
+  const Layout &L = c.L;
+  const int lo[3] = {L.ghost, 0, 0};
+  const int hi[3] = {L.ghost + 1, L.NJ + 1, L.NK + 1};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  // J edges span (NI+1, NJ, NK+1), K edges (NI+1, NJ+1, NK).
+  if (j < L.NJ && c.b(Q_VALID + 1)[at]) c.p(P_EMF + 1)[at] = 0.0;
+  if (k < L.NK && c.b(Q_VALID + 2)[at]) c.p(P_EMF + 2)[at] = 0.0;
+}
+
+// ---- update ------------------------------------------------------------------------
+
+// Fused stress divergence (nonorthogonal_sweep.c:550-622), Reynolds update
+// (nonorthogonal_step.c:542-553), CT on the cell's faces (:233-298), B
+// recovery (:565) and Boris correction (:571-606) of one active cell,
+// writing the new level.
+__global__ void __launch_bounds__(128) k_update(K c) {
+  const Layout &L = c.L;
+  const int lo[3] = {L.ghost, L.ghost, L.ghost};
+  const int hi[3] = {L.ghost + L.nr, L.ghost + L.n, L.ghost + L.n};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  const long si = L.di(), sj = L.SK;
+  const long step[3] = {si, sj, 1};
+  const Options &o = c.o;
+  const double dt = c.sc->dt;
+  const double vol = c.p(P_VOL)[at];
+  const double ivol = 1.0 / vol;
+  double rate[5], mr[3];
+  for (int q = 0; q < 5; ++q) {
+    double diff = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      const double *fp = c.p(P_FLUID + 5 * d + q);
+      diff += fp[at] - fp[at + step[d]];
+    }
+    rate[q] = diff * ivol;
+  }
+  for (int m = 0; m < 3; ++m) {
+    double diff = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      const double *mp = c.p(P_MAXWELL + 3 * d + m);
+      diff += mp[at] - mp[at + step[d]];
+    }
+    mr[m] = diff * ivol;
+    mr[m] += c.p(P_FORCE + m)[at];
+  }
+  double u0[5], u[5];
+  for (int q = 0; q < 5; ++q) {
+    u0[q] = c.p(plane_u(c.cur, q))[at];
+    u[q] = u0[q] + dt * rate[q];
+  }
+  // CT: new flux of the six faces (identical arithmetic in both cells of a
+  // shared face).
+  const double *E0 = c.p(P_EMF), *E1 = c.p(P_EMF + 1), *E2 = c.p(P_EMF + 2);
+  double flux[6];
+  for (int s = 0; s < 2; ++s) {
+    const long a0 = at + s * si;
+    const double curl0 = E2[a0 + sj] - E2[a0] - E1[a0 + 1] + E1[a0];
+    flux[0 + s] = c.p(plane_phi(c.cur, 0))[a0] - dt * curl0;
+    const long a1 = at + s * sj;
+    const double curl1 = E0[a1 + 1] - E0[a1] - E2[a1 + si] + E2[a1];
+    flux[2 + s] = c.p(plane_phi(c.cur, 1))[a1] - dt * curl1;
+    const long a2 = at + s;
+    const double curl2 = E1[a2 + si] - E1[a2] - E0[a2 + sj] + E0[a2];
+    flux[4 + s] = c.p(plane_phi(c.cur, 2))[a2] - dt * curl2;
+  }
+  double fc[6][3], cen[3], rec[9], bn[3];
+  for (int d = 0; d < 3; ++d)
+    for (int s = 0; s < 2; ++s)
+      for (int m = 0; m < 3; ++m)
+        fc[2 * d + s][m] = c.p(P_FACE + FACE_PLANES * d + FO_CEN + m)[at + s * step[d]];
+  for (int m = 0; m < 3; ++m) cen[m] = c.p(P_CEN + m)[at];
+  for (int q = 0; q < 9; ++q) rec[q] = c.p(P_REC + q)[at];
+  const int piv = c.b(Q_PIV)[at];
+  if (piv == 0xff || !recover_cell(flux, fc, cen, vol, rec, piv, bn)) {
+    c.fail(71, at);
+    return;
+  }
+  if (o.boris) {
+    double btn[3], bto[3];
+    for (int m = 0; m < 3; ++m) {
+      const double b0 = c.p(P_B0C + m)[at];
+      btn[m] = bn[m] + b0;
+      bto[m] = c.p(plane_b(c.cur, m))[at] + b0;
+    }
+    if (!boris_update(u, u0, btn, bto, rate, mr, o.c, dt, o.gamma, o.rho_floor,
+                      o.p_floor)) {
+      c.fail(72, at);
+      return;
+    }
+  } else {
+    maxwell_update(u, mr, dt, o.gamma, o.rho_floor, o.p_floor);
+  }
+  for (int q = 0; q < 5; ++q) c.p(plane_u(c.old, q))[at] = u[q];
+  for (int m = 0; m < 3; ++m) c.p(plane_b(c.old, m))[at] = bn[m];
+  c.p(plane_phi(c.old, 0))[at] = flux[0];
+  c.p(plane_phi(c.old, 1))[at] = flux[2];
+  c.p(plane_phi(c.old, 2))[at] = flux[4];
+  if (i == hi[0] - 1) c.p(plane_phi(c.old, 0))[at + si] = flux[1];
+  if (j == hi[1] - 1) c.p(plane_phi(c.old, 1))[at + sj] = flux[3];
+  if (k == hi[2] - 1) c.p(plane_phi(c.old, 2))[at + 1] = flux[5];
+}
+
+// Ghost values of the new level as the reference leaves them in storage:
+// ghost U and ghost face fluxes keep the step's start values (the reference
+// copies current into old before updating only active elements).
+__global__ void k_keep_ghosts(K c) {
+  const Layout &L = c.L;
+  const int lo[3] = {0, 0, 0};
+  const int hi[3] = {L.NI + 1, L.NJ + 1, L.NK + 1};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  const int g = L.ghost;
+  const bool ai = i >= g && i < g + L.nr, aj = j >= g && j < g + L.n,
+             ak = k >= g && k < g + L.n;
+  const bool cell = i < L.NI && j < L.NJ && k < L.NK;
+  if (cell && !(ai && aj && ak))
+    for (int q = 0; q < 5; ++q) c.p(plane_u(c.old, q))[at] = c.p(plane_u(c.cur, q))[at];
+  const bool fi = i >= g && i <= g + L.nr, fj = j >= g && j <= g + L.n,
+             fk = k >= g && k <= g + L.n;
+  const bool face[3] = {i <= L.NI && j < L.NJ && k < L.NK && !(fi && aj && ak),
+                        i < L.NI && j <= L.NJ && k < L.NK && !(ai && fj && ak),
+                        i < L.NI && j < L.NJ && k <= L.NK && !(ai && aj && fk)};
+  for (int d = 0; d < 3; ++d)
+    if (face[d]) c.p(plane_phi(c.old, d))[at] = c.p(plane_phi(c.cur, d))[at];
+}
+
+}  // namespace
+
+static unsigned nblocks(long n, int threads) {
+  return (unsigned)((n + threads - 1) / threads);
+}
+
+static K make_k(Problem &P) {
+  K c;
+  c.L = P.L;
+  c.pl = P.planes;
+  c.by = P.bytes;
+  c.sc = P.scalars;
+  c.cur = P.level;
+  c.old = 1 - P.level;
+  Options &o = c.o;
+  const gcs_problem_config &cfg = P.cfg;
+  o.gamma = cfg.startup.gamma;
+  o.rho_floor = 1e-6;
+  o.p_floor = 1e-8;
+  o.pdm = 1.0;
+  o.hogs_h = 0.25;
+  o.hogs_m = 0.25;
+  o.c = 10.0;
+  o.emf_rho_floor = 1e-6;
+  o.emf_pdm = 1.0;
+  o.emf_diff = 0.5;
+  o.emf_cfl = cfg.cfl;
+  o.boris = cfg.use_boris != 0;
+  o.hogs = 1;
+  o.wall_rho_floor = 1e-6;
+  o.wall_p_floor = 1e-8;
+  o.wall_cfl = cfg.cfl;
+  o.strict_wall = cfg.strict_inner_gas_wall != 0;
+  return c;
+}
+
+int launch_recover_all(Problem &P, int phi_first, int b_first) {
+  K c = make_k(P);
+  const Layout &L = P.L;
+  const long n = 6L * L.NI * L.NJ * L.NK;
+  k_recover<<<nblocks(n, 256), 256, 0, P.stream>>>(c, phi_first, b_first, 0, 0);
+  GCS_HIP(hipGetLastError());
+  return 0;
+}
+
+int enqueue_step(Problem &P, bool explicit_dt) {
+  K c = make_k(P);
+  const Layout &L = P.L;
+  hipStream_t s = P.stream;
+  const int T = 256;
+  const long n_active = 6L * L.nr * L.n * L.n;
+  const long n_cells = 6L * L.NI * L.NJ * L.NK;
+  const long n_box = 6L * (L.NI + 1) * (L.NJ + 1) * (L.NK + 1);
+  const WindSeries &w = P.wind_series;
+
+  if (!explicit_dt) {
+    k_cfl<<<nblocks(n_active, T), T, 0, s>>>(c);
+    k_dt<<<1, 1, 0, s>>>(c, P.cfg.dt_fraction);
+  }
+  k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams, 1);
+  k_halo_cells<<<nblocks(P.n_halo_cells, T), T, 0, s>>>(c, P.halo_cells, P.n_halo_cells);
+  if (!P.earth) k_radial_outflow<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c);
+  k_halo_faces<<<nblocks(P.n_halo_faces, T), T, 0, s>>>(c, P.halo_faces, P.n_halo_faces);
+  if (P.earth) {
+    k_earth_cells<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c, w);
+    for (int d = 0; d < 3; ++d) {
+      const long n_out = (L.NI + (d == 0)) - (L.ghost + L.nr + (d == 0));
+      const long n = 6L * (L.ghost + n_out) * (L.NJ + (d == 1)) * (L.NK + (d == 2));
+      k_earth_faces<<<nblocks(n, T), T, 0, s>>>(c, d, w);
+    }
+    k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.cur, 0), plane_b(c.cur, 0), 1, 0);
+    k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.old, 0), plane_b(c.old, 0), 1, 0);
+  }
+  k_predict<<<nblocks(n_box, T), T, 0, s>>>(c);
+  for (int d = 0; d < 3; ++d) {
+    const long n = 6L * (L.nr + (d == 0)) * (L.n + (d == 1)) * (L.n + (d == 2));
+    k_face_flux<<<nblocks(n, 128), 128, 0, s>>>(c, d);
+  }
+  k_sync_flux<<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams);
+  if (P.earth) k_inner_trap<<<nblocks(6L * L.n * L.n, T), T, 0, s>>>(c);
+  for (int d = 0; d < 3; ++d) {
+    long n = 6;
+    const int ext[3] = {L.NI, L.NJ, L.NK};
+    const int act[3] = {L.nr, L.n, L.n};
+    for (int a = 0; a < 3; ++a)
+      n *= a == d ? act[a] + 1 : std::min(L.ghost + act[a] + 4, ext[a]) - (L.ghost - 4);
+    k_face_velocity<<<nblocks(n, T), T, 0, s>>>(c, d);
+  }
+  for (int e = 0; e < 3; ++e) {
+    const long n = 6L * (L.nr + (e != 0)) * (L.n + (e != 1)) * (L.n + (e != 2));
+    k_edge_emf<<<nblocks(n, 128), 128, 0, s>>>(c, e);
+  }
+  if (P.earth) {
+    for (int d = 0; d < 3; ++d) {
+      const int outer = L.ghost + L.nr;
+      const long n = 6L * (std::min(outer + 1, L.NI + (d != 0)) - (outer - 3)) *
+                     (L.NJ + (d != 1)) * (L.NK + (d != 2));
+      k_earth_emf_outer<<<nblocks(n, T), T, 0, s>>>(c, d, w);
+    }
+    k_earth_emf_diffusive<<<nblocks(6L * (L.n + 1) * (L.n + 1), T), T, 0, s>>>(c, w);
+    k_earth_emf_inner<<<nblocks(6L * (L.NJ + 1) * (L.NK + 1), T), T, 0, s>>>(c);
+  }
+  k_sync_edges<<<nblocks(P.n_edge_groups, T), T, 0, s>>>(c, P.edge_groups, P.edge_members,
+                                                         P.n_edge_groups);
+  k_update<<<nblocks(n_active, 128), 128, 0, s>>>(c);
+  // Final seam average of the new face flux, then B of the touched cells.
+  K n = c;
+  n.cur = c.old;
+  n.old = c.cur;
+  k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(n, P.seams, P.n_seams, 0);
+  k_recover_list<<<nblocks(P.n_seam_cells, T), T, 0, s>>>(n, P.seam_cells, P.n_seam_cells,
+                                                          plane_phi(n.cur, 0),
+                                                          plane_b(n.cur, 0));
+  if (P.fidelity) {
+    k_keep_ghosts<<<nblocks(n_box, T), T, 0, s>>>(c);
+    k_recover<<<nblocks(n_cells, T), T, 0, s>>>(n, plane_phi(n.cur, 0), plane_b(n.cur, 0), 0, 1);
+  }
+  if (!explicit_dt) k_clock<<<1, 1, 0, s>>>(c);
+  GCS_HIP(hipGetLastError());
+  P.level = 1 - P.level;
+  P.host_dirty = true;
+  return 0;
+}
+
+}  // namespace gcs

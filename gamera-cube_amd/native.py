@@ -1,0 +1,65 @@
+"""Loader for the in-tree HIP library (gamera-cube_amd/libgcs.so).
+
+There is no CPU fallback: if the library is missing or the GPU cannot run it,
+these calls raise.
+"""
+import ctypes
+import os
+
+from .config import ProblemConfig
+from .problem import Problem
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgcs.so")
+
+_lib = None
+
+
+def library():
+    """Load libgcs.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: run `python gamera-cube_amd/build.py`")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.gcs_last_error.restype = ctypes.c_char_p
+        lib.gcs_config_size.restype = ctypes.c_size_t
+        if lib.gcs_config_size() != ctypes.sizeof(ProblemConfig):
+            raise RuntimeError("ProblemConfig does not match gcs_problem_config")
+        lib.gcs_problem_counters.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_double)]
+        _lib = lib
+    return _lib
+
+
+def last_error():
+    return library().gcs_last_error().decode()
+
+
+class GpuProblem(Problem):
+    """A six-face Earth problem resident on the GPU."""
+
+    def __init__(self, config):
+        lib = library()
+        try:
+            super().__init__(lib, "gcs_problem", config)
+        except RuntimeError as exc:
+            raise RuntimeError(f"{exc}: {last_error()}") from None
+
+    def advance(self, steps):
+        try:
+            super().advance(steps)
+        except RuntimeError as exc:
+            raise RuntimeError(f"{exc}: {last_error()}") from None
+
+    def step_explicit(self, ratio, dt, time, previous_time):
+        try:
+            super().step_explicit(ratio, dt, time, previous_time)
+        except RuntimeError as exc:
+            raise RuntimeError(f"{exc}: {last_error()}") from None
+
+    def counters(self, face):
+        out = (ctypes.c_double * 2)()
+        self._lib.gcs_problem_counters(self._handle, face, out)
+        return out[0], out[1]
