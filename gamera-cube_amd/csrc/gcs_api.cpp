@@ -352,6 +352,64 @@ int gcs_problem_edge_valid(Problem *P, int face, int dir, unsigned char **data,
   return 0;
 }
 
+// Keep (1) or skip (0) the work that only reproduces the reference's ghost
+// values in storage; the step's active results are the same either way.
+int gcs_problem_set_fidelity(Problem *P, int on) {
+  if (P == nullptr) return -1;
+  P->fidelity = on != 0;
+  return 0;
+}
+
+int gcs_stage_count(void) { return gcs::kStages; }
+const char *gcs_stage_name(int stage) { return gcs::stage_name(stage); }
+
+// Run `steps` driver steps timed by HIP events on the problem's stream.
+// elapsed_ms: start to end of all steps. With profile != 0, stage_ms[s]
+// receives the summed time from the start of stage s to the start of the
+// next recorded stage, over all steps (kStages entries).
+int gcs_problem_time_steps(Problem *P, long steps, int profile,
+                           double *elapsed_ms, double *stage_ms) {
+  if (P == nullptr || steps <= 0 || elapsed_ms == nullptr) return -1;
+  P->profiling = profile != 0;
+  if (P->profiling) {
+    const size_t need = (size_t)steps * gcs::kStages + 1;
+    while (P->stage_events.size() < need) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return -1;
+      P->stage_events.push_back(e);
+    }
+    P->stage_ids.assign(P->stage_events.size(), -1);
+    P->stage_cursor = 0;
+  }
+  if (hipEventRecord(P->ev_start, P->stream) != hipSuccess) return -1;
+  for (long n = 0; n < steps; ++n)
+    if (gcs::enqueue_step(*P, false) != 0) {
+      P->profiling = false;
+      return -1;
+    }
+  if (hipEventRecord(P->ev_stop, P->stream) != hipSuccess) return -1;
+  if (gcs::check_failure(*P) != 0) {
+    P->profiling = false;
+    return -1;
+  }
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, P->ev_start, P->ev_stop) != hipSuccess) return -1;
+  *elapsed_ms = ms;
+  if (P->profiling && stage_ms != nullptr) {
+    for (int s = 0; s < gcs::kStages; ++s) stage_ms[s] = 0.0;
+    for (size_t e = 0; e + 1 < P->stage_cursor; ++e) {
+      if (P->stage_ids[e] == gcs::S_END) continue;
+      float t = 0.0f;
+      if (hipEventElapsedTime(&t, P->stage_events[e], P->stage_events[e + 1]) !=
+          hipSuccess)
+        return -1;
+      stage_ms[P->stage_ids[e]] += t;
+    }
+  }
+  P->profiling = false;
+  return 0;
+}
+
 // Emergency-diffusion counters of one cube face (count, max speed).
 void gcs_problem_counters(Problem *P, int face, double out[2]) {
   gcs::StepScalars s;

@@ -107,10 +107,26 @@ struct Problem {
   WindSeries wind_series;
   bool earth = true;               // Earth radial boundary active
   bool fidelity = true;            // reproduce ghost values of the storage
+  // Stage timing: when `profiling`, enqueue_step records an event before
+  // every stage into `stage_events` (one row of kMaxStages per step).
+  bool profiling = false;
+  std::vector<hipEvent_t> stage_events;
+  std::vector<int> stage_ids;      // stage id of each recorded event
+  size_t stage_cursor = 0;
 
   double *plane(int p) const { return planes + (long)p * L.total; }
   uint8_t *byte_plane(int q) const { return bytes + (long)q * L.total; }
 };
+
+// Stages of one step, in launch order (names for timing reports).
+enum Stage {
+  S_CFL, S_DT, S_SYNC_PHI, S_HALO_CELLS, S_OUTFLOW, S_HALO_FACES,
+  S_EARTH_CELLS, S_EARTH_FACES, S_EARTH_RECOVER, S_PREDICT, S_FACE_FLUX0,
+  S_FACE_FLUX1, S_FACE_FLUX2, S_SYNC_FLUX, S_INNER_TRAP, S_FACE_VELOCITY,
+  S_EDGE_EMF0, S_EDGE_EMF1, S_EDGE_EMF2, S_EARTH_EMF, S_SYNC_EDGES, S_UPDATE,
+  S_FINAL_SYNC, S_SEAM_RECOVER, S_FIDELITY, S_CLOCK, S_END, kStages
+};
+const char *stage_name(int stage);
 
 // Setup: geometry, background, initial state and plans.
 int setup_problem(Problem &P);

@@ -1073,6 +1073,24 @@ int launch_recover_all(Problem &P, int phi_first, int b_first) {
   return 0;
 }
 
+const char *stage_name(int stage) {
+  static const char *names[kStages] = {
+      "cfl", "dt", "sync_phi", "halo_cells", "radial_outflow", "halo_faces",
+      "earth_cells", "earth_faces", "earth_recover", "predict", "face_flux_i",
+      "face_flux_j", "face_flux_k", "sync_flux", "inner_trap", "face_velocity",
+      "edge_emf_i", "edge_emf_j", "edge_emf_k", "earth_emf", "sync_edges",
+      "update", "final_sync", "seam_recover", "fidelity", "clock", "end"};
+  return stage >= 0 && stage < kStages ? names[stage] : "?";
+}
+
+// Record a stage boundary when profiling (events are preallocated).
+static void mark(Problem &P, int stage) {
+  if (!P.profiling || P.stage_cursor >= P.stage_events.size()) return;
+  (void)hipEventRecord(P.stage_events[P.stage_cursor], P.stream);
+  P.stage_ids[P.stage_cursor] = stage;
+  ++P.stage_cursor;
+}
+
 int enqueue_step(Problem &P, bool explicit_dt) {
   K c = make_k(P);
   const Layout &L = P.L;
@@ -1084,30 +1102,48 @@ int enqueue_step(Problem &P, bool explicit_dt) {
   const WindSeries &w = P.wind_series;
 
   if (!explicit_dt) {
+    mark(P, S_CFL);
     k_cfl<<<nblocks(n_active, T), T, 0, s>>>(c);
+    mark(P, S_DT);
     k_dt<<<1, 1, 0, s>>>(c, P.cfg.dt_fraction);
   }
+  mark(P, S_SYNC_PHI);
   k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams, 1);
+  mark(P, S_HALO_CELLS);
   k_halo_cells<<<nblocks(P.n_halo_cells, T), T, 0, s>>>(c, P.halo_cells, P.n_halo_cells);
-  if (!P.earth) k_radial_outflow<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c);
+  if (!P.earth) {
+    mark(P, S_OUTFLOW);
+    k_radial_outflow<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c);
+  }
+  mark(P, S_HALO_FACES);
   k_halo_faces<<<nblocks(P.n_halo_faces, T), T, 0, s>>>(c, P.halo_faces, P.n_halo_faces);
   if (P.earth) {
+    mark(P, S_EARTH_CELLS);
     k_earth_cells<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c, w);
+    mark(P, S_EARTH_FACES);
     for (int d = 0; d < 3; ++d) {
       const long n_out = (L.NI + (d == 0)) - (L.ghost + L.nr + (d == 0));
       const long n = 6L * (L.ghost + n_out) * (L.NJ + (d == 1)) * (L.NK + (d == 2));
       k_earth_faces<<<nblocks(n, T), T, 0, s>>>(c, d, w);
     }
+    mark(P, S_EARTH_RECOVER);
     k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.cur, 0), plane_b(c.cur, 0), 1, 0);
     k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.old, 0), plane_b(c.old, 0), 1, 0);
   }
+  mark(P, S_PREDICT);
   k_predict<<<nblocks(n_box, T), T, 0, s>>>(c);
   for (int d = 0; d < 3; ++d) {
     const long n = 6L * (L.nr + (d == 0)) * (L.n + (d == 1)) * (L.n + (d == 2));
+    mark(P, S_FACE_FLUX0 + d);
     k_face_flux<<<nblocks(n, 128), 128, 0, s>>>(c, d);
   }
+  mark(P, S_SYNC_FLUX);
   k_sync_flux<<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams);
-  if (P.earth) k_inner_trap<<<nblocks(6L * L.n * L.n, T), T, 0, s>>>(c);
+  if (P.earth) {
+    mark(P, S_INNER_TRAP);
+    k_inner_trap<<<nblocks(6L * L.n * L.n, T), T, 0, s>>>(c);
+  }
+  mark(P, S_FACE_VELOCITY);
   for (int d = 0; d < 3; ++d) {
     long n = 6;
     const int ext[3] = {L.NI, L.NJ, L.NK};
@@ -1118,9 +1154,11 @@ int enqueue_step(Problem &P, bool explicit_dt) {
   }
   for (int e = 0; e < 3; ++e) {
     const long n = 6L * (L.nr + (e != 0)) * (L.n + (e != 1)) * (L.n + (e != 2));
+    mark(P, S_EDGE_EMF0 + e);
     k_edge_emf<<<nblocks(n, 128), 128, 0, s>>>(c, e);
   }
   if (P.earth) {
+    mark(P, S_EARTH_EMF);
     for (int d = 0; d < 3; ++d) {
       const int outer = L.ghost + L.nr;
       const long n = 6L * (std::min(outer + 1, L.NI + (d != 0)) - (outer - 3)) *
@@ -1130,22 +1168,31 @@ int enqueue_step(Problem &P, bool explicit_dt) {
     k_earth_emf_diffusive<<<nblocks(6L * (L.n + 1) * (L.n + 1), T), T, 0, s>>>(c, w);
     k_earth_emf_inner<<<nblocks(6L * (L.NJ + 1) * (L.NK + 1), T), T, 0, s>>>(c);
   }
+  mark(P, S_SYNC_EDGES);
   k_sync_edges<<<nblocks(P.n_edge_groups, T), T, 0, s>>>(c, P.edge_groups, P.edge_members,
                                                          P.n_edge_groups);
+  mark(P, S_UPDATE);
   k_update<<<nblocks(n_active, 128), 128, 0, s>>>(c);
   // Final seam average of the new face flux, then B of the touched cells.
   K n = c;
   n.cur = c.old;
   n.old = c.cur;
+  mark(P, S_FINAL_SYNC);
   k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(n, P.seams, P.n_seams, 0);
+  mark(P, S_SEAM_RECOVER);
   k_recover_list<<<nblocks(P.n_seam_cells, T), T, 0, s>>>(n, P.seam_cells, P.n_seam_cells,
                                                           plane_phi(n.cur, 0),
                                                           plane_b(n.cur, 0));
   if (P.fidelity) {
+    mark(P, S_FIDELITY);
     k_keep_ghosts<<<nblocks(n_box, T), T, 0, s>>>(c);
     k_recover<<<nblocks(n_cells, T), T, 0, s>>>(n, plane_phi(n.cur, 0), plane_b(n.cur, 0), 0, 1);
   }
-  if (!explicit_dt) k_clock<<<1, 1, 0, s>>>(c);
+  if (!explicit_dt) {
+    mark(P, S_CLOCK);
+    k_clock<<<1, 1, 0, s>>>(c);
+  }
+  mark(P, S_END);
   GCS_HIP(hipGetLastError());
   P.level = 1 - P.level;
   P.host_dirty = true;

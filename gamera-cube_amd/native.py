@@ -6,7 +6,7 @@ these calls raise.
 import ctypes
 import os
 
-from .config import ProblemConfig
+from .config import ProblemConfig  # noqa: F401
 from .problem import Problem
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -63,3 +63,35 @@ class GpuProblem(Problem):
         out = (ctypes.c_double * 2)()
         self._lib.gcs_problem_counters(self._handle, face, out)
         return out[0], out[1]
+
+
+def stage_names():
+    lib = library()
+    lib.gcs_stage_name.restype = ctypes.c_char_p
+    return [lib.gcs_stage_name(i).decode() for i in range(lib.gcs_stage_count())]
+
+
+def _time_steps(self, steps, profile=True):
+    """(total ms, {stage: ms summed over the steps}) timed with HIP events."""
+    lib = self._lib
+    n = lib.gcs_stage_count()
+    elapsed = ctypes.c_double()
+    stages = (ctypes.c_double * n)()
+    fn = lib.gcs_problem_time_steps
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_int,
+                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    if fn(self._handle, int(steps), 1 if profile else 0, ctypes.byref(elapsed),
+          stages) != 0:
+        raise RuntimeError(f"gcs_problem_time_steps failed: {last_error()}")
+    names = stage_names()
+    return elapsed.value, {names[i]: stages[i] for i in range(n)} if profile else {}
+
+
+def _set_fidelity(self, on):
+    self._lib.gcs_problem_set_fidelity.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    if self._lib.gcs_problem_set_fidelity(self._handle, 1 if on else 0) != 0:
+        raise RuntimeError("gcs_problem_set_fidelity failed")
+
+
+GpuProblem.time_steps = _time_steps
+GpuProblem.set_fidelity = _set_fidelity

@@ -1,0 +1,85 @@
+/*
+ * C ABI of libgcs.so, the MI355X implementation of the GAMERA-Cube
+ * six-face time step.
+ *
+ * Problem API: a device-resident Earth problem described by a
+ * gcs_problem_config (gamera_cs_problem.h). It owns everything the
+ * reference's serial Earth driver owns (test_cubedsphere_dipole_background.c
+ * :260-699: grid, storage, background, wind, boundary and dt rule) with the
+ * arrays in device memory; gcs_problem_field copies any of them back in the
+ * reference's layout.
+ *
+ * Every function returns 0 on success and -1 on failure, like the
+ * reference; gcs_last_error() describes the last failure of the calling
+ * thread (numerical failures in the reference's MHD_FAILURE form).
+ */
+#ifndef GAMERA_CS_STEP_H
+#define GAMERA_CS_STEP_H
+
+#include "gamera_cs_problem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gcs_problem gcs_problem;
+
+/* Description of the last error on this thread. */
+const char *gcs_last_error(void);
+
+/* sizeof(gcs_problem_config) as compiled into the library. */
+size_t gcs_config_size(void);
+
+/* Setup of the serial Earth driver (:330-546) on the device: cube
+ * vertices, 12-point Gauss geometry, face-frame calibration, dipole
+ * background, startup state, residual fluxes and the halo/seam plans. */
+gcs_problem *gcs_problem_create(const gcs_problem_config *config);
+void gcs_problem_destroy(gcs_problem *problem);
+
+/* `steps` iterations of the driver loop (:589-699): CFL dt rule, then
+ * gamera_cs_advance_six_with_background_planned (cubedsphere_step.c:13)
+ * with the Earth radial-ghost, inner-flux and EMF callbacks. */
+int gcs_problem_advance(gcs_problem *problem, long steps);
+
+/* One six-face step with caller-chosen predictor ratio, dt and boundary
+ * times (the arguments of cubedsphere_step.h:58-67); the clock is kept. */
+int gcs_problem_step_explicit(gcs_problem *problem, double predictor_ratio,
+                              double dt, double current_time,
+                              double previous_time);
+
+/* time, dt of the last step, previous dt, completed steps. */
+void gcs_problem_clock(gcs_problem *problem, double out[4]);
+
+/* Host copy of field `field` (GCS_FIELD_*) of cube face `face`, in the
+ * reference's array-of-structs layout; valid until the next step. */
+int gcs_problem_field(gcs_problem *problem, int face, int field,
+                      double **data, size_t *count);
+int gcs_problem_edge_valid(gcs_problem *problem, int face, int direction,
+                           unsigned char **data, size_t *count);
+
+/* Emergency-diffusion counters of one cube face: applied-face count and
+ * maximum interface speed (nonorthogonal_storage.h:29-30). */
+void gcs_problem_counters(gcs_problem *problem, int face, double out[2]);
+
+/* 1 (default): also reproduce the ghost values the reference leaves in
+ * storage after a step (ghost U and face flux keep their start-of-step
+ * values, ghost B is recovered from them). 0: skip that work; every
+ * ghost value is rewritten at the start of the next step, so results in
+ * the active domain are unchanged. No reference counterpart. */
+int gcs_problem_set_fidelity(gcs_problem *problem, int on);
+
+/* Timing of `steps` driver steps with HIP events on the problem's stream:
+ * total milliseconds, and with profile != 0 the milliseconds of each step
+ * stage summed over the steps (gcs_stage_count() entries named by
+ * gcs_stage_name). No reference counterpart (it times with MPI_Wtime,
+ * cubedsphere_profile.h:7-38). */
+int gcs_problem_time_steps(gcs_problem *problem, long steps, int profile,
+                           double *elapsed_ms, double *stage_ms);
+int gcs_stage_count(void);
+const char *gcs_stage_name(int stage);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

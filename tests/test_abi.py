@@ -1,0 +1,43 @@
+"""The C-ABI library loads and exports every function include/*.h declares
+(no GPU needed: nothing here calls into the device)."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+
+HEADERS = ["gamera_cs_step.h"]
+
+
+def declared_functions():
+    names = []
+    for h in HEADERS:
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"\b(g[a-z_0-9]+)\s*\([^;{]*\)\s*;", text)
+    return sorted(set(names))
+
+
+def test_header_declares_functions():
+    names = declared_functions()
+    assert "gcs_problem_create" in names and len(names) >= 8
+
+
+def test_library_exports_every_declared_symbol(gpu):
+    lib = gpu.library()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_config_struct_matches_library(gpu):
+    lib = gpu.library()
+    assert lib.gcs_config_size() == ctypes.sizeof(gpu.ProblemConfig)
+
+
+def test_invalid_config_is_rejected_before_any_device_work(gpu):
+    cfg = gpu.ProblemConfig()  # zero ghost / zero cells
+    lib = gpu.library()
+    lib.gcs_problem_create.restype = ctypes.c_void_p
+    lib.gcs_problem_create.argtypes = [ctypes.POINTER(gpu.ProblemConfig)]
+    assert not lib.gcs_problem_create(ctypes.byref(cfg))
+    assert "ghost" in gpu.last_error()

