@@ -192,14 +192,15 @@ __global__ void k_sync_edges(K c, const int *groups, const EdgeMember *m,
 
 // cubedsphere_halo.c:103-115
 GCS_DEV bool halo_valid(const double u[5]) {
+  // Evaluated without early exits: the flags are combined with '&' so that
+  // the caller's selection compiles to straight-line code (a short-circuit
+  // form here was structurized incorrectly by the ROCm 7.2 compiler, which
+  // then ignored the old-level test).
   const double rho = u[0];
-  if (!isfinite(rho) || rho <= 0.0) return false;
-  double m2 = 0.0;
-  for (int d = 0; d < 3; ++d) {
-    if (!isfinite(u[1 + d])) return false;
-    m2 += u[1 + d] * u[1 + d];
-  }
-  return isfinite(u[4]) && u[4] > 0.5 * m2 / rho;
+  const double m2 = u[1] * u[1] + u[2] * u[2] + u[3] * u[3];
+  const bool finite = isfinite(rho) & isfinite(u[1]) & isfinite(u[2]) &
+                      isfinite(u[3]) & isfinite(u[4]);
+  return finite & (rho > 0.0) & (u[4] > 0.5 * m2 / rho);
 }
 
 // 2 x 2 donor block sum in the reference's p-major / q-minor order.
@@ -228,15 +229,18 @@ __global__ void k_halo_cells(K c, const HaloCell *e, int n) {
     cb[d] = block_sum(c.p(plane_b(c.cur, d)), h.donor, sk, h.w);
     ob[d] = block_sum(c.p(plane_b(c.old, d)), h.donor, sk, h.w);
   }
-  int src = -1;
-  if (!halo_valid(cu) || !halo_valid(ou)) src = h.nearest;
+  // Nearest-donor fallback when either interpolated level is non-physical.
+  const bool ok = halo_valid(cu) & halo_valid(ou);
+  const int src = h.nearest;
   for (int q = 0; q < 5; ++q) {
-    c.p(plane_u(c.cur, q))[h.target] = src < 0 ? cu[q] : c.p(plane_u(c.cur, q))[src];
-    c.p(plane_u(c.old, q))[h.target] = src < 0 ? ou[q] : c.p(plane_u(c.old, q))[src];
+    const double nc = c.p(plane_u(c.cur, q))[src], no = c.p(plane_u(c.old, q))[src];
+    c.p(plane_u(c.cur, q))[h.target] = ok ? cu[q] : nc;
+    c.p(plane_u(c.old, q))[h.target] = ok ? ou[q] : no;
   }
   for (int d = 0; d < 3; ++d) {
-    c.p(plane_b(c.cur, d))[h.target] = src < 0 ? cb[d] : c.p(plane_b(c.cur, d))[src];
-    c.p(plane_b(c.old, d))[h.target] = src < 0 ? ob[d] : c.p(plane_b(c.old, d))[src];
+    const double nc = c.p(plane_b(c.cur, d))[src], no = c.p(plane_b(c.old, d))[src];
+    c.p(plane_b(c.cur, d))[h.target] = ok ? cb[d] : nc;
+    c.p(plane_b(c.old, d))[h.target] = ok ? ob[d] : no;
   }
 }
 
