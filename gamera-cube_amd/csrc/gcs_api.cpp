@@ -57,6 +57,8 @@ void make_wind(Problem &P) {
   w.reference[0] = c.wind_reference_x;
 }
 
+}  // namespace
+
 void free_problem(Problem *P) {
   if (P == nullptr) return;
   if (P->graph) (void)hipGraphExecDestroy(P->graph);
@@ -73,6 +75,23 @@ void free_problem(Problem *P) {
   if (P->ev_stop) (void)hipEventDestroy(P->ev_stop);
   if (P->stream) (void)hipStreamDestroy(P->stream);
   delete P;
+}
+
+int allocate_problem(Problem &P) {
+  const size_t nbytes = (size_t)PLANES * P.L.total * sizeof(double);
+  const size_t nflags = (size_t)BYTE_PLANES * P.L.total;
+  if (hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&P.planes, nbytes) != hipSuccess ||
+      hipMalloc(&P.bytes, nflags) != hipSuccess ||
+      hipMalloc(&P.scalars, sizeof(StepScalars)) != hipSuccess ||
+      hipMemset(P.planes, 0, nbytes) != hipSuccess ||
+      hipMemset(P.bytes, 0, nflags) != hipSuccess ||
+      hipEventCreate(&P.ev_start) != hipSuccess ||
+      hipEventCreate(&P.ev_stop) != hipSuccess) {
+    set_error("device allocation failed");
+    return -1;
+  }
+  return init_scalars(P);
 }
 
 int init_scalars(Problem &P) {
@@ -111,6 +130,8 @@ int check_failure(Problem &P) {
   P.steps = (long)s.steps;
   return 0;
 }
+
+namespace {
 
 // Copy one component plane of one cube face to the host in the
 // reference's k-fastest order for extent e.
@@ -262,22 +283,13 @@ Problem *gcs_problem_create(const gcs_problem_config *cfg) {
   P->cfg = *cfg;
   P->L = gcs::make_layout((int)cfg->radial_cells, (int)cfg->angular_cells,
                           (int)cfg->ghost);
-  const size_t nbytes = (size_t)gcs::PLANES * P->L.total * sizeof(double);
-  const size_t nflags = (size_t)gcs::BYTE_PLANES * P->L.total;
-  if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&P->planes, nbytes) != hipSuccess ||
-      hipMalloc(&P->bytes, nflags) != hipSuccess ||
-      hipMalloc(&P->scalars, sizeof(gcs::StepScalars)) != hipSuccess ||
-      hipMemset(P->planes, 0, nbytes) != hipSuccess ||
-      hipMemset(P->bytes, 0, nflags) != hipSuccess ||
-      hipEventCreate(&P->ev_start) != hipSuccess ||
-      hipEventCreate(&P->ev_stop) != hipSuccess) {
-    gcs::set_error("device allocation failed");
+  if (gcs::allocate_problem(*P) != 0) {
     gcs::free_problem(P);
     return nullptr;
   }
   gcs::make_wind(*P);
-  if (gcs::init_scalars(*P) != 0 || gcs::setup_problem(*P) != 0 ||
+  gcs::driver_options(*P);
+  if (gcs::setup_problem(*P) != 0 ||
       gcs::build_plans(*P) != 0) {
     gcs::free_problem(P);
     return nullptr;
@@ -305,6 +317,8 @@ int gcs_problem_step_explicit(Problem *P, double ratio, double dt, double time,
   s.ratio = ratio;
   s.time = time;
   s.previous_time = previous_time;
+  s.wall_time = time;
+  s.wall_dt = dt;
   if (hipMemcpy(P->scalars, &s, sizeof(s), hipMemcpyHostToDevice) != hipSuccess) return -1;
   if (gcs::enqueue_step(*P, true) != 0) return -1;
   return gcs::check_failure(*P);

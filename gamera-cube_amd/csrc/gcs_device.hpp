@@ -29,6 +29,7 @@ const char *last_error();
 // graph replays with the dt the previous step's CFL reduction chose.
 struct StepScalars {
   double dt, ratio, time, previous_time, previous_dt;
+  double wall_time, wall_dt;  // Earth wall EMF clock (gamera_cs_earth_boundary)
   unsigned long long cfl_min_bits; // min local CFL dt (IEEE bits, > 0)
   long long steps;
   unsigned int failed;     // nonzero: a kernel hit a reference failure path
@@ -74,8 +75,40 @@ struct EdgeMember {
   double sign;
 };
 
+// Physics options of one step (gamera_no_advance_options after the
+// consistency checks of cubedsphere_step.c:23-47) and of the Earth boundary
+// context (cubedsphere_earth_boundary.h:13-24).
+struct StepOptions {
+  // sweep (gamera_no_sweep_options): reconstruction, fluxes, HOGS
+  double gamma, rho_floor, p_floor, pdm, hogs_h, hogs_m, c;
+  int boris, hogs;
+  // edge EMF (gamera_no_emf_options)
+  double emf_rho_floor, emf_pdm, emf_diff, emf_c, emf_cfl;
+  int emf_boris;
+  // predictor and update (gamera_no_update_options)
+  double upd_gamma, upd_rho_floor, upd_p_floor, upd_c;
+  int upd_boris;
+  // dt rule of the problem driver (gamera_no_timestep_options)
+  double ts_gamma, ts_rho_floor, ts_p_floor, ts_cfl, ts_c;
+  int ts_boris;
+  // Earth wall (startup gamma, context floors and cfl)
+  double wall_gamma, wall_rho_floor, wall_p_floor, wall_cfl;
+  int strict_wall;
+};
+
+
+// Which boundary work runs on the device inside a step.
+struct StepBoundary {
+  bool outflow = false;     // gamera_cs_fill_radial_outflow_cells
+  bool earth_cells = false; // gamera_cs_earth_fill_radial_ghosts
+  bool inner_trap = false;  // gamera_cs_earth_inner_fluid_flux
+  bool earth_emf = false;   // gamera_cs_earth_no_mi_emf
+};
+
 struct Problem {
   gcs_problem_config cfg;
+  StepOptions opt;
+  StepBoundary bnd;
   Layout L;
   double *planes = nullptr;        // PLANES * L.total doubles
   uint8_t *bytes = nullptr;        // BYTE_PLANES * L.total
@@ -105,7 +138,6 @@ struct Problem {
   int *seam_cells = nullptr;       // active cells with a seam face
   int n_seam_cells = 0;
   WindSeries wind_series;
-  bool earth = true;               // Earth radial boundary active
   bool fidelity = true;            // reproduce ghost values of the storage
   // Stage timing: when `profiling`, enqueue_step records an event before
   // every stage into `stage_events` (one row of kMaxStages per step).
@@ -113,6 +145,7 @@ struct Problem {
   std::vector<hipEvent_t> stage_events;
   std::vector<int> stage_ids;      // stage id of each recorded event
   size_t stage_cursor = 0;
+  std::vector<double> staging;     // host buffer for plane transfers
 
   double *plane(int p) const { return planes + (long)p * L.total; }
   uint8_t *byte_plane(int q) const { return bytes + (long)q * L.total; }
@@ -128,13 +161,36 @@ enum Stage {
 };
 const char *stage_name(int stage);
 
+// Device buffers, stream and events of a problem whose layout L is set;
+// scalars are initialised. free_problem releases everything.
+int allocate_problem(Problem &P);
+void free_problem(Problem *P);
+int init_scalars(Problem &P);
+int read_scalars(Problem &P, StepScalars &s);
+// 0, or -1 with the first failing kernel stage and element in the reference's
+// MHD_FAILURE wording; updates the host clock mirror.
+int check_failure(Problem &P);
+
 // Setup: geometry, background, initial state and plans.
 int setup_problem(Problem &P);
 int build_plans(Problem &P);
+// Options and boundary of the serial Earth driver
+// (test_cubedsphere_dipole_background.c:405-449).
+void driver_options(Problem &P);
 // One step of the driver loop (dt rule + six-face step + clock) on
 // P.stream; with explicit_dt the step uses the dt, ratio and times already
 // in the device scalars and leaves the clock alone.
 int enqueue_step(Problem &P, bool explicit_dt);
+// The same six-face step in four phases, so that host callbacks can run
+// between them (cubedsphere_step.c:49-152): ghost fill (through the radial
+// boundary), face fluxes (through fluid_flux_sync), edge EMFs (through
+// edge_emf_sync), then seam EMF average, update and final seam sync. The
+// last phase swaps the time levels.
+enum Phase { PHASE_GHOSTS, PHASE_FLUX, PHASE_EMF, PHASE_UPDATE };
+int enqueue_phase(Problem &P, Phase phase);
+// Constants derived from uploaded geometry and background: recovery
+// pivots, edge-B solve, B0 normal and edge-tangential components.
+int derive_constants(Problem &P);
 // Copy one reference-layout field of one cube face to the host.
 int download_field(Problem &P, int face, int field, std::vector<double> &out);
 int download_valid(Problem &P, int face, int dir, std::vector<uint8_t> &out);

@@ -1,0 +1,761 @@
+// Drop-in C ABI of the reference's serial six-face step
+// (cubedsphere/cubedsphere_step.h:16-67): the four gamera_cs_advance_six*
+// entry points with the reference's prototypes and error behaviour, run on
+// the GPU. Declared in include/gamera_cs_dropin.h.
+//
+// The caller keeps owning grid, storage, background and plans in the
+// reference's host layout. Each (grid, storage) pair gets a device mirror on
+// its first call: the geometry and background are uploaded once (both must
+// stay unchanged while the mirror lives, as the reference requires of its
+// fixed grid and halo plan, cubedsphere_halo.c:561-570), the constants
+// derived from them are computed on the device, and the halo and seam plans
+// are built from the uploaded geometry. The caller's own halo plan is
+// accepted and not read: both plans are functions of the same grid.
+//
+// State coherence. By default every call uploads the state the step reads
+// and downloads everything the reference's step writes (all time levels,
+// predicted arrays, face fluxes, EMFs, counters), so host code between
+// calls (dt rule, output, pressure control) sees what it would see with the
+// reference. In resident mode (gcs_dropin_set_resident) the state stays on
+// the device between calls; gcs_dropin_sync_to_host and
+// gcs_dropin_mark_host_modified move it explicitly.
+//
+// Callbacks. The reference's own Earth callbacks
+// (cubedsphere_earth_boundary.h:26-58: gamera_cs_earth_fill_radial_ghosts,
+// gamera_cs_earth_inner_fluid_flux, gamera_cs_earth_no_mi_emf) are
+// recognised by address and run as device kernels. Any other callback, and
+// the Earth ones when their context needs something the device path does
+// not hold (MI coupling, a different background, more wind samples), runs
+// on the host at its place in the step: the state is downloaded, the
+// callback called, and the state uploaded again.
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gamera_cs_dropin.h"
+#include "gcs_device.hpp"
+#include "gcs_transfer.hpp"
+
+namespace gcs {
+namespace {
+
+using RadialFn = gamera_cs_radial_boundary_function;
+using SyncFn = int (*)(gamera_no_storage *, const gamera_no_grid *,
+                       const size_t *, const size_t *, void *);
+
+struct Callbacks {
+  RadialFn radial = nullptr;
+  SyncFn flux = nullptr;
+  SyncFn emf = nullptr;
+  bool registered = false;
+};
+
+struct Mirror {
+  const gamera_no_grid *grid = nullptr;
+  gamera_no_storage *storage = nullptr;
+  // Arrays the mirror was built from: a new allocation behind the same
+  // struct addresses makes a new mirror.
+  const void *arrays[GAMERA_CS_FACE_COUNT][3] = {};
+  const gamera_no_background_field *background = nullptr;
+  const void *background_arrays[GAMERA_CS_FACE_COUNT] = {};
+  bool has_background = false;
+  Problem *P = nullptr;
+  bool device_current = false;  // resident mode: device state is newest
+};
+
+std::mutex g_lock;
+std::vector<Mirror *> g_mirrors;
+Callbacks g_cb;
+bool g_resident = false;
+
+// cubedsphere_step.c:7-11
+bool same_value(double a, double b) {
+  const double scale = std::fmax(1.0, std::fmax(std::fabs(a), std::fabs(b)));
+  return std::isfinite(a) && std::isfinite(b) && std::fabs(a - b) <= 8e-15 * scale;
+}
+
+// cubedsphere_step.c:23-47: rejected before anything is written.
+bool valid_call(const gamera_no_grid *grid, gamera_no_storage *storage,
+                const size_t *active, size_t ghost, double ratio, double dt,
+                const gamera_no_advance_options *o,
+                const gamera_no_background_field *background,
+                RadialFn radial, double current_time, double previous_time) {
+  if (grid == nullptr || storage == nullptr || active == nullptr || o == nullptr ||
+      ghost < 4 || !std::isfinite(dt) || dt <= 0.0 ||
+      (radial != nullptr &&
+       (!std::isfinite(current_time) || !std::isfinite(previous_time))) ||
+      !std::isfinite(ratio) || active[1] != active[2] ||
+      o->cell_source != nullptr || o->indexed_cell_source != nullptr ||
+      o->update.use_background != o->stress.use_background ||
+      o->update.use_background != o->emf.use_background ||
+      (o->update.use_background && background == nullptr) ||
+      !std::isfinite(o->source_time) ||
+      !same_value(o->stress.gamma, o->update.gamma) ||
+      !same_value(o->stress.density_floor, o->update.density_floor) ||
+      !same_value(o->stress.pressure_floor, o->update.pressure_floor) ||
+      !same_value(o->emf.density_floor, o->update.density_floor) ||
+      o->stress.use_mhd != o->update.use_mhd ||
+      o->stress.use_boris != o->update.use_boris ||
+      o->emf.use_boris != o->update.use_boris ||
+      (o->update.use_boris &&
+       (!same_value(o->stress.light_speed, o->update.light_speed) ||
+        !same_value(o->emf.light_speed, o->update.light_speed))))
+    return false;
+  return true;
+}
+
+void resolve_callbacks() {
+  if (g_cb.registered) return;
+  g_cb.radial = reinterpret_cast<RadialFn>(
+      dlsym(RTLD_DEFAULT, "gamera_cs_earth_fill_radial_ghosts"));
+  g_cb.flux = reinterpret_cast<SyncFn>(
+      dlsym(RTLD_DEFAULT, "gamera_cs_earth_inner_fluid_flux"));
+  g_cb.emf = reinterpret_cast<SyncFn>(
+      dlsym(RTLD_DEFAULT, "gamera_cs_earth_no_mi_emf"));
+}
+
+// ---- mirrors --------------------------------------------------------------------
+
+bool extents_match(const gamera_no_grid *grid, const gamera_no_storage *storage,
+                   const Layout &L) {
+  const size_t ce[3] = {(size_t)L.NI, (size_t)L.NJ, (size_t)L.NK};
+  for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f) {
+    for (int a = 0; a < 3; ++a)
+      if (grid[f].cell_extent[a] != ce[a] || storage[f].cell_extent[a] != ce[a] ||
+          grid[f].vertex_extent[a] != ce[a] + 1)
+        return false;
+    for (int d = 0; d < 3; ++d) {
+      size_t fe[3], ee[3];
+      face_extent(L, d, fe);
+      edge_extent(L, d, ee);
+      for (int a = 0; a < 3; ++a)
+        if (grid[f].face[d].extent[a] != fe[a] || storage[f].face_extent[d][a] != fe[a] ||
+            grid[f].edge[d].extent[a] != ee[a] || storage[f].edge_extent[d][a] != ee[a])
+          return false;
+    }
+    if (grid[f].vertex == nullptr || grid[f].cell == nullptr ||
+        storage[f].conserved == nullptr || storage[f].old_conserved == nullptr ||
+        storage[f].predicted_conserved == nullptr || storage[f].cell_magnetic == nullptr ||
+        storage[f].old_cell_magnetic == nullptr ||
+        storage[f].predicted_cell_magnetic == nullptr)
+      return false;
+    for (int d = 0; d < 3; ++d)
+      if (grid[f].face[d].value == nullptr || grid[f].edge[d].value == nullptr ||
+          grid[f].edge[d].valid == nullptr || storage[f].face_flux[d] == nullptr ||
+          storage[f].old_face_flux[d] == nullptr ||
+          storage[f].predicted_face_flux[d] == nullptr ||
+          storage[f].edge_emf[d] == nullptr || storage[f].fluid_face_flux[d] == nullptr ||
+          storage[f].maxwell_face_flux[d] == nullptr)
+        return false;
+  }
+  return true;
+}
+
+void record_arrays(Mirror &m) {
+  for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f) {
+    m.arrays[f][0] = m.grid[f].vertex;
+    m.arrays[f][1] = m.grid[f].cell;
+    m.arrays[f][2] = m.storage[f].conserved;
+  }
+}
+
+bool arrays_unchanged(const Mirror &m) {
+  for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f)
+    if (m.arrays[f][0] != m.grid[f].vertex || m.arrays[f][1] != m.grid[f].cell ||
+        m.arrays[f][2] != m.storage[f].conserved)
+      return false;
+  return true;
+}
+
+int upload_geometry(Problem &P, const gamera_no_grid *grid) {
+  const Layout &L = P.L;
+  const size_t ce[3] = {(size_t)L.NI, (size_t)L.NJ, (size_t)L.NK};
+  const size_t ve[3] = {(size_t)L.NI + 1, (size_t)L.NJ + 1, (size_t)L.NK + 1};
+  const size_t per_cell = sizeof(gamera_no_cell_geometry) / sizeof(double);
+  const size_t per_face = sizeof(gamera_no_face_geometry) / sizeof(double);
+  const size_t per_edge = sizeof(gamera_no_edge_geometry) / sizeof(double);
+  static_assert(sizeof(gamera_no_face_geometry) == FACE_PLANES * sizeof(double),
+                "face geometry planes");
+  for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f) {
+    const gamera_no_grid &g = grid[f];
+    if (host_to_planes(P, P_VERT, 3, f, ve, g.vertex[0].value) != 0) return -1;
+    const double *cell = reinterpret_cast<const double *>(g.cell);
+    if (host_to_plane(P, P_VOL, f, ce, cell, per_cell) != 0) return -1;
+    for (int e = 0; e < 3; ++e) {
+      if (host_to_plane(P, P_CEN + e, f, ce, cell + 1 + e, per_cell) != 0) return -1;
+      const size_t cfl = offsetof(gamera_no_cell_geometry, cfl_length) / sizeof(double);
+      if (host_to_plane(P, P_CFL + e, f, ce, cell + cfl + e, per_cell) != 0) return -1;
+    }
+    for (int d = 0; d < 3; ++d) {
+      size_t fe[3], ee[3];
+      face_extent(L, d, fe);
+      edge_extent(L, d, ee);
+      const double *face = reinterpret_cast<const double *>(g.face[d].value);
+      for (int q = 0; q < (int)per_face; ++q)
+        if (host_to_plane(P, P_FACE + FACE_PLANES * d + q, f, fe, face + q, per_face) != 0)
+          return -1;
+      const double *edge = reinterpret_cast<const double *>(g.edge[d].value);
+      for (int q = 0; q < (int)per_edge; ++q)
+        if (host_to_plane(P, P_EDGE + EDGE_PLANES * d + q, f, ee, edge + q, per_edge) != 0)
+          return -1;
+      if (host_to_byte_plane(P, Q_VALID + d, f, ee, g.edge[d].valid) != 0) return -1;
+    }
+  }
+  return 0;
+}
+
+// Background planes; with no background they are zero, which gives the
+// reference's use_background = false arithmetic (it adds nothing).
+int upload_background(Problem &P, const gamera_no_background_field *bg) {
+  const Layout &L = P.L;
+  const size_t ce[3] = {(size_t)L.NI, (size_t)L.NJ, (size_t)L.NK};
+  std::vector<double> zero;
+  for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f) {
+    const gamera_no_background_field *b = bg ? &bg[f] : nullptr;
+    auto put = [&](int plane, int ncomp, const size_t e[3], const double *src) {
+      if (src != nullptr) return host_to_planes(P, plane, ncomp, f, e, src);
+      zero.assign(e[0] * e[1] * e[2] * ncomp, 0.0);
+      return host_to_planes(P, plane, ncomp, f, e, zero.data());
+    };
+    if (b && (b->cell_magnetic == nullptr || b->cell_force == nullptr)) {
+      set_error("background field arrays missing");
+      return -1;
+    }
+    if (put(P_B0C, 3, ce, b ? b->cell_magnetic[0].value : nullptr) != 0 ||
+        put(P_FORCE, 3, ce, b ? b->cell_force[0].value : nullptr) != 0)
+      return -1;
+    for (int d = 0; d < 3; ++d) {
+      size_t fe[3], ee[3];
+      face_extent(L, d, fe);
+      edge_extent(L, d, ee);
+      if (b && (b->face_magnetic[d] == nullptr || b->face_flux[d] == nullptr ||
+                b->edge_magnetic[d] == nullptr)) {
+        set_error("background field arrays missing");
+        return -1;
+      }
+      if (put(P_B0F + 3 * d, 3, fe, b ? b->face_magnetic[d][0].value : nullptr) != 0 ||
+          put(P_B0FLUX + d, 1, fe, b ? b->face_flux[d] : nullptr) != 0 ||
+          put(P_B0E + 3 * d, 3, ee, b ? b->edge_magnetic[d][0].value : nullptr) != 0)
+        return -1;
+    }
+  }
+  return 0;
+}
+
+// The arrays one six-face step reads or writes, in both directions.
+int state_to_device(Problem &P, const gamera_no_storage *st) {
+  const Layout &L = P.L;
+  const size_t ce[3] = {(size_t)L.NI, (size_t)L.NJ, (size_t)L.NK};
+  const int cur = P.level, old = 1 - P.level;
+  for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f) {
+    const gamera_no_storage &s = st[f];
+    if (host_to_planes(P, plane_u(cur, 0), 5, f, ce, s.conserved) != 0 ||
+        host_to_planes(P, plane_u(old, 0), 5, f, ce, s.old_conserved) != 0 ||
+        host_to_planes(P, P_UPRED, 5, f, ce, s.predicted_conserved) != 0 ||
+        host_to_planes(P, plane_b(cur, 0), 3, f, ce, s.cell_magnetic[0].value) != 0 ||
+        host_to_planes(P, plane_b(old, 0), 3, f, ce, s.old_cell_magnetic[0].value) != 0 ||
+        host_to_planes(P, P_BPRED, 3, f, ce, s.predicted_cell_magnetic[0].value) != 0)
+      return -1;
+    for (int d = 0; d < 3; ++d) {
+      size_t fe[3], ee[3];
+      face_extent(L, d, fe);
+      edge_extent(L, d, ee);
+      if (host_to_plane(P, plane_phi(cur, d), f, fe, s.face_flux[d], 1) != 0 ||
+          host_to_plane(P, plane_phi(old, d), f, fe, s.old_face_flux[d], 1) != 0 ||
+          host_to_plane(P, P_PHIPRED + d, f, fe, s.predicted_face_flux[d], 1) != 0 ||
+          host_to_plane(P, P_EMF + d, f, ee, s.edge_emf[d], 1) != 0 ||
+          host_to_planes(P, P_FLUID + 5 * d, 5, f, fe, s.fluid_face_flux[d]) != 0 ||
+          host_to_planes(P, P_MAXWELL + 3 * d, 3, f, fe,
+                         s.maxwell_face_flux[d][0].value) != 0)
+        return -1;
+    }
+  }
+  return 0;
+}
+
+int state_to_host(Problem &P, gamera_no_storage *st) {
+  const Layout &L = P.L;
+  const size_t ce[3] = {(size_t)L.NI, (size_t)L.NJ, (size_t)L.NK};
+  const int cur = P.level, old = 1 - P.level;
+  GCS_HIP(hipStreamSynchronize(P.stream));
+  for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f) {
+    gamera_no_storage &s = st[f];
+    if (planes_to_host(P, plane_u(cur, 0), 5, f, ce, s.conserved) != 0 ||
+        planes_to_host(P, plane_u(old, 0), 5, f, ce, s.old_conserved) != 0 ||
+        planes_to_host(P, P_UPRED, 5, f, ce, s.predicted_conserved) != 0 ||
+        planes_to_host(P, plane_b(cur, 0), 3, f, ce, s.cell_magnetic[0].value) != 0 ||
+        planes_to_host(P, plane_b(old, 0), 3, f, ce, s.old_cell_magnetic[0].value) != 0 ||
+        planes_to_host(P, P_BPRED, 3, f, ce, s.predicted_cell_magnetic[0].value) != 0)
+      return -1;
+    for (int d = 0; d < 3; ++d) {
+      size_t fe[3], ee[3];
+      face_extent(L, d, fe);
+      edge_extent(L, d, ee);
+      if (plane_to_host(P, plane_phi(cur, d), f, fe, s.face_flux[d], 1) != 0 ||
+          plane_to_host(P, plane_phi(old, d), f, fe, s.old_face_flux[d], 1) != 0 ||
+          plane_to_host(P, P_PHIPRED + d, f, fe, s.predicted_face_flux[d], 1) != 0 ||
+          plane_to_host(P, P_EMF + d, f, ee, s.edge_emf[d], 1) != 0 ||
+          planes_to_host(P, P_FLUID + 5 * d, 5, f, fe, s.fluid_face_flux[d]) != 0 ||
+          planes_to_host(P, P_MAXWELL + 3 * d, 3, f, fe,
+                         s.maxwell_face_flux[d][0].value) != 0)
+        return -1;
+    }
+  }
+  return 0;
+}
+
+void release(Mirror *m) {
+  if (m == nullptr) return;
+  free_problem(m->P);
+  delete m;
+}
+
+Mirror *build_mirror(const gamera_no_grid *grid, gamera_no_storage *storage,
+                     const size_t active[3], size_t ghost,
+                     const gamera_no_background_field *background) {
+  if (ghost != 4) {
+    set_error("the device step is built for ghost = 4");
+    return nullptr;
+  }
+  if (active[0] < 2 || active[1] < 9) {
+    set_error("grid needs at least 2 radial and 9 angular active cells");
+    return nullptr;
+  }
+  const long cells = (long)(active[0] + 9) * (active[1] + 9) *
+                     ((active[1] + 9 + 7) / 8 * 8) * 6;
+  if (cells >= (1L << 31)) {
+    set_error("grid too large for 32-bit indices");
+    return nullptr;
+  }
+  Mirror *m = new Mirror();
+  m->grid = grid;
+  m->storage = storage;
+  m->P = new Problem();
+  Problem &P = *m->P;
+  std::memset(&P.cfg, 0, sizeof(P.cfg));
+  P.L = make_layout((int)active[0], (int)active[1], (int)ghost);
+  if (!extents_match(grid, storage, P.L)) {
+    set_error("grid/storage extents do not match active + 2 ghost");
+    release(m);
+    return nullptr;
+  }
+  P.bnd = StepBoundary();  // plans cover every non-physical face
+  P.fidelity = true;
+  if (allocate_problem(P) != 0 || upload_geometry(P, grid) != 0 ||
+      upload_background(P, background) != 0 || derive_constants(P) != 0 ||
+      build_plans(P) != 0) {
+    release(m);
+    return nullptr;
+  }
+  m->background = background;
+  m->has_background = background != nullptr;
+  for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f)
+    m->background_arrays[f] = background ? background[f].cell_magnetic : nullptr;
+  record_arrays(*m);
+  return m;
+}
+
+Mirror *find(const gamera_no_grid *grid, const gamera_no_storage *storage) {
+  for (Mirror *m : g_mirrors)
+    if (m->grid == grid && m->storage == storage) return m;
+  return nullptr;
+}
+
+Mirror *mirror_for(const gamera_no_grid *grid, gamera_no_storage *storage,
+                   const size_t active[3], size_t ghost,
+                   const gamera_no_background_field *background) {
+  Mirror *m = find(grid, storage);
+  if (m != nullptr) {
+    const Layout &L = m->P->L;
+    const bool same_shape = (size_t)L.nr == active[0] && (size_t)L.n == active[1] &&
+                            (size_t)L.ghost == ghost && arrays_unchanged(*m) &&
+                            extents_match(grid, storage, L);
+    if (!same_shape) {
+      for (size_t n = 0; n < g_mirrors.size(); ++n)
+        if (g_mirrors[n] == m) g_mirrors.erase(g_mirrors.begin() + (long)n);
+      release(m);
+      m = nullptr;
+    }
+  }
+  if (m == nullptr) {
+    m = build_mirror(grid, storage, active, ghost, background);
+    if (m == nullptr) return nullptr;
+    g_mirrors.push_back(m);
+    return m;
+  }
+  bool bg_same = m->background == background && m->has_background == (background != nullptr);
+  for (int f = 0; bg_same && background && f < GAMERA_CS_FACE_COUNT; ++f)
+    bg_same = m->background_arrays[f] == background[f].cell_magnetic;
+  if (!bg_same) {
+    if (upload_background(*m->P, background) != 0 || derive_constants(*m->P) != 0)
+      return nullptr;
+    m->background = background;
+    m->has_background = background != nullptr;
+    for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f)
+      m->background_arrays[f] = background ? background[f].cell_magnetic : nullptr;
+  }
+  return m;
+}
+
+// ---- one call -------------------------------------------------------------------
+
+void set_options(Problem &P, const gamera_no_advance_options &o) {
+  StepOptions &s = P.opt;
+  s.gamma = o.stress.gamma;
+  s.rho_floor = o.stress.density_floor;
+  s.p_floor = o.stress.pressure_floor;
+  s.pdm = o.stress.pdm_coefficient;
+  s.hogs_h = o.stress.hydro_hogs_coefficient;
+  s.hogs_m = o.stress.magnetic_hogs_coefficient;
+  s.c = o.stress.light_speed;
+  s.boris = o.stress.use_boris;
+  s.hogs = o.stress.use_hogs;
+  s.emf_rho_floor = o.emf.density_floor;
+  s.emf_pdm = o.emf.pdm_coefficient;
+  s.emf_diff = o.emf.diffusion_coefficient;
+  s.emf_c = o.emf.light_speed;
+  s.emf_cfl = o.emf.cfl;
+  s.emf_boris = o.emf.use_boris;
+  s.upd_gamma = o.update.gamma;
+  s.upd_rho_floor = o.update.density_floor;
+  s.upd_p_floor = o.update.pressure_floor;
+  s.upd_c = o.update.light_speed;
+  s.upd_boris = o.update.use_boris;
+}
+
+// The Earth context can run on the device when it matches what the mirror
+// holds (cubedsphere_earth_boundary.c:9-14 for the rest of the checks).
+bool earth_on_device(const Mirror &m, const gamera_cs_earth_boundary *b,
+                     const gamera_no_background_field *background, bool need_no_mi) {
+  if (b == nullptr || b->startup == nullptr || b->wind == nullptr ||
+      b->background == nullptr || b->background != background ||
+      !m.has_background || !std::isfinite(b->density_floor) ||
+      !(b->density_floor > 0.0) || !std::isfinite(b->pressure_floor) ||
+      !(b->pressure_floor > 0.0))
+    return false;
+  if (need_no_mi && b->mi != nullptr) return false;
+  const gamera_solar_wind_series &w = *b->wind;
+  return w.count >= 1 && w.count <= GCS_MAX_WIND_SAMPLES && w.time != nullptr &&
+         w.state != nullptr;
+}
+
+void set_wind(Problem &P, const gamera_cs_earth_boundary &b) {
+  WindSeries &w = P.wind_series;
+  std::memset(&w, 0, sizeof(w));
+  const gamera_solar_wind_series &s = *b.wind;
+  w.count = (int)s.count;
+  w.linear = s.linear_interpolation;
+  for (size_t n = 0; n < s.count; ++n) {
+    w.time[n] = s.time[n];
+    const gamera_solar_wind_state &x = s.state[n];
+    w.state[n][0] = x.density;
+    w.state[n][1] = x.pressure;
+    for (int d = 0; d < 3; ++d) {
+      w.state[n][2 + d] = x.velocity[d];
+      w.state[n][5 + d] = x.magnetic[d];
+    }
+  }
+  for (int d = 0; d < 3; ++d) w.reference[d] = s.reference[d];
+  w.time_offset = s.time_offset;
+  w.by = s.by_coefficient;
+  w.bz = s.bz_coefficient;
+}
+
+int report_failure(Problem &P, const StepScalars &s) {
+  const long per = P.L.face_stride;
+  const long at = s.fail_index;
+  const int f = (int)(at / per);
+  const long r = at - f * per;
+  const char *rank = std::getenv("PMI_RANK");
+  if (rank == nullptr) rank = std::getenv("SLURM_PROCID");
+  char msg[200];
+  std::snprintf(msg, sizeof(msg),
+                "MHD_FAILURE rank=%s stage=%u face=%d local=%ld,%ld,%ld",
+                rank ? rank : "unknown", s.fail_stage, f, r / P.L.di(),
+                (r % P.L.di()) / P.L.SK, r % P.L.SK);
+  std::fprintf(stderr, "%s\n", msg);
+  set_error(msg);
+  return -1;
+}
+
+int advance(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+            gamera_no_storage storage[GAMERA_CS_FACE_COUNT], const size_t active[3],
+            size_t ghost, double ratio, double dt, const gamera_no_advance_options *o,
+            const gamera_no_background_field background[GAMERA_CS_FACE_COUNT],
+            RadialFn radial, double current_time, double previous_time,
+            void *boundary_context) {
+  if (!valid_call(grid, storage, active, ghost, ratio, dt, o, background, radial,
+                  current_time, previous_time)) {
+    set_error("invalid arguments (cubedsphere_step.c:23-47)");
+    return -1;
+  }
+  if (!o->update.use_mhd) {
+    set_error("hydrodynamic-only steps (use_mhd = false) are not built on the device");
+    return -1;
+  }
+  std::lock_guard<std::mutex> guard(g_lock);
+  resolve_callbacks();
+  const gamera_no_background_field *bg = o->update.use_background ? background : nullptr;
+  Mirror *m = mirror_for(grid, storage, active, ghost, bg);
+  if (m == nullptr) return -1;
+  Problem &P = *m->P;
+  set_options(P, *o);
+
+  // Boundary work: device kernels for the reference's Earth callbacks.
+  const auto *rb = static_cast<const gamera_cs_earth_boundary *>(boundary_context);
+  const auto *eb = static_cast<const gamera_cs_earth_boundary *>(o->edge_emf_context);
+  StepBoundary &bd = P.bnd;
+  bd = StepBoundary();
+  bool host_radial = false, host_flux = false, host_emf = false;
+  const gamera_cs_earth_boundary *wall = nullptr;
+  if (radial != nullptr) {
+    if (radial == g_cb.radial && earth_on_device(*m, rb, bg, true)) {
+      bd.earth_cells = true;  // rewrites every radial ghost the outflow fill writes
+      wall = rb;
+    } else {
+      bd.outflow = true;
+      host_radial = true;
+    }
+  } else {
+    bd.outflow = true;
+  }
+  if (o->fluid_flux_sync != nullptr) {
+    if (o->fluid_flux_sync == g_cb.flux) bd.inner_trap = true;
+    else host_flux = true;
+  }
+  if (o->edge_emf_sync != nullptr) {
+    if (o->edge_emf_sync == g_cb.emf && earth_on_device(*m, eb, bg, true)) {
+      bd.earth_emf = true;
+      if (wall == nullptr) wall = eb;
+    } else {
+      host_emf = true;
+    }
+  }
+  if (bd.earth_cells && bd.earth_emf && rb != eb) {
+    // Two different Earth contexts: keep the EMF one on the host.
+    bd.earth_emf = false;
+    host_emf = true;
+  }
+  if (wall != nullptr) {
+    StepOptions &s = P.opt;
+    s.wall_gamma = wall->startup->gamma;
+    s.wall_rho_floor = wall->density_floor;
+    s.wall_p_floor = wall->pressure_floor;
+    s.wall_cfl = wall->cfl;
+    s.strict_wall = wall->strict_inner_gas_wall != 0;
+    set_wind(P, *wall);
+  }
+
+  // Scalars of this call.
+  StepScalars sc;
+  if (read_scalars(P, sc) != 0) return -1;
+  sc.dt = dt;
+  sc.ratio = ratio;
+  sc.time = current_time;
+  sc.previous_time = previous_time;
+  sc.wall_time = wall ? wall->step_time : current_time;
+  sc.wall_dt = wall ? wall->step_dt : dt;
+  sc.failed = 0;
+  sc.fail_stage = 0;
+  sc.fail_index = 0;
+  for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f) {
+    sc.emergency_count[f] = 0;
+    sc.emergency_speed_bits[f] = 0;
+  }
+  GCS_HIP(hipMemcpy(P.scalars, &sc, sizeof(sc), hipMemcpyHostToDevice));
+  if (bd.earth_emf) {
+    // cubedsphere_earth_boundary.c:304, :387
+    if (!std::isfinite(sc.wall_time) || !(sc.wall_dt > 0.0) || !(P.opt.wall_cfl > 0.0)) {
+      set_error("Earth EMF context needs finite step_time, step_dt > 0 and cfl > 0");
+      return -1;
+    }
+  }
+
+  if (!(g_resident && m->device_current) && state_to_device(P, storage) != 0)
+    return -1;
+  m->device_current = true;
+  const size_t lower[3] = {ghost, ghost, ghost};
+  const size_t upper[3] = {ghost + active[0], ghost + active[1], ghost + active[2]};
+  auto host_round = [&](auto &&call) -> int {
+    if (state_to_host(P, storage) != 0) return -1;
+    const int status = call();
+    if (state_to_device(P, storage) != 0) return -1;
+    return status;
+  };
+  auto check = [&]() -> int {
+    StepScalars now;
+    if (read_scalars(P, now) != 0) return -1;
+    return now.failed ? report_failure(P, now) : 0;
+  };
+
+  if (enqueue_phase(P, PHASE_GHOSTS) != 0) return -1;
+  if (host_radial) {
+    if (check() != 0) return -1;
+    if (host_round([&] {
+          return radial(grid, storage, active, ghost, current_time, previous_time,
+                        boundary_context);
+        }) != 0)
+      return -1;
+  }
+  if (enqueue_phase(P, PHASE_FLUX) != 0) return -1;
+  if (host_flux) {
+    if (check() != 0) return -1;
+    if (host_round([&] {
+          for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f)
+            if (o->fluid_flux_sync(&storage[f], &grid[f], lower, upper,
+                                   o->fluid_flux_context) != 0)
+              return -1;
+          return 0;
+        }) != 0)
+      return -1;
+  }
+  if (enqueue_phase(P, PHASE_EMF) != 0) return -1;
+  if (host_emf) {
+    if (check() != 0) return -1;
+    if (host_round([&] {
+          for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f)
+            if (o->edge_emf_sync(&storage[f], &grid[f], lower, upper,
+                                 o->edge_emf_context) != 0)
+              return -1;
+          return 0;
+        }) != 0)
+      return -1;
+  }
+  if (enqueue_phase(P, PHASE_UPDATE) != 0) return -1;
+
+  StepScalars after;
+  if (read_scalars(P, after) != 0) return -1;
+  const int status = after.failed ? report_failure(P, after) : 0;
+  // nonorthogonal_sweep.c:541-544: counters accumulate in the storage.
+  for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f) {
+    double speed;
+    std::memcpy(&speed, &after.emergency_speed_bits[f], sizeof(speed));
+    storage[f].emergency_diffusion_face_count += (size_t)after.emergency_count[f];
+    storage[f].emergency_diffusion_max_interface_speed =
+        std::fmax(storage[f].emergency_diffusion_max_interface_speed, speed);
+  }
+  if (!g_resident || status != 0) {
+    if (state_to_host(P, storage) != 0) return -1;
+  }
+  return status;
+}
+
+}  // namespace
+}  // namespace gcs
+
+extern "C" {
+
+int gamera_cs_advance_six_with_background_planned(
+    const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+    gamera_no_storage storage[GAMERA_CS_FACE_COUNT], const size_t active[3],
+    size_t ghost, double predictor_ratio, double dt,
+    const gamera_no_advance_options *options,
+    const gamera_no_background_field background[GAMERA_CS_FACE_COUNT],
+    const gamera_cs_halo_plan *halo_plan,
+    gamera_cs_radial_boundary_function radial_boundary, double current_time,
+    double previous_time, void *boundary_context) {
+  (void)halo_plan;
+  return gcs::advance(grid, storage, active, ghost, predictor_ratio, dt, options,
+                      background, radial_boundary, current_time, previous_time,
+                      boundary_context);
+}
+
+int gamera_cs_advance_six_with_background(
+    const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+    gamera_no_storage storage[GAMERA_CS_FACE_COUNT], const size_t active[3],
+    size_t ghost, double predictor_ratio, double dt,
+    const gamera_no_advance_options *options,
+    const gamera_no_background_field background[GAMERA_CS_FACE_COUNT],
+    gamera_cs_radial_boundary_function radial_boundary, double current_time,
+    double previous_time, void *boundary_context) {
+  return gcs::advance(grid, storage, active, ghost, predictor_ratio, dt, options,
+                      background, radial_boundary, current_time, previous_time,
+                      boundary_context);
+}
+
+int gamera_cs_advance_six_with_boundary(
+    const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+    gamera_no_storage storage[GAMERA_CS_FACE_COUNT], const size_t active[3],
+    size_t ghost, double predictor_ratio, double dt,
+    const gamera_no_advance_options *options,
+    gamera_cs_radial_boundary_function radial_boundary, double current_time,
+    double previous_time, void *boundary_context) {
+  return gcs::advance(grid, storage, active, ghost, predictor_ratio, dt, options,
+                      nullptr, radial_boundary, current_time, previous_time,
+                      boundary_context);
+}
+
+int gamera_cs_advance_six(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+                          gamera_no_storage storage[GAMERA_CS_FACE_COUNT],
+                          const size_t active[3], size_t ghost,
+                          double predictor_ratio, double dt,
+                          const gamera_no_advance_options *options) {
+  return gcs::advance(grid, storage, active, ghost, predictor_ratio, dt, options,
+                      nullptr, nullptr, 0.0, 0.0, nullptr);
+}
+
+void gcs_dropin_register_earth_callbacks(
+    gamera_cs_radial_boundary_function radial,
+    int (*inner_fluid_flux)(gamera_no_storage *, const gamera_no_grid *,
+                            const size_t *, const size_t *, void *),
+    int (*no_mi_emf)(gamera_no_storage *, const gamera_no_grid *, const size_t *,
+                     const size_t *, void *)) {
+  std::lock_guard<std::mutex> guard(gcs::g_lock);
+  gcs::g_cb.radial = radial;
+  gcs::g_cb.flux = inner_fluid_flux;
+  gcs::g_cb.emf = no_mi_emf;
+  gcs::g_cb.registered = radial != nullptr || inner_fluid_flux != nullptr ||
+                         no_mi_emf != nullptr;
+}
+
+int gcs_dropin_set_resident(int on) {
+  std::lock_guard<std::mutex> guard(gcs::g_lock);
+  gcs::g_resident = on != 0;
+  return 0;
+}
+
+int gcs_dropin_sync_to_host(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+                            gamera_no_storage storage[GAMERA_CS_FACE_COUNT]) {
+  std::lock_guard<std::mutex> guard(gcs::g_lock);
+  gcs::Mirror *m = gcs::find(grid, storage);
+  if (m == nullptr) {
+    gcs::set_error("no device mirror for this grid/storage");
+    return -1;
+  }
+  return m->device_current ? gcs::state_to_host(*m->P, storage) : 0;
+}
+
+int gcs_dropin_mark_host_modified(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+                                  gamera_no_storage storage[GAMERA_CS_FACE_COUNT]) {
+  std::lock_guard<std::mutex> guard(gcs::g_lock);
+  gcs::Mirror *m = gcs::find(grid, storage);
+  if (m != nullptr) m->device_current = false;
+  return 0;
+}
+
+int gcs_dropin_release(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+                       gamera_no_storage storage[GAMERA_CS_FACE_COUNT]) {
+  std::lock_guard<std::mutex> guard(gcs::g_lock);
+  for (size_t n = 0; n < gcs::g_mirrors.size(); ++n)
+    if (gcs::g_mirrors[n]->grid == grid && gcs::g_mirrors[n]->storage == storage) {
+      gcs::release(gcs::g_mirrors[n]);
+      gcs::g_mirrors.erase(gcs::g_mirrors.begin() + (long)n);
+      return 0;
+    }
+  return 0;
+}
+
+int gcs_dropin_mirror_count(void) {
+  std::lock_guard<std::mutex> guard(gcs::g_lock);
+  return (int)gcs::g_mirrors.size();
+}
+
+}  // extern "C"

@@ -223,7 +223,7 @@ int build_plans(Problem &P) {
             for (int a = 0; a < 3; ++a)
               if (c3[a] < g || c3[a] >= g + act[a] + (dir == a)) physical = false;
             if (physical) continue;
-            if (P.earth && (i < g || i >= outer + (dir == 0))) continue;
+            if (P.bnd.earth_cells && (i < g || i >= outer + (dir == 0))) continue;
             const long at = L.at(f, i, j, k);
             const double x[3] = {fcen[(3 * dir) * L.total + at],
                                  fcen[(3 * dir + 1) * L.total + at],

@@ -299,7 +299,23 @@ __global__ void k_cell_geometry(Ctx c) {
     const double *ua = c.p(P_SCRATCH_AREA + d);
     c.p(P_CFL + d)[at] = vol / fmax(ua[at], ua[at + step[d]]);
   }
-  // Response of the moment formula to the three unit uniform fields.
+}
+
+// Calibration of the cell-field recovery (cubedsphere_recovery.c:12-98): the
+// response of the moment formula to the three unit uniform fields, inverted
+// by Gauss-Jordan elimination with partial pivoting. The pivots and
+// multipliers depend on geometry only and are kept, so the per-step
+// recovery replays the elimination on the moments alone.
+__global__ void k_recovery_constants(Ctx c) {
+  int f, i, j, k;
+  const Layout &L = c.L;
+  if (!box_index(blockIdx.x * (long)blockDim.x + threadIdx.x, L.NI, L.NJ, L.NK,
+                 f, i, j, k))
+    return;
+  const long at = L.at(f, i, j, k);
+  const long step[3] = {L.di(), (long)L.SK, 1};
+  const double vol = c.p(P_VOL)[at];
+  const V3 cen = {{c.p(P_CEN)[at], c.p(P_CEN + 1)[at], c.p(P_CEN + 2)[at]}};
   double fc[6][3], resp[3][3];
   for (int d = 0; d < 3; ++d)
     for (int s = 0; s < 2; ++s)
@@ -804,6 +820,40 @@ __global__ void k_subtract_background(Ctx c, int d) {
 
 // Recover cell B of every cell of level `level` from its face fluxes
 // (defined with the step kernels).
+// B0 normal component on each face (flux over area, nonorthogonal_sweep.c:
+// 470-474) and B0 edge components along the edge tangents (:719-725), from
+// background planes that were uploaded rather than computed here.
+__global__ void k_background_derived(Ctx c) {
+  int f, i, j, k;
+  const Layout &L = c.L;
+  if (!box_index(blockIdx.x * (long)blockDim.x + threadIdx.x, L.NI + 1, L.NJ + 1,
+                 L.NK + 1, f, i, j, k))
+    return;
+  const long at = L.at(f, i, j, k);
+  const int ext[3] = {L.NI, L.NJ, L.NK};
+  const int pos[3] = {i, j, k};
+  for (int d = 0; d < 3; ++d) {
+    bool face = true, edge = true;
+    for (int a = 0; a < 3; ++a) {
+      if (pos[a] > ext[a] - (a == d ? 0 : 1)) face = false;
+      if (pos[a] > ext[a] - (a == d ? 1 : 0)) edge = false;
+    }
+    if (face)
+      c.p(P_B0N + d)[at] = c.p(P_B0FLUX + d)[at] / c.p(P_FACE + FACE_PLANES * d + FO_AREA)[at];
+    if (edge) {
+      const int eb = P_EDGE + EDGE_PLANES * d;
+      double r[3], t1[3], t2[3];
+      for (int e = 0; e < 3; ++e) {
+        r[e] = c.p(P_B0E + 3 * d + e)[at];
+        t1[e] = c.p(eb + EO_T1 + e)[at];
+        t2[e] = c.p(eb + EO_T2 + e)[at];
+      }
+      c.p(P_B0ET + 2 * d)[at] = dot3(r, t1);
+      c.p(P_B0ET + 2 * d + 1)[at] = dot3(r, t2);
+    }
+  }
+}
+
 int launch_recover_all(Problem &P, int phi_first, int b_first);
 
 static unsigned blocks_for(long n, int threads) {
@@ -901,6 +951,7 @@ int setup_problem(Problem &P) {
   for (int d = 0; d < 3; ++d)
     k_face_geometry<<<blocks_for(nface(d), 64), 64, 0, s>>>(c, d);
   k_cell_geometry<<<blocks_for(nc, 64), 64, 0, s>>>(c);
+  k_recovery_constants<<<blocks_for(nc, 64), 64, 0, s>>>(c);
   for (int d = 0; d < 3; ++d)
     k_edge_geometry<<<blocks_for(nedge(d), T), T, 0, s>>>(c, d);
   for (int d = 0; d < 3; ++d)
@@ -951,6 +1002,23 @@ int setup_problem(Problem &P) {
   }
   P.level = 0;
   P.host_dirty = true;
+  return 0;
+}
+
+int derive_constants(Problem &P) {
+  const Layout &L = P.L;
+  Ctx c{L, P.planes, P.bytes, P.scalars};
+  hipStream_t s = P.stream;
+  const long nc = 6L * L.NI * L.NJ * L.NK;
+  const long nv = 6L * (L.NI + 1) * (L.NJ + 1) * (L.NK + 1);
+  k_recovery_constants<<<blocks_for(nc, 64), 64, 0, s>>>(c);
+  for (int d = 0; d < 3; ++d) {
+    const long ne = 6L * (L.NI + (d != 0)) * (L.NJ + (d != 1)) * (L.NK + (d != 2));
+    k_edge_solve<<<blocks_for(ne, 256), 256, 0, s>>>(c, d);
+  }
+  k_background_derived<<<blocks_for(nv, 256), 256, 0, s>>>(c);
+  GCS_HIP(hipGetLastError());
+  GCS_HIP(hipStreamSynchronize(s));
   return 0;
 }
 

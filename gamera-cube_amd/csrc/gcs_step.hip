@@ -33,14 +33,7 @@ namespace gcs {
 
 namespace {
 
-struct Options {  // validated on the host (cubedsphere_step.c:23-47)
-  double gamma, rho_floor, p_floor, pdm, hogs_h, hogs_m, c;
-  double emf_rho_floor, emf_pdm, emf_diff, emf_cfl;
-  int boris, hogs;
-  // Earth boundary
-  double wall_rho_floor, wall_p_floor, wall_cfl;
-  int strict_wall;
-};
+
 
 struct K {
   Layout L;
@@ -48,7 +41,7 @@ struct K {
   uint8_t *by;
   StepScalars *sc;
   int cur, old;  // time-level slots of this step
-  Options o;
+  StepOptions o;
   __device__ double *p(int q) const { return pl + (long)q * L.total; }
   __device__ const uint8_t *b(int q) const { return by + (long)q * L.total; }
   __device__ void fail(unsigned stage, long index) const {
@@ -94,19 +87,19 @@ __global__ void k_cfl(K c) {
     double u[5];
     for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
     Prim w;
-    if (!cons_to_prim(u, c.o.gamma, c.o.rho_floor, c.o.p_floor, w)) {
+    if (!cons_to_prim(u, c.o.ts_gamma, c.o.ts_rho_floor, c.o.ts_p_floor, w)) {
       c.fail(1, at);
     } else {
-      const double cs = sqrt(c.o.gamma * w.p / w.rho);
+      const double cs = sqrt(c.o.ts_gamma * w.p / w.rho);
       double m[3];
       for (int d = 0; d < 3; ++d) m[d] = c.p(plane_b(c.cur, d))[at] + c.p(P_B0C + d)[at];
       double va = sqrt(dot3(m, m)) / sqrt(w.rho);
-      if (c.o.boris) va = c.o.c * va / hypot(c.o.c, va);
+      if (c.o.ts_boris) va = c.o.ts_c * va / hypot(c.o.ts_c, va);
       const double speed = sqrt(dot3(w.v, w.v)) + hypot(cs, va);
       double len = c.p(P_CFL)[at];
       len = fmin(len, c.p(P_CFL + 1)[at]);
       len = fmin(len, c.p(P_CFL + 2)[at]);
-      const double cand = c.o.emf_cfl * len / speed;
+      const double cand = c.o.ts_cfl * len / speed;
       if (!isfinite(cand) || cand <= 0.0) c.fail(2, at);
       else best = cand;
     }
@@ -134,6 +127,10 @@ __global__ void k_dt(K c, double fraction) {
     s->ratio = 0.5 * s->dt / s->previous_dt;
     s->previous_time = s->time - s->previous_dt;
   }
+  // The driver sets the Earth context's step clock to the step's own
+  // (test_cubedsphere_dipole_background.c:677-678).
+  s->wall_time = s->time;
+  s->wall_dt = s->dt;
 }
 
 __global__ void k_clock(K c) {
@@ -385,7 +382,7 @@ __global__ void k_earth_cells(K c, WindSeries w) {
     double u[5];
     for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(level, q))[s];
     Prim pw;
-    if (!cons_to_prim(u, c.o.gamma, c.o.wall_rho_floor, c.o.wall_p_floor, pw) ||
+    if (!cons_to_prim(u, c.o.wall_gamma, c.o.wall_rho_floor, c.o.wall_p_floor, pw) ||
         !(rad > 0.0)) {
       c.fail(11, t);
       return;
@@ -406,14 +403,14 @@ __global__ void k_earth_cells(K c, WindSeries w) {
       }
       double vn = 0.0;
       for (int d = 0; d < 3; ++d) vn += pw.v[d] * x[d] / rad;
-      const double cs = sqrt(c.o.gamma * pw.p / pw.rho);
+      const double cs = sqrt(c.o.wall_gamma * pw.p / pw.rho);
       if (vn < cs)
         for (int d = 0; d < 3; ++d) pw.v[d] += (cs - vn) * x[d] / rad;
       pw.rho = wt * sw[0] + (1.0 - wt) * pw.rho;
       pw.p = wt * sw[1] + (1.0 - wt) * pw.p;
       for (int d = 0; d < 3; ++d) pw.v[d] = wt * sw[2 + d] + (1.0 - wt) * pw.v[d];
     }
-    if (!prim_to_cons(pw, c.o.gamma, c.o.wall_rho_floor, c.o.wall_p_floor, u)) {
+    if (!prim_to_cons(pw, c.o.wall_gamma, c.o.wall_rho_floor, c.o.wall_p_floor, u)) {
       c.fail(13, t);
       return;
     }
@@ -555,7 +552,7 @@ __global__ void k_predict(K c) {
     uo[q] = c.p(plane_u(c.old, q))[at];
     uc[q] = c.p(plane_u(c.cur, q))[at];
   }
-  if (!predict_cell(uo, uc, r, c.o.gamma, c.o.rho_floor, c.o.p_floor, up)) {
+  if (!predict_cell(uo, uc, r, c.o.upd_gamma, c.o.upd_rho_floor, c.o.upd_p_floor, up)) {
     c.fail(31, at);
     return;
   }
@@ -596,7 +593,7 @@ __global__ void __launch_bounds__(128) k_face_flux(K c, int d) {
   if (!box(tid(), lo, hi, f, i, j, k)) return;
   const long at = L.at(f, i, j, k);
   const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
-  const Options &o = c.o;
+  const StepOptions &o = c.o;
   double vol[8];
   Prim ws[8];
   for (int n = 0; n < 8; ++n) {
@@ -752,7 +749,7 @@ __global__ void __launch_bounds__(128) k_edge_emf(K c, int e) {
     return;
   }
   const long step[3] = {L.di(), (long)L.SK, 1};
-  const Options &o = c.o;
+  const StepOptions &o = c.o;
   const int eb = P_EDGE + EDGE_PLANES * e;
   // Edge velocity: area-weighted central8 of t1-face velocities along t2.
   double a1[8], v[3];
@@ -814,7 +811,7 @@ __global__ void __launch_bounds__(128) k_edge_emf(K c, int e) {
   }
   double emf;
   if (!edge_emf(v1, v2, bt[0], bt[1], jump, rho, c.p(eb + EO_LEN)[at], o.emf_diff,
-                o.boris, o.c, o.emf_cfl, c.sc->dt, emf)) {
+                o.emf_boris, o.emf_c, o.emf_cfl, c.sc->dt, emf)) {
     c.fail(55, at);
     return;
   }
@@ -836,7 +833,7 @@ __global__ void k_earth_emf_outer(K c, int d, WindSeries w) {
   double x[3];
   for (int m = 0; m < 3; ++m) x[m] = 0.5 * (c.p(P_VERT + m)[at] + c.p(P_VERT + m)[up]);
   double sw[8], wt;
-  if (!sample_wind(w, x, c.sc->time + 0.5 * c.sc->dt, sw, wt)) {
+  if (!sample_wind(w, x, c.sc->wall_time + 0.5 * c.sc->wall_dt, sw, wt)) {
     c.fail(61, at);
     return;
   }
@@ -887,7 +884,7 @@ __global__ void k_earth_emf_diffusive(K c, WindSeries w) {
     xj[m] = 0.5 * (c.p(P_VERT + m)[at] + c.p(P_VERT + m)[at + sj]);
     xk[m] = 0.5 * (c.p(P_VERT + m)[at] + c.p(P_VERT + m)[at + 1]);
   }
-  const double t = c.sc->time + 0.5 * c.sc->dt;
+  const double t = c.sc->wall_time + 0.5 * c.sc->wall_dt;
   double wj[8], wk[8], gj, gk;
   if (!(dl > 0.0) || !sample_wind(w, xj, t, wj, gj) || !sample_wind(w, xk, t, wk, gk)) {
     c.fail(63, at);
@@ -895,7 +892,7 @@ __global__ void k_earth_emf_diffusive(K c, WindSeries w) {
   }
   const double c2 = w.by * w.by + w.bz * w.bz;
   const double front = fmax(fabs(wj[2]), fabs(wk[2])) / sqrt(1.0 + c2);
-  const double speed = fmin(front, c.o.wall_cfl * dl / c.sc->dt);
+  const double speed = fmin(front, c.o.wall_cfl * dl / c.sc->wall_dt);
   c.p(P_EMF + 1)[at] += gj * speed * cj * dl;
   c.p(P_EMF + 2)[at] += gk * speed * ck * dl;
 }
@@ -928,7 +925,7 @@ __global__ void __launch_bounds__(128) k_update(K c) {
   const long at = L.at(f, i, j, k);
   const long si = L.di(), sj = L.SK;
   const long step[3] = {si, sj, 1};
-  const Options &o = c.o;
+  const StepOptions &o = c.o;
   const double dt = c.sc->dt;
   const double vol = c.p(P_VOL)[at];
   const double ivol = 1.0 / vol;
@@ -982,20 +979,20 @@ __global__ void __launch_bounds__(128) k_update(K c) {
     c.fail(71, at);
     return;
   }
-  if (o.boris) {
+  if (o.upd_boris) {
     double btn[3], bto[3];
     for (int m = 0; m < 3; ++m) {
       const double b0 = c.p(P_B0C + m)[at];
       btn[m] = bn[m] + b0;
       bto[m] = c.p(plane_b(c.cur, m))[at] + b0;
     }
-    if (!boris_update(u, u0, btn, bto, rate, mr, o.c, dt, o.gamma, o.rho_floor,
-                      o.p_floor)) {
+    if (!boris_update(u, u0, btn, bto, rate, mr, o.upd_c, dt, o.upd_gamma,
+                      o.upd_rho_floor, o.upd_p_floor)) {
       c.fail(72, at);
       return;
     }
   } else {
-    maxwell_update(u, mr, dt, o.gamma, o.rho_floor, o.p_floor);
+    maxwell_update(u, mr, dt, o.upd_gamma, o.upd_rho_floor, o.upd_p_floor);
   }
   for (int q = 0; q < 5; ++q) c.p(plane_u(c.old, q))[at] = u[q];
   for (int m = 0; m < 3; ++m) c.p(plane_b(c.old, m))[at] = bn[m];
@@ -1046,26 +1043,50 @@ static K make_k(Problem &P) {
   c.sc = P.scalars;
   c.cur = P.level;
   c.old = 1 - P.level;
-  Options &o = c.o;
+  c.o = P.opt;
+  return c;
+}
+
+void driver_options(Problem &P) {
+  // test_cubedsphere_dipole_background.c:405-449 (floors 1e-6 / 1e-8, PDM 1,
+  // HOGS 0.25 / 0.25, Boris c = 10, EMF diffusion 0.5).
+  StepOptions &o = P.opt;
   const gcs_problem_config &cfg = P.cfg;
-  o.gamma = cfg.startup.gamma;
+  const double gamma = cfg.startup.gamma;
+  const int boris = cfg.use_boris != 0;
+  o.gamma = gamma;
   o.rho_floor = 1e-6;
   o.p_floor = 1e-8;
   o.pdm = 1.0;
   o.hogs_h = 0.25;
   o.hogs_m = 0.25;
   o.c = 10.0;
+  o.boris = boris;
+  o.hogs = 1;
   o.emf_rho_floor = 1e-6;
   o.emf_pdm = 1.0;
   o.emf_diff = 0.5;
+  o.emf_c = 10.0;
   o.emf_cfl = cfg.cfl;
-  o.boris = cfg.use_boris != 0;
-  o.hogs = 1;
+  o.emf_boris = boris;
+  o.upd_gamma = gamma;
+  o.upd_rho_floor = 1e-6;
+  o.upd_p_floor = 1e-8;
+  o.upd_c = 10.0;
+  o.upd_boris = boris;
+  o.ts_gamma = gamma;
+  o.ts_rho_floor = 1e-6;
+  o.ts_p_floor = 1e-8;
+  o.ts_cfl = cfg.cfl;
+  o.ts_c = 10.0;
+  o.ts_boris = boris;
+  o.wall_gamma = gamma;
   o.wall_rho_floor = 1e-6;
   o.wall_p_floor = 1e-8;
   o.wall_cfl = cfg.cfl;
   o.strict_wall = cfg.strict_inner_gas_wall != 0;
-  return c;
+  P.bnd.outflow = false;
+  P.bnd.earth_cells = P.bnd.inner_trap = P.bnd.earth_emf = true;
 }
 
 int launch_recover_all(Problem &P, int phi_first, int b_first) {
@@ -1095,7 +1116,7 @@ static void mark(Problem &P, int stage) {
   ++P.stage_cursor;
 }
 
-int enqueue_step(Problem &P, bool explicit_dt) {
+int enqueue_phase(Problem &P, Phase phase) {
   K c = make_k(P);
   const Layout &L = P.L;
   hipStream_t s = P.stream;
@@ -1104,102 +1125,124 @@ int enqueue_step(Problem &P, bool explicit_dt) {
   const long n_cells = 6L * L.NI * L.NJ * L.NK;
   const long n_box = 6L * (L.NI + 1) * (L.NJ + 1) * (L.NK + 1);
   const WindSeries &w = P.wind_series;
+  const StepBoundary &bd = P.bnd;
 
+  if (phase == PHASE_GHOSTS) {
+    // cubedsphere_step.c:51-65
+    mark(P, S_SYNC_PHI);
+    k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams, 1);
+    mark(P, S_HALO_CELLS);
+    k_halo_cells<<<nblocks(P.n_halo_cells, T), T, 0, s>>>(c, P.halo_cells, P.n_halo_cells);
+    if (bd.outflow) {
+      mark(P, S_OUTFLOW);
+      k_radial_outflow<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c);
+    }
+    mark(P, S_HALO_FACES);
+    k_halo_faces<<<nblocks(P.n_halo_faces, T), T, 0, s>>>(c, P.halo_faces, P.n_halo_faces);
+    if (bd.earth_cells) {
+      mark(P, S_EARTH_CELLS);
+      k_earth_cells<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c, w);
+      mark(P, S_EARTH_FACES);
+      for (int d = 0; d < 3; ++d) {
+        const long n_out = (L.NI + (d == 0)) - (L.ghost + L.nr + (d == 0));
+        const long n = 6L * (L.ghost + n_out) * (L.NJ + (d == 1)) * (L.NK + (d == 2));
+        k_earth_faces<<<nblocks(n, T), T, 0, s>>>(c, d, w);
+      }
+      mark(P, S_EARTH_RECOVER);
+      k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.cur, 0), plane_b(c.cur, 0), 1, 0);
+      k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.old, 0), plane_b(c.old, 0), 1, 0);
+    }
+  } else if (phase == PHASE_FLUX) {
+    // cubedsphere_step.c:67-98
+    mark(P, S_PREDICT);
+    k_predict<<<nblocks(n_box, T), T, 0, s>>>(c);
+    for (int d = 0; d < 3; ++d) {
+      const long n = 6L * (L.nr + (d == 0)) * (L.n + (d == 1)) * (L.n + (d == 2));
+      mark(P, S_FACE_FLUX0 + d);
+      k_face_flux<<<nblocks(n, 128), 128, 0, s>>>(c, d);
+    }
+    mark(P, S_SYNC_FLUX);
+    k_sync_flux<<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams);
+    if (bd.inner_trap) {
+      mark(P, S_INNER_TRAP);
+      k_inner_trap<<<nblocks(6L * L.n * L.n, T), T, 0, s>>>(c);
+    }
+  } else if (phase == PHASE_EMF) {
+    // cubedsphere_step.c:99-126 up to the seam EMF average
+    mark(P, S_FACE_VELOCITY);
+    for (int d = 0; d < 3; ++d) {
+      long n = 6;
+      const int ext[3] = {L.NI, L.NJ, L.NK};
+      const int act[3] = {L.nr, L.n, L.n};
+      for (int a = 0; a < 3; ++a)
+        n *= a == d ? act[a] + 1 : std::min(L.ghost + act[a] + 4, ext[a]) - (L.ghost - 4);
+      k_face_velocity<<<nblocks(n, T), T, 0, s>>>(c, d);
+    }
+    for (int e = 0; e < 3; ++e) {
+      const long n = 6L * (L.nr + (e != 0)) * (L.n + (e != 1)) * (L.n + (e != 2));
+      mark(P, S_EDGE_EMF0 + e);
+      k_edge_emf<<<nblocks(n, 128), 128, 0, s>>>(c, e);
+    }
+    if (bd.earth_emf) {
+      mark(P, S_EARTH_EMF);
+      for (int d = 0; d < 3; ++d) {
+        const int outer = L.ghost + L.nr;
+        const long n = 6L * (std::min(outer + 1, L.NI + (d != 0)) - (outer - 3)) *
+                       (L.NJ + (d != 1)) * (L.NK + (d != 2));
+        k_earth_emf_outer<<<nblocks(n, T), T, 0, s>>>(c, d, w);
+      }
+      k_earth_emf_diffusive<<<nblocks(6L * (L.n + 1) * (L.n + 1), T), T, 0, s>>>(c, w);
+      k_earth_emf_inner<<<nblocks(6L * (L.NJ + 1) * (L.NK + 1), T), T, 0, s>>>(c);
+    }
+  } else {
+    // cubedsphere_step.c:125-152
+    mark(P, S_SYNC_EDGES);
+    k_sync_edges<<<nblocks(P.n_edge_groups, T), T, 0, s>>>(c, P.edge_groups, P.edge_members,
+                                                           P.n_edge_groups);
+    mark(P, S_UPDATE);
+    k_update<<<nblocks(n_active, 128), 128, 0, s>>>(c);
+    // Final seam average of the new face flux, then B of the touched cells.
+    K n = c;
+    n.cur = c.old;
+    n.old = c.cur;
+    mark(P, S_FINAL_SYNC);
+    k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(n, P.seams, P.n_seams, 0);
+    mark(P, S_SEAM_RECOVER);
+    k_recover_list<<<nblocks(P.n_seam_cells, T), T, 0, s>>>(n, P.seam_cells, P.n_seam_cells,
+                                                            plane_phi(n.cur, 0),
+                                                            plane_b(n.cur, 0));
+    if (P.fidelity) {
+      mark(P, S_FIDELITY);
+      k_keep_ghosts<<<nblocks(n_box, T), T, 0, s>>>(c);
+      k_recover<<<nblocks(n_cells, T), T, 0, s>>>(n, plane_phi(n.cur, 0), plane_b(n.cur, 0), 0, 1);
+    }
+    P.level = 1 - P.level;
+    P.host_dirty = true;
+  }
+  GCS_HIP(hipGetLastError());
+  return 0;
+}
+
+int enqueue_step(Problem &P, bool explicit_dt) {
+  K c = make_k(P);
+  const Layout &L = P.L;
+  hipStream_t s = P.stream;
+  const int T = 256;
+  const long n_active = 6L * L.nr * L.n * L.n;
   if (!explicit_dt) {
     mark(P, S_CFL);
     k_cfl<<<nblocks(n_active, T), T, 0, s>>>(c);
     mark(P, S_DT);
     k_dt<<<1, 1, 0, s>>>(c, P.cfg.dt_fraction);
   }
-  mark(P, S_SYNC_PHI);
-  k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams, 1);
-  mark(P, S_HALO_CELLS);
-  k_halo_cells<<<nblocks(P.n_halo_cells, T), T, 0, s>>>(c, P.halo_cells, P.n_halo_cells);
-  if (!P.earth) {
-    mark(P, S_OUTFLOW);
-    k_radial_outflow<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c);
-  }
-  mark(P, S_HALO_FACES);
-  k_halo_faces<<<nblocks(P.n_halo_faces, T), T, 0, s>>>(c, P.halo_faces, P.n_halo_faces);
-  if (P.earth) {
-    mark(P, S_EARTH_CELLS);
-    k_earth_cells<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c, w);
-    mark(P, S_EARTH_FACES);
-    for (int d = 0; d < 3; ++d) {
-      const long n_out = (L.NI + (d == 0)) - (L.ghost + L.nr + (d == 0));
-      const long n = 6L * (L.ghost + n_out) * (L.NJ + (d == 1)) * (L.NK + (d == 2));
-      k_earth_faces<<<nblocks(n, T), T, 0, s>>>(c, d, w);
-    }
-    mark(P, S_EARTH_RECOVER);
-    k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.cur, 0), plane_b(c.cur, 0), 1, 0);
-    k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.old, 0), plane_b(c.old, 0), 1, 0);
-  }
-  mark(P, S_PREDICT);
-  k_predict<<<nblocks(n_box, T), T, 0, s>>>(c);
-  for (int d = 0; d < 3; ++d) {
-    const long n = 6L * (L.nr + (d == 0)) * (L.n + (d == 1)) * (L.n + (d == 2));
-    mark(P, S_FACE_FLUX0 + d);
-    k_face_flux<<<nblocks(n, 128), 128, 0, s>>>(c, d);
-  }
-  mark(P, S_SYNC_FLUX);
-  k_sync_flux<<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams);
-  if (P.earth) {
-    mark(P, S_INNER_TRAP);
-    k_inner_trap<<<nblocks(6L * L.n * L.n, T), T, 0, s>>>(c);
-  }
-  mark(P, S_FACE_VELOCITY);
-  for (int d = 0; d < 3; ++d) {
-    long n = 6;
-    const int ext[3] = {L.NI, L.NJ, L.NK};
-    const int act[3] = {L.nr, L.n, L.n};
-    for (int a = 0; a < 3; ++a)
-      n *= a == d ? act[a] + 1 : std::min(L.ghost + act[a] + 4, ext[a]) - (L.ghost - 4);
-    k_face_velocity<<<nblocks(n, T), T, 0, s>>>(c, d);
-  }
-  for (int e = 0; e < 3; ++e) {
-    const long n = 6L * (L.nr + (e != 0)) * (L.n + (e != 1)) * (L.n + (e != 2));
-    mark(P, S_EDGE_EMF0 + e);
-    k_edge_emf<<<nblocks(n, 128), 128, 0, s>>>(c, e);
-  }
-  if (P.earth) {
-    mark(P, S_EARTH_EMF);
-    for (int d = 0; d < 3; ++d) {
-      const int outer = L.ghost + L.nr;
-      const long n = 6L * (std::min(outer + 1, L.NI + (d != 0)) - (outer - 3)) *
-                     (L.NJ + (d != 1)) * (L.NK + (d != 2));
-      k_earth_emf_outer<<<nblocks(n, T), T, 0, s>>>(c, d, w);
-    }
-    k_earth_emf_diffusive<<<nblocks(6L * (L.n + 1) * (L.n + 1), T), T, 0, s>>>(c, w);
-    k_earth_emf_inner<<<nblocks(6L * (L.NJ + 1) * (L.NK + 1), T), T, 0, s>>>(c);
-  }
-  mark(P, S_SYNC_EDGES);
-  k_sync_edges<<<nblocks(P.n_edge_groups, T), T, 0, s>>>(c, P.edge_groups, P.edge_members,
-                                                         P.n_edge_groups);
-  mark(P, S_UPDATE);
-  k_update<<<nblocks(n_active, 128), 128, 0, s>>>(c);
-  // Final seam average of the new face flux, then B of the touched cells.
-  K n = c;
-  n.cur = c.old;
-  n.old = c.cur;
-  mark(P, S_FINAL_SYNC);
-  k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(n, P.seams, P.n_seams, 0);
-  mark(P, S_SEAM_RECOVER);
-  k_recover_list<<<nblocks(P.n_seam_cells, T), T, 0, s>>>(n, P.seam_cells, P.n_seam_cells,
-                                                          plane_phi(n.cur, 0),
-                                                          plane_b(n.cur, 0));
-  if (P.fidelity) {
-    mark(P, S_FIDELITY);
-    k_keep_ghosts<<<nblocks(n_box, T), T, 0, s>>>(c);
-    k_recover<<<nblocks(n_cells, T), T, 0, s>>>(n, plane_phi(n.cur, 0), plane_b(n.cur, 0), 0, 1);
-  }
+  for (Phase ph : {PHASE_GHOSTS, PHASE_FLUX, PHASE_EMF, PHASE_UPDATE})
+    if (enqueue_phase(P, ph) != 0) return -1;
   if (!explicit_dt) {
     mark(P, S_CLOCK);
     k_clock<<<1, 1, 0, s>>>(c);
   }
   mark(P, S_END);
   GCS_HIP(hipGetLastError());
-  P.level = 1 - P.level;
-  P.host_dirty = true;
   return 0;
 }
 

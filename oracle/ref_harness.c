@@ -50,6 +50,18 @@ typedef struct {
   gamera_cs_mi_dipole_config mi;
 } refh_config;
 
+/* The prototype of gamera_cs_advance_six_with_background_planned
+ * (cubedsphere_step.h:58-67). */
+typedef int (*refh_step_fn)(
+    const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+    gamera_no_storage storage[GAMERA_CS_FACE_COUNT], const size_t active[3],
+    size_t ghost, double predictor_ratio, double dt,
+    const gamera_no_advance_options *options,
+    const gamera_no_background_field background[GAMERA_CS_FACE_COUNT],
+    const gamera_cs_halo_plan *halo_plan,
+    gamera_cs_radial_boundary_function radial_boundary, double current_time,
+    double previous_time, void *boundary_context);
+
 typedef struct {
   refh_config cfg;
   size_t active[3], ghost, extent[3];
@@ -66,7 +78,31 @@ typedef struct {
   double time, previous_dt, dt;
   long steps;
   int mi_updates;
+  refh_step_fn step;                          /* the six-face step called */
+  gamera_cs_radial_boundary_function radial;  /* its radial boundary */
 } refh_problem;
+
+/* Wrappers of the reference's Earth callbacks at addresses of their own:
+ * same results, but a step implementation cannot recognise them. */
+static int wrapped_radial(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+                          gamera_no_storage storage[GAMERA_CS_FACE_COUNT],
+                          const size_t active[3], size_t ghost,
+                          double current_time, double previous_time,
+                          void *context) {
+  return gamera_cs_earth_fill_radial_ghosts(grid, storage, active, ghost,
+                                            current_time, previous_time,
+                                            context);
+}
+static int wrapped_flux(gamera_no_storage *storage, const gamera_no_grid *grid,
+                        const size_t lower[3], const size_t upper[3],
+                        void *context) {
+  return gamera_cs_earth_inner_fluid_flux(storage, grid, lower, upper, context);
+}
+static int wrapped_emf(gamera_no_storage *storage, const gamera_no_grid *grid,
+                       const size_t lower[3], const size_t upper[3],
+                       void *context) {
+  return gamera_cs_earth_no_mi_emf(storage, grid, lower, upper, context);
+}
 
 size_t refh_config_size(void) { return sizeof(refh_config); }
 
@@ -145,6 +181,8 @@ refh_problem *refh_create(const refh_config *cfg) {
   refh_problem *p = calloc(1, sizeof(*p));
   if (p == NULL) return NULL;
   p->cfg = *cfg;
+  p->step = gamera_cs_advance_six_with_background_planned;
+  p->radial = gamera_cs_earth_fill_radial_ghosts;
   p->ghost = cfg->ghost;
   p->active[0] = cfg->radial_cells;
   p->active[1] = p->active[2] = cfg->angular_cells;
@@ -257,11 +295,10 @@ static int advance_one(refh_problem *p) {
   p->boundary.step_dt = p->dt;
   p->boundary.step_time = p->time;
   const double ratio = p->steps == 0 ? 0.0 : 0.5 * p->dt / p->previous_dt;
-  if (gamera_cs_advance_six_with_background_planned(
-          p->grid, p->storage, p->active, p->ghost, ratio, p->dt, &p->options,
-          p->background, p->halo_plan, gamera_cs_earth_fill_radial_ghosts,
-          p->time, p->steps == 0 ? p->time : p->time - p->previous_dt,
-          &p->boundary) != 0)
+  if (p->step(p->grid, p->storage, p->active, p->ghost, ratio, p->dt,
+              &p->options, p->background, p->halo_plan, p->radial, p->time,
+              p->steps == 0 ? p->time : p->time - p->previous_dt,
+              &p->boundary) != 0)
     return -1;
   p->time += p->dt;
   p->previous_dt = p->dt;
@@ -282,10 +319,56 @@ int refh_step_explicit(refh_problem *p, double ratio, double dt,
   if (p == NULL) return -1;
   p->boundary.step_dt = dt;
   p->boundary.step_time = time;
-  return gamera_cs_advance_six_with_background_planned(
-      p->grid, p->storage, p->active, p->ghost, ratio, dt, &p->options,
-      p->background, p->halo_plan, gamera_cs_earth_fill_radial_ghosts, time,
-      previous_time, &p->boundary);
+  return p->step(p->grid, p->storage, p->active, p->ghost, ratio, dt,
+                 &p->options, p->background, p->halo_plan, p->radial, time,
+                 previous_time, &p->boundary);
+}
+
+/* Run later steps through `step` (NULL: the reference's own step), e.g. a
+ * drop-in implementation with the same prototype. */
+void refh_set_step(refh_problem *p, void *step) {
+  if (p == NULL) return;
+  p->step = step != NULL ? (refh_step_fn)step
+                         : gamera_cs_advance_six_with_background_planned;
+}
+
+/* 1: pass the Earth callbacks through wrappers (see wrapped_radial). */
+void refh_use_wrapped_callbacks(refh_problem *p, int on) {
+  if (p == NULL) return;
+  p->radial = on ? wrapped_radial : gamera_cs_earth_fill_radial_ghosts;
+  p->options.fluid_flux_sync = on ? wrapped_flux : gamera_cs_earth_inner_fluid_flux;
+  p->options.edge_emf_sync = on ? wrapped_emf : gamera_cs_earth_no_mi_emf;
+}
+
+/* One step of the plain entry point gamera_cs_advance_six
+ * (cubedsphere_step.h:16-21) on this problem's arrays: no background, no
+ * radial boundary (radial outflow ghosts), no callbacks. `step` has that
+ * prototype; NULL calls the reference's own. The clock is left alone. */
+typedef int (*refh_plain_fn)(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+                             gamera_no_storage storage[GAMERA_CS_FACE_COUNT],
+                             const size_t active[3], size_t ghost,
+                             double predictor_ratio, double dt,
+                             const gamera_no_advance_options *options);
+int refh_step_plain(refh_problem *p, void *step, double ratio, double dt) {
+  if (p == NULL) return -1;
+  gamera_no_advance_options o = p->options;
+  o.stress.use_background = false;
+  o.emf.use_background = false;
+  o.update.use_background = false;
+  o.fluid_flux_sync = NULL;
+  o.edge_emf_sync = NULL;
+  o.fluid_flux_context = NULL;
+  o.edge_emf_context = NULL;
+  refh_plain_fn fn = step != NULL ? (refh_plain_fn)step : gamera_cs_advance_six;
+  return fn(p->grid, p->storage, p->active, p->ghost, ratio, dt, &o);
+}
+
+/* The caller-owned arrays, for a step implementation's bookkeeping. */
+const gamera_no_grid *refh_grid(const refh_problem *p) {
+  return p ? p->grid : NULL;
+}
+gamera_no_storage *refh_storage(refh_problem *p) {
+  return p ? p->storage : NULL;
 }
 
 void refh_clock(const refh_problem *p, double out[4]) {

@@ -6,7 +6,7 @@ import re
 
 from conftest import ROOT
 
-HEADERS = ["gamera_cs_step.h"]
+HEADERS = ["gamera_cs_step.h", "gamera_cs_dropin.h"]
 
 
 def declared_functions():

@@ -1,0 +1,147 @@
+"""The drop-in entry points (include/gamera_cs_dropin.h) called by the
+compiled reference's own driver loop in place of its step.
+
+Two copies of one reference problem run side by side: one steps with the
+reference's gamera_cs_advance_six_with_background_planned, the other with
+libgcs.so's function of the same prototype (the harness calls it through a
+pointer; the reference's dt rule, Earth context and storage are used
+unchanged). Every array the step writes is compared after each step with the
+reference's scaled measure |a - b| / max(1, |a|).
+
+hydro_rate and maxwell_rate are not compared: the device fuses them into the
+update and leaves those host arrays untouched.
+"""
+import ctypes
+import os
+
+import pytest
+
+from conftest import REF_LIB, gcs_config, problem, scaled_error
+
+pytestmark = pytest.mark.gpu
+
+# Per step, as for the problem API (ulp differences of ROCm's transcendental
+# functions from glibc's grow by at most about 1e-14 per step).
+STEP_TOL = 1e-13
+
+WRITTEN = ([problem.CONSERVED, problem.OLD_CONSERVED, problem.PREDICTED_CONSERVED,
+            problem.CELL_MAGNETIC, problem.OLD_CELL_MAGNETIC,
+            problem.PREDICTED_CELL_MAGNETIC]
+           + [problem.FACE_FLUX + d for d in range(3)]
+           + [problem.OLD_FACE_FLUX + d for d in range(3)]
+           + [problem.PREDICTED_FACE_FLUX + d for d in range(3)]
+           + [problem.EDGE_EMF + d for d in range(3)]
+           + [problem.FLUID_FACE_FLUX + d for d in range(3)]
+           + [problem.MAXWELL_FACE_FLUX + d for d in range(3)])
+
+
+@pytest.fixture(scope="module")
+def ref_global():
+    """The compiled reference, loaded into the global symbol table so that
+    libgcs.so finds its Earth callbacks by name."""
+    if not os.path.exists(REF_LIB):
+        pytest.skip("compiled reference (oracle/_ref) not built")
+    lib = ctypes.CDLL(REF_LIB, mode=ctypes.RTLD_GLOBAL)
+    lib.refh_set_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.refh_use_wrapped_callbacks.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.refh_step_plain.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_double, ctypes.c_double]
+    lib.refh_grid.restype = ctypes.c_void_p
+    lib.refh_grid.argtypes = [ctypes.c_void_p]
+    lib.refh_storage.restype = ctypes.c_void_p
+    lib.refh_storage.argtypes = [ctypes.c_void_p]
+    return lib
+
+
+def address(lib, name):
+    return ctypes.cast(getattr(lib, name), ctypes.c_void_p).value
+
+
+def worst(a, b, fields=WRITTEN):
+    out = (0.0, None)
+    for fid in fields:
+        for f in range(gcs_config.FACE_COUNT):
+            e = scaled_error(a.field(f, fid), b.field(f, fid))
+            if e > out[0]:
+                out = (e, (fid, f))
+    return out
+
+
+def release(gpu_lib, ref_lib, p):
+    gpu_lib.gcs_dropin_release.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    gpu_lib.gcs_dropin_release(ref_lib.refh_grid(p._handle),
+                               ref_lib.refh_storage(p._handle))
+
+
+def run_pair(ref_lib, gpu, name, steps, wrapped):
+    lib = gpu.library()
+    cfg = gcs_config.workload(name)
+    a = problem.Problem(ref_lib, "refh", cfg)
+    b = problem.Problem(ref_lib, "refh", cfg)
+    ref_lib.refh_set_step(b._handle,
+                          address(lib, "gamera_cs_advance_six_with_background_planned"))
+    if wrapped:
+        ref_lib.refh_use_wrapped_callbacks(a._handle, 1)
+        ref_lib.refh_use_wrapped_callbacks(b._handle, 1)
+    try:
+        for s in range(steps):
+            a.advance(1)
+            try:
+                b.advance(1)
+            except RuntimeError:
+                pytest.fail(f"drop-in step {s + 1} failed: {gpu.last_error()}")
+            err, where = worst(a, b)
+            ca, cb = a.clock(), b.clock()
+            print(f"dropin {name} wrapped={wrapped} step {s + 1} err {err:.3e} at {where}",
+                  flush=True)
+            assert err <= STEP_TOL * (s + 1), (s + 1, where)
+            assert abs(ca["dt"] - cb["dt"]) <= STEP_TOL * (s + 1) * ca["dt"]
+    finally:
+        release(lib, ref_lib, b)
+
+
+@pytest.mark.parametrize("name,steps", [("tiny", 4), ("small", 8)])
+def test_dropin_matches_reference_step(ref_global, gpu, name, steps):
+    """Earth callbacks recognised by address and run as device kernels."""
+    run_pair(ref_global, gpu, name, steps, wrapped=False)
+
+
+def test_dropin_runs_unknown_callbacks_on_host(ref_global, gpu):
+    """Callbacks at other addresses run on the host between device phases."""
+    run_pair(ref_global, gpu, "small", 3, wrapped=True)
+
+
+def test_plain_entry_point_matches_reference(ref_global, gpu):
+    """gamera_cs_advance_six: no background, radial outflow ghosts."""
+    lib = gpu.library()
+    cfg = gcs_config.workload("small")
+    a = problem.Problem(ref_global, "refh", cfg)
+    b = problem.Problem(ref_global, "refh", cfg)
+    plain = address(lib, "gamera_cs_advance_six")
+    dt = a.clock()["dt"]
+    try:
+        for s, ratio in enumerate((0.0, 0.5, 0.5, 0.5)):
+            assert ref_global.refh_step_plain(a._handle, None, ratio, dt) == 0
+            if ref_global.refh_step_plain(b._handle, plain, ratio, dt) != 0:
+                pytest.fail(f"plain drop-in step failed: {gpu.last_error()}")
+            err, where = worst(a, b)
+            print(f"plain step {s + 1} err {err:.3e} at {where}", flush=True)
+            assert err <= STEP_TOL * (s + 1), where
+    finally:
+        release(lib, ref_global, b)
+    assert lib.gcs_dropin_mirror_count() == 0
+
+
+def test_invalid_arguments_are_rejected_before_writing(ref_global, gpu):
+    """cubedsphere_step.c:23-47: a non-finite dt returns -1 and leaves the
+    storage untouched."""
+    lib = gpu.library()
+    cfg = gcs_config.workload("tiny")
+    b = problem.Problem(ref_global, "refh", cfg)
+    before = [b.field(f, problem.CONSERVED).copy() for f in range(6)]
+    plain = address(lib, "gamera_cs_advance_six")
+    assert ref_global.refh_step_plain(b._handle, plain, 0.0, float("nan")) == -1
+    assert ref_global.refh_step_plain(b._handle, plain, float("inf"), 0.1) == -1
+    for f in range(6):
+        assert (b.field(f, problem.CONSERVED) == before[f]).all()
+    assert lib.gcs_dropin_mirror_count() == 0
