@@ -10,8 +10,9 @@ potential held at zero. The state is resident in HBM before timing starts.
 Prints one JSON line: cell-updates/s over K driver steps (CFL dt rule + full
 six-face step), the step's fraction of the HBM roofline under the
 1216 B/cell-update model, the dominant kernel's roofline from HIP events on
-the step's stream, and the compiled reference library's CPU rate on the
-reference's own CPU-runnable case (6 x 24^2 x 48).
+the step's stream, and the CPU rate of the oracle (a C restatement of the
+reference step, OpenMP on all host threads) on a two-step sample of the
+same workload, started from the GPU problem's own arrays.
 """
 import argparse
 import ctypes
@@ -28,8 +29,13 @@ HBM_PEAK_GBS = 8000.0
 # Compulsory bytes of one cell-update of the whole step (FP64; 30 doubles of
 # state + 122 of static geometry/background).
 STEP_BYTES_PER_CELL = 1216
-# Algorithmic (compulsory) bytes per unit of each heavy stage, and the unit
-# count per launch as a function of (nr, n); see DESIGN.md "Kernels".
+# Algorithmic (compulsory) bytes per unit of each heavy stage: every array
+# element the kernel needs, read or written once per launch. Face flux: one
+# cell's U*, B*, volume (9 doubles), the face frame, B0, B0.n, Phi* (15),
+# fluid + Maxwell flux out (8). Edge EMF: edge frame, solve, B0.t, length
+# (14), two transverse faces' area/velocity/Phi* (7), rho* (1), EMF out (1).
+# Update: 66 doubles in, U/B/Phi out (11). Predict: 38 in, 11 out. The second
+# entry is the unit count per launch as a function of (nr, n).
 STAGE_MODEL = {
     "face_flux_i": (256, lambda nr, n: 6 * (nr + 1) * n * n),
     "face_flux_j": (256, lambda nr, n: 6 * nr * (n + 1) * n),
@@ -42,26 +48,36 @@ STAGE_MODEL = {
 }
 
 
-def cpu_baseline(steps=20):
-    """Compiled reference library on the reference's CPU case (config 0)."""
-    path = os.path.join(ROOT, "oracle", "_ref", "libgamera_ref.so")
+def cpu_baseline(gpu_problem, cfg, steps=2):
+    """The oracle's CPU step on the same workload, from the GPU's arrays."""
+    path = os.path.join(ROOT, "oracle", "liboracle.so")
     if not os.path.exists(path):
         return None
-    cfgm = importlib.import_module("gamera-cube_amd.config")
     prob = importlib.import_module("gamera-cube_amd.problem")
     lib = ctypes.CDLL(path)
-    cfg = cfgm.workload("config0")
-    p = prob.Problem(lib, "refh", cfg)
-    p.advance(1)
+    lib.orc_set_clock.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_long]
+    o = prob.Problem(lib, "orc", cfg)
+    o.copy_fields(gpu_problem, prob.GEOMETRY_FIELDS + prob.BACKGROUND_FIELDS
+                  + prob.STATE_FIELDS)
+    for f in range(6):
+        for d in range(3):
+            o.edge_valid(f, d)[:] = gpu_problem.edge_valid(f, d)
+    c = gpu_problem.clock()
+    lib.orc_set_clock(o._handle, c["time"], c["dt"], c["previous_dt"], c["steps"])
+    o.advance(1)  # first step builds the oracle's plans: not timed
     t0 = time.perf_counter()
-    p.advance(steps)
+    o.advance(steps)
     seconds = time.perf_counter() - t0
-    p.close()
+    o.close()
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n, nr = cfg.angular_cells, cfg.radial_cells
     return {"value": cfg.active_cells * steps / seconds, "unit": "cell-updates/s",
-            "cores": cores, "kind": "reference",
-            "sample": f"6x24^2x48 (config 0), {steps} steps of the compiled "
-                      f"reference serial library (OpenMP), {seconds:.1f} s"}
+            "cores": cores, "kind": "port",
+            "sample": f"{steps} driver steps (after one untimed) of the oracle, a C "
+                      f"restatement of the reference step with OpenMP, on the same "
+                      f"6x{n}^2x{nr} problem started from the GPU's arrays, "
+                      f"{seconds:.1f} s"}
 
 
 def main():
@@ -71,7 +87,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="config1_nomi")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
+                    help="only run a plain copy kernel over N doubles (read N, "
+                         "write N) three times: the known-size reference for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(args.copy_calibration, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_doubles": args.copy_calibration,
+                          "bytes_read": 8 * args.copy_calibration,
+                          "bytes_written": 8 * args.copy_calibration}))
+        return 0
     if args.gpus != 1:
         print("bench.py runs one GPU: the six-face step is a single-GPU "
               "build; --gpus above 1 is not supported", file=sys.stderr)
@@ -124,11 +156,11 @@ def main():
         "stage_ms_per_step": {k: round(v / args.steps, 4) for k, v in stages.items() if v > 0},
         "cpu_baseline": None,
     }
-    g.close()
     if not args.no_cpu_baseline:
-        print("bench: CPU baseline (compiled reference, config 0) ...",
+        print("bench: CPU baseline (oracle, same workload) ...",
               file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline()
+        result["cpu_baseline"] = cpu_baseline(g, cfg)
+    g.close()
     print(json.dumps(result))
     return 0
 

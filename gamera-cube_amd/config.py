@@ -212,4 +212,10 @@ def workload(name):
         return earth_problem(8, 12)
     if name == "small":
         return earth_problem(16, 12, perturbation=0.01, seed=7)
+    if name == "small_mi":     # MI on a 32 x 12 polar grid (Pedersen only)
+        return earth_problem(16, 12, perturbation=0.01, seed=7, mi=True,
+                             mi_azimuth=32, mi_colatitude=12)
+    if name == "small_mi_hall":
+        return earth_problem(16, 12, perturbation=0.01, seed=7, mi=True,
+                             hall_siemens=5.0, mi_azimuth=32, mi_colatitude=12)
     raise KeyError(name)

@@ -320,6 +320,30 @@ typedef struct {
   gamera_no_dipole dipole;
 } gamera_cs_earth_startup;
 
+/* NO/nonorthogonal_mi.h:11-14, :54-72: polar ionosphere solver. Arrays are
+ * colatitude-major with longitude contiguous; row 0 is the pole. */
+enum { GAMERA_MI_NORTH = -1, GAMERA_MI_SOUTH = 1 };
+typedef struct {
+  size_t longitude_count;
+  size_t colatitude_count;
+  double maximum_colatitude;
+  double ionosphere_radius_m;
+  double pedersen_siemens;
+  double hall_siemens;
+  double low_latitude_potential_v;
+  int hemisphere;
+  int maximum_iterations;
+  double relative_tolerance;
+  double absolute_tolerance;
+} gamera_mi_solver_config;
+
+typedef struct {
+  int iterations;
+  double initial_residual;
+  double final_residual;
+  int converged;
+} gamera_mi_solver_stats;
+
 /* cubedsphere_mi_dipole.h:14-28; the solver state is opaque. */
 typedef struct gamera_cs_mi_dipole gamera_cs_mi_dipole;
 typedef struct gamera_cs_mi_dipole_config {

@@ -153,12 +153,43 @@ int orc_earth_emf(gamera_no_storage *s, const gamera_no_grid *g,
                   const gamera_cs_earth_boundary *b, const void *mi);
 
 /* ---- orc_mi.c: ionosphere coupling --------------------------------------- */
+int orc_mi_compose_ghost_velocity(vec3 source, vec3 b0, vec3 drift, vec3 *ghost);
+int orc_mi_cell_current(const gamera_no_grid *g, const vec3 *b, size_t i,
+                        size_t j, size_t k, vec3 *current);
+double orc_mi_mapped_colatitude(double theta, double r, double ri);
+double orc_mi_field_ratio(double theta, double r, double ri);
+double orc_mi_cos_inclination(double theta, int hemisphere);
+int orc_mi_apply_pedersen(const gamera_mi_solver_config *c, const double *x,
+                          double *y);
+int orc_mi_solve_pedersen(const gamera_mi_solver_config *c, const double *fac,
+                          double *potential, gamera_mi_solver_stats *stats);
+int orc_mi_apply_tensor(const gamera_mi_solver_config *c, const double *sp,
+                        const double *sh, const double *x, double *y);
+int orc_mi_solve_tensor(const gamera_mi_solver_config *c, const double *fac,
+                        const double *sp, const double *sh, double *potential,
+                        gamera_mi_solver_stats *stats);
+
 typedef struct orc_mi orc_mi;
+int orc_mi_create(const gamera_no_grid grid[ORC_FACES],
+                  gamera_no_storage st[ORC_FACES],
+                  const gamera_no_background_field bg[ORC_FACES],
+                  const gamera_cs_earth_startup *startup, const size_t active[3],
+                  size_t ghost, const gamera_cs_mi_dipole_config *cfg,
+                  double hall_siemens, orc_mi **out);
+void orc_mi_destroy(orc_mi *mi);
+int orc_mi_update_due(const orc_mi *mi, double time_code);
+int orc_mi_maybe_update(orc_mi *mi, double time_code);
 int orc_mi_ready(const orc_mi *mi);
+int orc_mi_potential(const orc_mi *mi, vec3 point, double *potential_v);
 int orc_mi_ghost_velocity(const orc_mi *mi, vec3 point, vec3 source,
                           vec3 *ghost_velocity);
 int orc_mi_inner_emf(const orc_mi *mi, gamera_no_storage *s,
                      const gamera_no_grid *g, size_t wall_i);
+int orc_mi_potential_values(const orc_mi *mi, int hemi, const double **values,
+                            size_t *count);
+int orc_mi_fac_values(const orc_mi *mi, int hemi, const double **values,
+                      size_t *count);
+int orc_mi_solve_stats(const orc_mi *mi, int hemi, gamera_mi_solver_stats *out);
 
 /* ---- the six-face step (cubedsphere_step.c:13-154, Earth callbacks) ------ */
 typedef struct {

@@ -26,6 +26,8 @@ typedef struct {
   gamera_no_timestep_options timestep;
   orc_halo_plan *halo;
   orc_edge_plan *edges;
+  orc_mi *mi;           /* created on the first step when mi_enabled */
+  int mi_updates;
   double time, dt, previous_dt;
   long steps;
 } orc_problem;
@@ -67,12 +69,13 @@ void orc_destroy(orc_problem *p) {
   }
   orc_halo_plan_free(p->halo);
   orc_edge_plan_free(p->edges);
+  orc_mi_destroy(p->mi);
   free(p);
 }
 
 orc_problem *orc_create(const gcs_problem_config *cfg) {
   if (cfg == NULL || cfg->ghost < 4 || cfg->wind_count < 1 ||
-      cfg->wind_count > GCS_MAX_WIND_SAMPLES || cfg->mi_enabled)
+      cfg->wind_count > GCS_MAX_WIND_SAMPLES)
     return NULL;
   orc_problem *p = calloc(1, sizeof(*p));
   if (p == NULL) return NULL;
@@ -181,6 +184,13 @@ static int prepare(orc_problem *p) {
     p->halo = orc_halo_plan_build(p->grid, p->active, p->ghost);
   if (p->edges == NULL)
     p->edges = orc_edge_plan_build(p->grid, p->active, p->ghost);
+  /* the MI sample shell is read from the copied-in geometry
+   * (cubedsphere_mi_dipole.c:157-221) */
+  if (p->cfg.mi_enabled && p->mi == NULL &&
+      orc_mi_create(p->grid, p->storage, p->field, &p->cfg.startup, p->active,
+                    p->ghost, &p->cfg.mi, p->cfg.hall_siemens, &p->mi) != 0)
+    return -1;
+  p->boundary.mi = (const struct gamera_cs_mi_dipole *)p->mi;
   return p->halo != NULL && p->edges != NULL ? 0 : -1;
 }
 
@@ -205,12 +215,12 @@ int orc_step_explicit(orc_problem *p, double ratio, double dt, double time,
   if (p == NULL || prepare(p) != 0) return -1;
   p->boundary.step_dt = dt;
   p->boundary.step_time = time;
-  const orc_step_context ctx = {p->halo, p->edges, &p->boundary, NULL};
+  const orc_step_context ctx = {p->halo, p->edges, &p->boundary, p->mi};
   return orc_advance_six(p->grid, p->storage, p->active, p->ghost, ratio, dt,
                          &p->options, p->field, &ctx, time, previous_time);
 }
 
-/* One iteration of the driver loop (:593-699), without MI. */
+/* One iteration of the driver loop (:593-699): dt rule, MI cadence, step. */
 int orc_advance(orc_problem *p, long count) {
   if (p == NULL || count < 0 || prepare(p) != 0) return -1;
   for (long n = 0; n < count; ++n) {
@@ -219,6 +229,11 @@ int orc_advance(orc_problem *p, long count) {
     p->dt = p->steps == 0 ? p->cfg.dt_fraction * limit
                           : fmin(p->cfg.dt_fraction * limit,
                                  1.25 * p->previous_dt);
+    if (p->mi != NULL) {
+      const int updated = orc_mi_maybe_update(p->mi, p->time);
+      if (updated < 0) return -1;
+      p->mi_updates += updated;
+    }
     const double ratio = p->steps == 0 ? 0.0 : 0.5 * p->dt / p->previous_dt;
     if (orc_step_explicit(p, ratio, p->dt, p->time,
                           p->steps == 0 ? p->time
@@ -345,4 +360,30 @@ void orc_counters(const orc_problem *p, int face, double out[2]) {
 int orc_unit_up7_pdm(const double vol[8], const double q[8], double pdm,
                      double out[2]) {
   return orc_up7_pdm(vol, q, pdm, &out[0], &out[1]);
+}
+
+int orc_mi_updates(const orc_problem *p) { return p ? p->mi_updates : 0; }
+
+/* Both hemispheres' potential [hemi][colatitude][azimuth] in volts, like
+ * gamera_cs_mi_dipole_export_potential. */
+int orc_mi_potential_export(const orc_problem *p, double *out, size_t count) {
+  if (p == NULL || p->mi == NULL || out == NULL || !orc_mi_ready(p->mi)) return -1;
+  for (int h = 0; h < 2; ++h) {
+    const double *v;
+    size_t n;
+    if (orc_mi_potential_values(p->mi, h, &v, &n) != 0 || count != 2 * n) return -1;
+    memcpy(out + (size_t)h * n, v, n * sizeof(double));
+  }
+  return 0;
+}
+
+int orc_mi_fac_export(const orc_problem *p, double *out, size_t count) {
+  if (p == NULL || p->mi == NULL || out == NULL || !orc_mi_ready(p->mi)) return -1;
+  for (int h = 0; h < 2; ++h) {
+    const double *v;
+    size_t n;
+    if (orc_mi_fac_values(p->mi, h, &v, &n) != 0 || count != 2 * n) return -1;
+    memcpy(out + (size_t)h * n, v, n * sizeof(double));
+  }
+  return 0;
 }

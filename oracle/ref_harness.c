@@ -476,6 +476,20 @@ void refh_counters(const refh_problem *p, int face, double out[2]) {
 
 int refh_mi_updates(const refh_problem *p) { return p->mi_updates; }
 
+/* Both hemispheres' sampled field-aligned current [hemi][colatitude]
+ * [azimuth] in A/m^2 (cubedsphere_mi_dipole.h:30-41). */
+int refh_mi_fac(const refh_problem *p, double *out, size_t count) {
+  gamera_cs_mi_analysis_view v;
+  if (p == NULL || p->mi == NULL ||
+      gamera_cs_mi_dipole_analysis_view(p->mi, &v) != 0)
+    return -1;
+  const size_t n = v.azimuth_points * v.colatitude_points;
+  if (count != 2 * n) return -1;
+  for (int h = 0; h < 2; ++h)
+    memcpy(out + (size_t)h * n, v.fac_parallel_a_m2[h], n * sizeof(double));
+  return 0;
+}
+
 /* Both hemispheres' potential [hemi][colatitude][azimuth] in volts. */
 int refh_mi_potential(const refh_problem *p, double *out, size_t count) {
   if (p == NULL || p->mi == NULL) return -1;
