@@ -29,7 +29,6 @@ int check_config(const gcs_problem_config &c) {
     return set_error("wind_count out of range"), -1;
   if (!(c.startup.gamma > 1.0) || !(c.dt_fraction > 0.0) || !(c.cfl > 0.0))
     return set_error("invalid gamma, dt fraction or CFL"), -1;
-  if (c.mi_enabled) return set_error("MI coupling is not built yet"), -1;
   const long cells = (long)(c.radial_cells + 9) * (c.angular_cells + 9) *
                      ((c.angular_cells + 9 + 7) / 8 * 8) * 6;
   if (cells >= (1L << 31)) return set_error("grid too large for 32-bit indices"), -1;
@@ -61,6 +60,7 @@ void make_wind(Problem &P) {
 
 void free_problem(Problem *P) {
   if (P == nullptr) return;
+  mi_destroy(*P);
   if (P->graph) (void)hipGraphExecDestroy(P->graph);
   (void)hipFree(P->planes);
   (void)hipFree(P->bytes);
@@ -269,6 +269,20 @@ int download_valid(Problem &P, int face, int dir, std::vector<uint8_t> &out) {
 
 }  // namespace gcs
 
+namespace gcs {
+// One driver iteration: the MI cadence check on the clock of the previous
+// step (test_cubedsphere_dipole_background.c:617-625), then the step.
+int driver_step(Problem &P) {
+  if (P.mi != nullptr) {
+    StepScalars s;
+    if (read_scalars(P, s) != 0) return -1;
+    if (s.failed) return check_failure(P);
+    if (mi_maybe_update(P, s.time) < 0) return -1;
+  }
+  return enqueue_step(P, false);
+}
+}  // namespace gcs
+
 using gcs::Problem;
 
 extern "C" {
@@ -289,8 +303,8 @@ Problem *gcs_problem_create(const gcs_problem_config *cfg) {
   }
   gcs::make_wind(*P);
   gcs::driver_options(*P);
-  if (gcs::setup_problem(*P) != 0 ||
-      gcs::build_plans(*P) != 0) {
+  if (gcs::setup_problem(*P) != 0 || gcs::build_plans(*P) != 0 ||
+      (cfg->mi_enabled && gcs::mi_create(*P) != 0)) {
     gcs::free_problem(P);
     return nullptr;
   }
@@ -302,7 +316,7 @@ void gcs_problem_destroy(Problem *P) { gcs::free_problem(P); }
 int gcs_problem_advance(Problem *P, long steps) {
   if (P == nullptr || steps < 0) return -1;
   for (long n = 0; n < steps; ++n)
-    if (gcs::enqueue_step(*P, false) != 0) return -1;
+    if (gcs::driver_step(*P) != 0) return -1;
   return gcs::check_failure(*P);
 }
 
@@ -397,7 +411,7 @@ int gcs_problem_time_steps(Problem *P, long steps, int profile,
   }
   if (hipEventRecord(P->ev_start, P->stream) != hipSuccess) return -1;
   for (long n = 0; n < steps; ++n)
-    if (gcs::enqueue_step(*P, false) != 0) {
+    if (gcs::driver_step(*P) != 0) {
       P->profiling = false;
       return -1;
     }
@@ -432,6 +446,39 @@ void gcs_problem_counters(Problem *P, int face, double out[2]) {
   double speed;
   std::memcpy(&speed, &s.emergency_speed_bits[face], sizeof(speed));
   out[1] = speed;
+}
+
+// Ionosphere coupling state: ints = {updates, ready, iterations north,
+// iterations south}, doubles = {final residual north, south, epoch (s),
+// last solve (ms)}.
+int gcs_problem_mi_report(Problem *P, int ints[4], double doubles[4]) {
+  if (P == nullptr || P->mi == nullptr || ints == nullptr || doubles == nullptr) {
+    gcs::set_error("no ionosphere coupling on this problem");
+    return -1;
+  }
+  gcs::MiReport r;
+  gcs::mi_report(*P, r);
+  ints[0] = r.updates;
+  ints[1] = r.ready;
+  ints[2] = r.iterations[0];
+  ints[3] = r.iterations[1];
+  doubles[0] = r.final_residual[0];
+  doubles[1] = r.final_residual[1];
+  doubles[2] = r.epoch_s;
+  doubles[3] = r.solve_ms;
+  return 0;
+}
+
+// what = 0: potential (V), 1: sampled field-aligned current (A/m^2); both
+// [hemisphere][colatitude][azimuth]. Valid until the next call.
+int gcs_problem_mi_field(Problem *P, int what, double **data, size_t *count) {
+  if (P == nullptr || data == nullptr || count == nullptr || (what != 0 && what != 1))
+    return -1;
+  static thread_local std::vector<double> buf;
+  if (gcs::mi_download(*P, what, buf) != 0) return -1;
+  *data = buf.data();
+  *count = buf.size();
+  return 0;
 }
 
 }  // extern "C"

@@ -97,6 +97,20 @@ struct StepOptions {
 };
 
 
+// The ionosphere potential as the step kernels read it
+// (cubedsphere_mi_dipole.c:442-543); potential == nullptr: not ready.
+struct MiView {
+  const double *potential = nullptr;  // [2][colatitude][azimuth], volts
+  int np = 0, nt = 0;
+  double ri = 0.0, max_colat = 0.0;   // ionosphere radius (R_E), cap
+  double potential_norm = 0.0, length_norm = 0.0, speed_norm = 0.0,
+         b_norm = 0.0;
+  double moment[3] = {0.0, 0.0, 0.0}; // startup dipole and radial cut
+  double cut_radius = 0.0, cut_length = 0.0;
+};
+
+struct MiState;  // owner of the MI arrays and solver (gcs_mi.hip)
+
 // Which boundary work runs on the device inside a step.
 struct StepBoundary {
   bool outflow = false;     // gamera_cs_fill_radial_outflow_cells
@@ -146,6 +160,8 @@ struct Problem {
   std::vector<int> stage_ids;      // stage id of each recorded event
   size_t stage_cursor = 0;
   std::vector<double> staging;     // host buffer for plane transfers
+  MiState *mi = nullptr;           // ionosphere coupling (mi_enabled)
+  MiView mi_view;                  // what the step kernels read
 
   double *plane(int p) const { return planes + (long)p * L.total; }
   uint8_t *byte_plane(int q) const { return bytes + (long)q * L.total; }
@@ -157,9 +173,12 @@ enum Stage {
   S_EARTH_CELLS, S_EARTH_FACES, S_EARTH_RECOVER, S_PREDICT, S_FACE_FLUX0,
   S_FACE_FLUX1, S_FACE_FLUX2, S_SYNC_FLUX, S_INNER_TRAP, S_FACE_VELOCITY,
   S_EDGE_EMF0, S_EDGE_EMF1, S_EDGE_EMF2, S_EARTH_EMF, S_SYNC_EDGES, S_UPDATE,
-  S_FINAL_SYNC, S_SEAM_RECOVER, S_FIDELITY, S_CLOCK, S_END, kStages
+  S_FINAL_SYNC, S_SEAM_RECOVER, S_FIDELITY, S_CLOCK, S_END, S_MI, kStages
 };
 const char *stage_name(int stage);
+// Record a stage boundary on the problem's stream when profiling.
+void mark_stage(Problem &P, int stage);
+int driver_step(Problem &P);
 
 // Device buffers, stream and events of a problem whose layout L is set;
 // scalars are initialised. free_problem releases everything.
@@ -171,8 +190,25 @@ int read_scalars(Problem &P, StepScalars &s);
 // MHD_FAILURE wording; updates the host clock mirror.
 int check_failure(Problem &P);
 
+// Ionosphere coupling (gcs_mi.hip): create from P.cfg after setup; before
+// a step at code time t, update the potential when the cadence is due
+// (cubedsphere_mi_dipole.c:372-436). Returns 1 updated, 0 not due, -1.
+int mi_create(Problem &P);
+int mi_maybe_update(Problem &P, double time_code);
+void mi_destroy(Problem &P);
+struct MiReport {
+  int updates, ready;
+  int iterations[2];
+  double final_residual[2], epoch_s, solve_ms;
+};
+void mi_report(const Problem &P, MiReport &out);
+int mi_download(Problem &P, int what, std::vector<double> &out);  // 0 potential, 1 fac
+
 // Setup: geometry, background, initial state and plans.
 int setup_problem(Problem &P);
+// Cube face and angular coordinates of a direction, with the C library's
+// atan2 like the reference's plans (cubedsphere_mesh.c:77-103).
+bool cube_inverse(const double x[3], int &face, double &a, double &b);
 int build_plans(Problem &P);
 // Options and boundary of the serial Earth driver
 // (test_cubedsphere_dipole_background.c:405-449).

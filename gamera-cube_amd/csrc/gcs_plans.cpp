@@ -30,6 +30,8 @@ double project(const double v[3], const int f[3]) {
   return v[0] * (double)f[0] + v[1] * (double)f[1] + v[2] * (double)f[2];
 }
 
+}  // namespace
+
 // cubedsphere_mesh.c:77-103
 bool cube_inverse(const double x[3], int &face, double &a, double &b) {
   for (int d = 0; d < 3; ++d)
@@ -49,6 +51,9 @@ bool cube_inverse(const double x[3], int &face, double &a, double &b) {
   b = atan2(project(x, kFrame[sel][2]), dom);
   return true;
 }
+
+namespace {
+
 
 // cubedsphere_mesh.c:58-75
 bool cube_direction(int face, double a, double b, double out[3]) {

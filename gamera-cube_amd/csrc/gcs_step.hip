@@ -42,6 +42,7 @@ struct K {
   StepScalars *sc;
   int cur, old;  // time-level slots of this step
   StepOptions o;
+  MiView mi;
   __device__ double *p(int q) const { return pl + (long)q * L.total; }
   __device__ const uint8_t *b(int q) const { return by + (long)q * L.total; }
   __device__ void fail(unsigned stage, long index) const {
@@ -360,6 +361,92 @@ GCS_DEV bool sample_wind(const WindSeries &w, const double x[3], double t,
   return wind_sample(w, x, t, s) && wind_weight(w, x, t, weight);
 }
 
+// ---- ionosphere potential (cubedsphere_mi_dipole.c) ------------------------------
+
+constexpr double kPi = 3.141592653589793;  // acos(-1.0)
+
+// NO/nonorthogonal_mi.c:169-181
+GCS_DEV double mi_mapped_colatitude(double theta, double r, double ri) {
+  double s = sin(theta) * sqrt(ri / r);
+  s = fmax(-1.0, fmin(1.0, s));
+  return asin(s);
+}
+
+// :442-476: bilinear in (mapped colatitude, longitude); zero beyond the cap.
+GCS_DEV bool mi_potential(const MiView &m, const double x[3], double &out) {
+  const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  if (!(r >= m.ri)) return false;
+  const int h = x[2] >= 0.0 ? 0 : 1;
+  const double tm = acos(fmin(1.0, fabs(x[2]) / r));
+  const double ti = mi_mapped_colatitude(tm, r, m.ri);
+  if (!isfinite(ti)) return false;
+  if (ti >= m.max_colat) {
+    out = 0.0;
+    return true;
+  }
+  const double tq = ti / m.max_colat * (double)(m.nt - 1);
+  const int t = (int)floor(tq);
+  const double wt = tq - (double)t;
+  double phi = atan2(x[1], x[0]);
+  if (phi < 0.0) phi += 2.0 * kPi;
+  const double pq = phi * (double)m.np / (2.0 * kPi);
+  const int p = (int)((long)floor(pq) % m.np), q = (p + 1) % m.np;
+  const double wp = pq - floor(pq);
+  const double *v = m.potential + (long)h * m.nt * m.np;
+  const double lower = (1.0 - wp) * v[t * m.np + p] + wp * v[t * m.np + q];
+  const double upper = (1.0 - wp) * v[(t + 1) * m.np + p] + wp * v[(t + 1) * m.np + q];
+  out = (1.0 - wt) * lower + wt * upper;
+  return isfinite(out);
+}
+
+// :505-543 with NO/nonorthogonal_mi.c:20-41: E from the potential by
+// central differences, the E x B drift of the startup dipole without its
+// radial part, and the ghost velocity 2 v_EB - v.
+GCS_DEV bool mi_ghost_velocity(const MiView &m, const double x[3], const double v[3],
+                               double g[3]) {
+  const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  if (!(r > 0.0)) return false;
+  const double eps = 1e-4 * fmax(1.0, r);
+  double e[3];
+  for (int c = 0; c < 3; ++c) {
+    double xp[3] = {x[0], x[1], x[2]}, xm[3] = {x[0], x[1], x[2]};
+    xp[c] += eps;
+    xm[c] -= eps;
+    double pp, pm;
+    if (!mi_potential(m, xp, pp) || !mi_potential(m, xm, pm)) return false;
+    e[c] = -(pp - pm) / (2.0 * eps * m.length_norm * m.speed_norm * m.b_norm);
+  }
+  // cubedsphere_earth_startup.c:92-102, nonorthogonal_background.c:422-442
+  double b[3];
+  {
+    double r2 = 0.0, pr = 0.0;
+    for (int c = 0; c < 3; ++c) r2 += x[c] * x[c];
+    if (!(r2 > DBL_MIN) || !isfinite(r2)) return false;
+    const double rr = sqrt(r2);
+    const double irc = 1.0 / (r2 * rr);
+    const double irf = irc / r2;
+    for (int c = 0; c < 3; ++c) pr += m.moment[c] * x[c];
+    for (int c = 0; c < 3; ++c) b[c] = (3.0 * pr * irf) * x[c] - irc * m.moment[c];
+    const double sc = (r - m.cut_radius) / m.cut_length;
+    const double w = sc <= 0.0 ? 1.0 : sc >= 1.0 ? 0.0 : 1.0 - 3.0 * sc * sc + 2.0 * sc * sc * sc;
+    for (int c = 0; c < 3; ++c) b[c] *= w;
+  }
+  double b2 = 0.0;
+  for (int c = 0; c < 3; ++c) b2 += b[c] * b[c];
+  if (!(b2 > 0.0) || !isfinite(b2)) return false;
+  double d[3] = {(e[1] * b[2] - e[2] * b[1]) / b2, (e[2] * b[0] - e[0] * b[2]) / b2,
+                 (e[0] * b[1] - e[1] * b[0]) / b2};
+  double rad = 0.0;
+  for (int c = 0; c < 3; ++c) rad += d[c] * x[c];
+  rad /= r * r;
+  for (int c = 0; c < 3; ++c) d[c] -= rad * x[c];
+  for (int c = 0; c < 3; ++c) {
+    g[c] = 2.0 * d[c] - v[c];
+    if (!isfinite(g[c])) return false;
+  }
+  return true;
+}
+
 // ---- Earth radial boundary -------------------------------------------------
 
 // cubedsphere_earth_boundary.c:27-112: radial ghost cells of both levels.
@@ -388,7 +475,17 @@ __global__ void k_earth_cells(K c, WindSeries w) {
       return;
     }
     if (inner) {
-      if (c.o.strict_wall) {
+      if (c.mi.potential != nullptr) {
+        // cubedsphere_earth_boundary.c:58-68: the ionospheric drift
+        double g[3];
+        if (!mi_ghost_velocity(c.mi, x, pw.v, g)) {
+          c.fail(14, t);
+          return;
+        }
+        if (c.o.strict_wall)
+          for (int d = 0; d < 3; ++d) g[d] = 0.5 * (g[d] + pw.v[d]);
+        for (int d = 0; d < 3; ++d) pw.v[d] = g[d];
+      } else if (c.o.strict_wall) {
         pw.v[0] = pw.v[1] = pw.v[2] = 0.0;
       } else {
         double vn = 0.0;
@@ -897,7 +994,9 @@ __global__ void k_earth_emf_diffusive(K c, WindSeries w) {
   c.p(P_EMF + 2)[at] += gk * speed * ck * dl;
 }
 
-// cubedsphere_earth_boundary.c:493-503: E_t = 0 on the inner wall shell.
+// cubedsphere_earth_boundary.c:429-440: E_t = 0 on the inner wall shell, or
+// with the ionosphere ready, -(potential difference) / potential norm along
+// each wall edge (cubedsphere_mi_dipole.c:478-503).
 __global__ void k_earth_emf_inner(K c) {
   const Layout &L = c.L;
   const int lo[3] = {L.ghost, 0, 0};
@@ -906,8 +1005,23 @@ __global__ void k_earth_emf_inner(K c) {
   if (!box(tid(), lo, hi, f, i, j, k)) return;
   const long at = L.at(f, i, j, k);
   // J edges span (NI+1, NJ, NK+1), K edges (NI+1, NJ+1, NK).
-  if (j < L.NJ && c.b(Q_VALID + 1)[at]) c.p(P_EMF + 1)[at] = 0.0;
-  if (k < L.NK && c.b(Q_VALID + 2)[at]) c.p(P_EMF + 2)[at] = 0.0;
+  for (int d = 1; d <= 2; ++d) {
+    if ((d == 1 && j >= L.NJ) || (d == 2 && k >= L.NK) || !c.b(Q_VALID + d)[at])
+      continue;
+    double emf = 0.0;
+    if (c.mi.potential != nullptr) {
+      const long end = at + (d == 1 ? (long)L.SK : 1L);
+      const double a[3] = {c.p(P_VERT)[at], c.p(P_VERT + 1)[at], c.p(P_VERT + 2)[at]};
+      const double b[3] = {c.p(P_VERT)[end], c.p(P_VERT + 1)[end], c.p(P_VERT + 2)[end]};
+      double pa, pb;
+      if (!mi_potential(c.mi, a, pa) || !mi_potential(c.mi, b, pb)) {
+        c.fail(64, at);
+        return;
+      }
+      emf = -(pb - pa) / c.mi.potential_norm;
+    }
+    c.p(P_EMF + d)[at] = emf;
+  }
 }
 
 // ---- update ------------------------------------------------------------------------
@@ -1044,6 +1158,7 @@ static K make_k(Problem &P) {
   c.cur = P.level;
   c.old = 1 - P.level;
   c.o = P.opt;
+  c.mi = P.mi_view;
   return c;
 }
 
@@ -1104,12 +1219,12 @@ const char *stage_name(int stage) {
       "earth_cells", "earth_faces", "earth_recover", "predict", "face_flux_i",
       "face_flux_j", "face_flux_k", "sync_flux", "inner_trap", "face_velocity",
       "edge_emf_i", "edge_emf_j", "edge_emf_k", "earth_emf", "sync_edges",
-      "update", "final_sync", "seam_recover", "fidelity", "clock", "end"};
+      "update", "final_sync", "seam_recover", "fidelity", "clock", "end", "mi_update"};
   return stage >= 0 && stage < kStages ? names[stage] : "?";
 }
 
 // Record a stage boundary when profiling (events are preallocated).
-static void mark(Problem &P, int stage) {
+void mark_stage(Problem &P, int stage) {
   if (!P.profiling || P.stage_cursor >= P.stage_events.size()) return;
   (void)hipEventRecord(P.stage_events[P.stage_cursor], P.stream);
   P.stage_ids[P.stage_cursor] = stage;
@@ -1129,47 +1244,47 @@ int enqueue_phase(Problem &P, Phase phase) {
 
   if (phase == PHASE_GHOSTS) {
     // cubedsphere_step.c:51-65
-    mark(P, S_SYNC_PHI);
+    mark_stage(P, S_SYNC_PHI);
     k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams, 1);
-    mark(P, S_HALO_CELLS);
+    mark_stage(P, S_HALO_CELLS);
     k_halo_cells<<<nblocks(P.n_halo_cells, T), T, 0, s>>>(c, P.halo_cells, P.n_halo_cells);
     if (bd.outflow) {
-      mark(P, S_OUTFLOW);
+      mark_stage(P, S_OUTFLOW);
       k_radial_outflow<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c);
     }
-    mark(P, S_HALO_FACES);
+    mark_stage(P, S_HALO_FACES);
     k_halo_faces<<<nblocks(P.n_halo_faces, T), T, 0, s>>>(c, P.halo_faces, P.n_halo_faces);
     if (bd.earth_cells) {
-      mark(P, S_EARTH_CELLS);
+      mark_stage(P, S_EARTH_CELLS);
       k_earth_cells<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c, w);
-      mark(P, S_EARTH_FACES);
+      mark_stage(P, S_EARTH_FACES);
       for (int d = 0; d < 3; ++d) {
         const long n_out = (L.NI + (d == 0)) - (L.ghost + L.nr + (d == 0));
         const long n = 6L * (L.ghost + n_out) * (L.NJ + (d == 1)) * (L.NK + (d == 2));
         k_earth_faces<<<nblocks(n, T), T, 0, s>>>(c, d, w);
       }
-      mark(P, S_EARTH_RECOVER);
+      mark_stage(P, S_EARTH_RECOVER);
       k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.cur, 0), plane_b(c.cur, 0), 1, 0);
       k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.old, 0), plane_b(c.old, 0), 1, 0);
     }
   } else if (phase == PHASE_FLUX) {
     // cubedsphere_step.c:67-98
-    mark(P, S_PREDICT);
+    mark_stage(P, S_PREDICT);
     k_predict<<<nblocks(n_box, T), T, 0, s>>>(c);
     for (int d = 0; d < 3; ++d) {
       const long n = 6L * (L.nr + (d == 0)) * (L.n + (d == 1)) * (L.n + (d == 2));
-      mark(P, S_FACE_FLUX0 + d);
+      mark_stage(P, S_FACE_FLUX0 + d);
       k_face_flux<<<nblocks(n, 128), 128, 0, s>>>(c, d);
     }
-    mark(P, S_SYNC_FLUX);
+    mark_stage(P, S_SYNC_FLUX);
     k_sync_flux<<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams);
     if (bd.inner_trap) {
-      mark(P, S_INNER_TRAP);
+      mark_stage(P, S_INNER_TRAP);
       k_inner_trap<<<nblocks(6L * L.n * L.n, T), T, 0, s>>>(c);
     }
   } else if (phase == PHASE_EMF) {
     // cubedsphere_step.c:99-126 up to the seam EMF average
-    mark(P, S_FACE_VELOCITY);
+    mark_stage(P, S_FACE_VELOCITY);
     for (int d = 0; d < 3; ++d) {
       long n = 6;
       const int ext[3] = {L.NI, L.NJ, L.NK};
@@ -1180,11 +1295,11 @@ int enqueue_phase(Problem &P, Phase phase) {
     }
     for (int e = 0; e < 3; ++e) {
       const long n = 6L * (L.nr + (e != 0)) * (L.n + (e != 1)) * (L.n + (e != 2));
-      mark(P, S_EDGE_EMF0 + e);
+      mark_stage(P, S_EDGE_EMF0 + e);
       k_edge_emf<<<nblocks(n, 128), 128, 0, s>>>(c, e);
     }
     if (bd.earth_emf) {
-      mark(P, S_EARTH_EMF);
+      mark_stage(P, S_EARTH_EMF);
       for (int d = 0; d < 3; ++d) {
         const int outer = L.ghost + L.nr;
         const long n = 6L * (std::min(outer + 1, L.NI + (d != 0)) - (outer - 3)) *
@@ -1196,23 +1311,23 @@ int enqueue_phase(Problem &P, Phase phase) {
     }
   } else {
     // cubedsphere_step.c:125-152
-    mark(P, S_SYNC_EDGES);
+    mark_stage(P, S_SYNC_EDGES);
     k_sync_edges<<<nblocks(P.n_edge_groups, T), T, 0, s>>>(c, P.edge_groups, P.edge_members,
                                                            P.n_edge_groups);
-    mark(P, S_UPDATE);
+    mark_stage(P, S_UPDATE);
     k_update<<<nblocks(n_active, 128), 128, 0, s>>>(c);
     // Final seam average of the new face flux, then B of the touched cells.
     K n = c;
     n.cur = c.old;
     n.old = c.cur;
-    mark(P, S_FINAL_SYNC);
+    mark_stage(P, S_FINAL_SYNC);
     k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(n, P.seams, P.n_seams, 0);
-    mark(P, S_SEAM_RECOVER);
+    mark_stage(P, S_SEAM_RECOVER);
     k_recover_list<<<nblocks(P.n_seam_cells, T), T, 0, s>>>(n, P.seam_cells, P.n_seam_cells,
                                                             plane_phi(n.cur, 0),
                                                             plane_b(n.cur, 0));
     if (P.fidelity) {
-      mark(P, S_FIDELITY);
+      mark_stage(P, S_FIDELITY);
       k_keep_ghosts<<<nblocks(n_box, T), T, 0, s>>>(c);
       k_recover<<<nblocks(n_cells, T), T, 0, s>>>(n, plane_phi(n.cur, 0), plane_b(n.cur, 0), 0, 1);
     }
@@ -1230,18 +1345,18 @@ int enqueue_step(Problem &P, bool explicit_dt) {
   const int T = 256;
   const long n_active = 6L * L.nr * L.n * L.n;
   if (!explicit_dt) {
-    mark(P, S_CFL);
+    mark_stage(P, S_CFL);
     k_cfl<<<nblocks(n_active, T), T, 0, s>>>(c);
-    mark(P, S_DT);
+    mark_stage(P, S_DT);
     k_dt<<<1, 1, 0, s>>>(c, P.cfg.dt_fraction);
   }
   for (Phase ph : {PHASE_GHOSTS, PHASE_FLUX, PHASE_EMF, PHASE_UPDATE})
     if (enqueue_phase(P, ph) != 0) return -1;
   if (!explicit_dt) {
-    mark(P, S_CLOCK);
+    mark_stage(P, S_CLOCK);
     k_clock<<<1, 1, 0, s>>>(c);
   }
-  mark(P, S_END);
+  mark_stage(P, S_END);
   GCS_HIP(hipGetLastError());
   return 0;
 }

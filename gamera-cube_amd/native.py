@@ -95,3 +95,36 @@ def _set_fidelity(self, on):
 
 GpuProblem.time_steps = _time_steps
 GpuProblem.set_fidelity = _set_fidelity
+
+
+def _mi_report(self):
+    """Ionosphere coupling: updates, readiness, iterations and residuals of
+    the last solve per hemisphere, its epoch (s) and duration (ms)."""
+    lib = self._lib
+    ints = (ctypes.c_int * 4)()
+    dbl = (ctypes.c_double * 4)()
+    lib.gcs_problem_mi_report.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    if lib.gcs_problem_mi_report(self._handle, ints, dbl) != 0:
+        raise RuntimeError(f"gcs_problem_mi_report: {last_error()}")
+    return {"updates": ints[0], "ready": bool(ints[1]),
+            "iterations": (ints[2], ints[3]), "final_residual": (dbl[0], dbl[1]),
+            "epoch_s": dbl[2], "solve_ms": dbl[3]}
+
+
+def _mi_field(self, what):
+    """Copy of the potential (what=0, V) or sampled FAC (what=1, A/m^2),
+    [hemisphere][colatitude][azimuth]."""
+    import numpy as np
+    lib = self._lib
+    ptr = ctypes.POINTER(ctypes.c_double)()
+    count = ctypes.c_size_t()
+    lib.gcs_problem_mi_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_void_p]
+    if lib.gcs_problem_mi_field(self._handle, what, ctypes.byref(ptr),
+                                ctypes.byref(count)) != 0:
+        raise RuntimeError(f"gcs_problem_mi_field: {last_error()}")
+    return np.ctypeslib.as_array(ptr, shape=(count.value,)).copy()
+
+
+GpuProblem.mi_report = _mi_report
+GpuProblem.mi_field = _mi_field

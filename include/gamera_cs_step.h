@@ -78,6 +78,17 @@ int gcs_problem_time_steps(gcs_problem *problem, long steps, int profile,
 int gcs_stage_count(void);
 const char *gcs_stage_name(int stage);
 
+/* Ionosphere coupling of a problem created with mi_enabled (the driver's
+ * gamera_cs_mi_dipole_*, cubedsphere_mi_dipole.h:51-98): ints = {updates,
+ * ready, solver iterations north, south}; doubles = {final residual north,
+ * south, epoch in s, last solve in ms}. */
+int gcs_problem_mi_report(gcs_problem *problem, int ints[4], double doubles[4]);
+/* what = 0: potential in V, 1: sampled field-aligned current in A/m^2,
+ * [hemisphere][colatitude][azimuth] like
+ * gamera_cs_mi_dipole_export_potential; valid until the next call. */
+int gcs_problem_mi_field(gcs_problem *problem, int what, double **data,
+                         size_t *count);
+
 #ifdef __cplusplus
 }
 #endif
