@@ -120,11 +120,22 @@ def main():
           file=sys.stderr, flush=True)
     if args.warmup:
         g.advance(args.warmup)
+    mi_before = g.mi_report()["updates"] if cfg.mi_enabled else 0
     ms, stages = g.time_steps(args.steps, profile=True)
     print(f"bench: {args.steps} steps in {ms:.1f} ms", file=sys.stderr, flush=True)
     per_step = ms / args.steps
     value = cfg.active_cells * args.steps / (ms * 1e-3)
     nr, n = cfg.radial_cells, cfg.angular_cells
+    if cfg.mi_enabled:
+        rep = g.mi_report()
+        mi_info = {"grid": f"2 x {cfg.mi.colatitude_points} x {cfg.mi.azimuth_points}",
+                   "pedersen_S": cfg.mi.pedersen_siemens, "hall_S": cfg.hall_siemens,
+                   "updates_in_timed_steps": rep["updates"] - mi_before,
+                   "last_solve_iterations": rep["iterations"],
+                   "last_solve_ms": rep["solve_ms"],
+                   "tolerance": cfg.mi.relative_tolerance}
+    else:
+        mi_info = "off: ionosphere potential held at zero (no-MI wall)"
     modeled = {k: v for k, v in stages.items() if k in STAGE_MODEL and v > 0}
     top = max(modeled, key=modeled.get)
     bytes_per_unit, units = STAGE_MODEL[top]
@@ -146,7 +157,7 @@ def main():
         "config": {"workload": args.workload,
                    "cells": f"6x{n}^2x{nr}",
                    "active_cells": cfg.active_cells,
-                   "mi": "ionosphere potential held at zero (MI solve not in this build)",
+                   "mi": mi_info,
                    "hbm_fraction_step_model": STEP_BYTES_PER_CELL * value / (HBM_PEAK_GBS * 1e9)},
         "roofline": {"bound": "hbm", "kernel": top,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

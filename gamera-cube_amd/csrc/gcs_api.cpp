@@ -382,6 +382,12 @@ int gcs_problem_edge_valid(Problem *P, int face, int dir, unsigned char **data,
 
 // Keep (1) or skip (0) the work that only reproduces the reference's ghost
 // values in storage; the step's active results are the same either way.
+int gcs_problem_set_precision(Problem *P, int mixed) {
+  if (P == nullptr || (mixed != 0 && mixed != 1)) return -1;
+  P->mixed = mixed == 1;
+  return 0;
+}
+
 int gcs_problem_set_fidelity(Problem *P, int on) {
   if (P == nullptr) return -1;
   P->fidelity = on != 0;

@@ -153,6 +153,7 @@ struct Problem {
   int n_seam_cells = 0;
   WindSeries wind_series;
   bool fidelity = true;            // reproduce ghost values of the storage
+  bool mixed = false;              // FP32 reconstruction, fluxes and EMF
   // Stage timing: when `profiling`, enqueue_step records an event before
   // every stage into `stage_events` (one row of kMaxStages per step).
   bool profiling = false;

@@ -35,7 +35,6 @@ namespace gcs {
 
 namespace {
 
-constexpr double kPi = 3.141592653589793;  // acos(-1.0)
 constexpr int kT = 256;
 
 // value = (1-wx) [(1-wy) v[b] + wy v[b+1]] + wx [(1-wy) v[b+s] + wy v[b+s+1]]

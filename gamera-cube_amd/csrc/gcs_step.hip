@@ -28,6 +28,7 @@
 
 #include "gcs_device.hpp"
 #include "gcs_math.hpp"
+#include "gcs_math_mixed.hpp"
 
 namespace gcs {
 
@@ -362,8 +363,6 @@ GCS_DEV bool sample_wind(const WindSeries &w, const double x[3], double t,
 }
 
 // ---- ionosphere potential (cubedsphere_mi_dipole.c) ------------------------------
-
-constexpr double kPi = 3.141592653589793;  // acos(-1.0)
 
 // NO/nonorthogonal_mi.c:169-181
 GCS_DEV double mi_mapped_colatitude(double theta, double r, double ri) {
@@ -915,6 +914,233 @@ __global__ void __launch_bounds__(128) k_edge_emf(K c, int e) {
   c.p(P_EMF + e)[at] = emf;
 }
 
+
+// ---- mixed precision: FP32 reconstruction, fluxes and EMF ---------------------
+//
+// The same sweeps as k_face_flux, k_face_velocity and k_edge_emf with the
+// arithmetic in float: inputs are read from the FP64 planes and rounded once,
+// results are stored back into the FP64 flux/EMF planes (the area scaling
+// is done in FP64), so the update, CT, CFL and ionosphere stay FP64.
+
+__global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
+  using R = float;
+  using namespace mixed;
+  const Layout &L = c.L;
+  const int lo[3] = {L.ghost, L.ghost, L.ghost};
+  const int hi[3] = {L.ghost + L.nr + (d == 0), L.ghost + L.n + (d == 1),
+                     L.ghost + L.n + (d == 2)};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
+  const StepOptions &o = c.o;
+  const R gamma = (R)o.gamma, rf = (R)o.rho_floor, pf = (R)o.p_floor, pdm = (R)o.pdm;
+  R vol[8];
+  PrimR<R> ws[8];
+  for (int n = 0; n < 8; ++n) {
+    const long cell = at + (n - 4) * st;
+    vol[n] = (R)c.p(P_VOL)[cell];
+    R u[5];
+    for (int q = 0; q < 5; ++q) u[q] = (R)c.p(P_UPRED + q)[cell];
+    if (!cons_to_prim_r(u, gamma, rf, pf, ws[n])) {
+      c.fail(41, at);
+      return;
+    }
+  }
+  const R iv = central8_r(vol);
+  if (!isfinite(iv) || fabs(iv) <= tiny<R>()) {
+    c.fail(42, at);
+    return;
+  }
+  PrimR<R> w[2];
+  {
+    R q8[8];
+    for (int n = 0; n < 8; ++n) q8[n] = ws[n].rho;
+    up7_pdm_r(vol, iv, q8, pdm, w[0].rho, w[1].rho);
+    for (int e = 0; e < 3; ++e) {
+      for (int n = 0; n < 8; ++n) q8[n] = ws[n].v[e];
+      up7_pdm_r(vol, iv, q8, pdm, w[0].v[e], w[1].v[e]);
+    }
+    for (int n = 0; n < 8; ++n) q8[n] = ws[n].p;
+    up7_pdm_r(vol, iv, q8, pdm, w[0].p, w[1].p);
+  }
+  R bb[2][3];
+  for (int e = 0; e < 3; ++e) {
+    R q8[8];
+    for (int n = 0; n < 8; ++n) q8[n] = (R)c.p(P_BPRED + e)[at + (n - 4) * st];
+    up7_pdm_r(vol, iv, q8, pdm, bb[0][e], bb[1][e]);
+  }
+  if (!valid_interface_r(w[0].rho, w[0].v, w[0].p) ||
+      !valid_interface_r(w[1].rho, w[1].v, w[1].p)) {
+    c.fail(43, at);
+    return;
+  }
+  const int fb = P_FACE + FACE_PLANES * d;
+  const double area = c.p(fb + FO_AREA)[at];
+  FaceFrameR<R> fr;
+  for (int e = 0; e < 3; ++e) {
+    fr.n[e] = (R)c.p(fb + FO_N + e)[at];
+    fr.t1[e] = (R)c.p(fb + FO_T1 + e)[at];
+    fr.t2[e] = (R)c.p(fb + FO_T2 + e)[at];
+  }
+  FluidFluxR<R> fl;
+  fluid_flux_r(w, gamma, fr, fl);
+  const R bn = (R)(c.p(P_PHIPRED + d)[at] / area);
+  const R b0[3] = {(R)c.p(P_B0F + 3 * d)[at], (R)c.p(P_B0F + 3 * d + 1)[at],
+                   (R)c.p(P_B0F + 3 * d + 2)[at]};
+  const R cl = (R)o.c;
+  MaxwellFluxR<R> mx;
+  if (!maxwell_flux_r(w, bb, bn, fr.n, b0, (R)c.p(P_B0N + d)[at], o.boris != 0, cl, fl.un,
+                      mx)) {
+    c.fail(44, at);
+    return;
+  }
+  if (o.hogs) hogs_r(fl, mx, (R)o.hogs_h, (R)o.hogs_m, o.boris != 0, cl);
+  if (o.hogs && o.boris) {
+    R s2 = R(0);
+    s2 += dot3r(w[0].v, w[0].v);
+    s2 += dot3r(w[1].v, w[1].v);
+    if (sqrt(R(0.5) * s2) >= R(1.5) * cl) {
+      const long lc = at - st;
+      for (int q = 0; q < 5; ++q) {
+        const double lo_u = c.p(P_UPRED + q)[lc], up_u = c.p(P_UPRED + q)[at];
+        if (!isfinite(lo_u) || !isfinite(up_u)) {
+          c.fail(45, at);
+          return;
+        }
+        fl.f[q] -= cl * (R)(up_u - lo_u);
+      }
+      R t2 = R(0);
+      for (int side = 0; side < 2; ++side)
+        for (int e = 0; e < 3; ++e) t2 += w[side].v[e] * w[side].v[e];
+      atomicAdd(&c.sc->emergency_count[f], 1ull);
+      atomicMax(&c.sc->emergency_speed_bits[f],
+                (unsigned long long)__double_as_longlong((double)sqrt(R(0.5) * t2)));
+    }
+  }
+  for (int q = 0; q < 5; ++q) c.p(P_FLUID + 5 * d + q)[at] = area * (double)fl.f[q];
+  for (int e = 0; e < 3; ++e) c.p(P_MAXWELL + 3 * d + e)[at] = area * (double)mx.m[e];
+}
+
+__global__ void k_face_velocity_fp32(K c, int d) {
+  using R = float;
+  using namespace mixed;
+  const Layout &L = c.L;
+  int lo[3], hi[3];
+  const int ext[3] = {L.NI, L.NJ, L.NK};
+  const int act[3] = {L.nr, L.n, L.n};
+  for (int a = 0; a < 3; ++a) {
+    if (a == d) {
+      lo[a] = L.ghost;
+      hi[a] = L.ghost + act[a] + 1;
+    } else {
+      lo[a] = L.ghost - 4;
+      hi[a] = min(L.ghost + act[a] + 4, ext[a]);
+    }
+  }
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
+  const R rf = (R)c.o.emf_rho_floor;
+  R vol[8], wv[3][8];
+  for (int n = 0; n < 8; ++n) {
+    const long cell = at + (n - 4) * st;
+    vol[n] = (R)c.p(P_VOL)[cell];
+    const R rho = fmax((R)c.p(P_UPRED)[cell], rf);
+    for (int e = 0; e < 3; ++e) wv[e][n] = vol[n] * (R)c.p(P_UPRED + 1 + e)[cell] / rho;
+  }
+  const R iv = central8_r(vol);
+  const bool ok = isfinite(iv) && fabs(iv) > tiny<R>();
+  for (int e = 0; e < 3; ++e)
+    c.p(P_FVEL + 3 * d + e)[at] = ok ? (double)(central8_r(wv[e]) / iv) : (double)NAN;
+}
+
+__global__ void __launch_bounds__(128) k_edge_emf_fp32(K c, int e) {
+  using R = float;
+  using namespace mixed;
+  const Layout &L = c.L;
+  const int t1 = (e + 1) % 3, t2 = (e + 2) % 3;
+  const int lo[3] = {L.ghost, L.ghost, L.ghost};
+  const int hi[3] = {L.ghost + L.nr + (e != 0), L.ghost + L.n + (e != 1),
+                     L.ghost + L.n + (e != 2)};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  if (!c.b(Q_VALID + e)[at] || !c.b(Q_EBOK + e)[at]) {
+    c.fail(51, at);
+    return;
+  }
+  const long step[3] = {L.di(), (long)L.SK, 1};
+  const StepOptions &o = c.o;
+  const int eb = P_EDGE + EDGE_PLANES * e;
+  R a1[8], v[3];
+  {
+    R wv[3][8];
+    for (int n = 0; n < 8; ++n) {
+      const long fa = at + (n - 4) * step[t2];
+      a1[n] = (R)c.p(P_FACE + FACE_PLANES * t1 + FO_AREA)[fa];
+      for (int m = 0; m < 3; ++m) wv[m][n] = a1[n] * (R)c.p(P_FVEL + 3 * t1 + m)[fa];
+    }
+    const R ea = central8_r(a1);
+    if (!isfinite(ea) || fabs(ea) <= tiny<R>()) {
+      c.fail(52, at);
+      return;
+    }
+    for (int m = 0; m < 3; ++m) v[m] = central8_r(wv[m]) / ea;
+  }
+  R et1[3], et2[3];
+  for (int m = 0; m < 3; ++m) {
+    et1[m] = (R)c.p(eb + EO_T1 + m)[at];
+    et2[m] = (R)c.p(eb + EO_T2 + m)[at];
+  }
+  const R v1 = dot3r(v, et1), v2 = dot3r(v, et2);
+  const R pdm = (R)o.emf_pdm;
+  R fl, fr, sl, sr;
+  {
+    R fld[8];
+    const double *ph = c.p(P_PHIPRED + t1);
+    for (int n = 0; n < 8; ++n) fld[n] = (R)ph[at + (n - 4) * step[t2]] / a1[n];
+    up7_pdm_r(a1, central8_r(a1), fld, pdm, fl, fr);
+  }
+  {
+    R a2[8], fld[8];
+    const double *ph = c.p(P_PHIPRED + t2);
+    for (int n = 0; n < 8; ++n) {
+      const long fa = at + (n - 4) * step[t1];
+      a2[n] = (R)c.p(P_FACE + FACE_PLANES * t2 + FO_AREA)[fa];
+      fld[n] = (R)ph[fa] / a2[n];
+    }
+    const R ia = central8_r(a2);
+    if (!isfinite(ia) || fabs(ia) <= tiny<R>()) {
+      c.fail(53, at);
+      return;
+    }
+    up7_pdm_r(a2, ia, fld, pdm, sl, sr);
+  }
+  R solve[5], bt[2];
+  for (int m = 0; m < 5; ++m) solve[m] = (R)c.p(eb + EO_SOLVE + m)[at];
+  edge_field_r(solve, R(0.5) * (fl + fr), R(0.5) * (sl + sr), bt);
+  bt[0] += (R)c.p(P_B0ET + 2 * e)[at];
+  bt[1] += (R)c.p(P_B0ET + 2 * e + 1)[at];
+  const R jump = (sr - sl) - (fr - fl);
+  R rho = R(0);
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      rho += R(0.25) * (R)c.p(P_UPRED)[at - a * step[t1] - b * step[t2]];
+  if (!isfinite(rho) || rho <= tiny<R>()) {
+    c.fail(54, at);
+    return;
+  }
+  R emf;
+  if (!edge_emf_r(v1, v2, bt[0], bt[1], jump, rho, (R)c.p(eb + EO_LEN)[at], (R)o.emf_diff,
+                  o.emf_boris != 0, (R)o.emf_c, (R)o.emf_cfl, (R)c.sc->dt, emf)) {
+    c.fail(55, at);
+    return;
+  }
+  c.p(P_EMF + e)[at] = (double)emf;
+}
+
 // cubedsphere_earth_boundary.c:445-489: outer shells blend toward the wind
 // line integral -(V x B) . l.
 __global__ void k_earth_emf_outer(K c, int d, WindSeries w) {
@@ -1274,7 +1500,10 @@ int enqueue_phase(Problem &P, Phase phase) {
     for (int d = 0; d < 3; ++d) {
       const long n = 6L * (L.nr + (d == 0)) * (L.n + (d == 1)) * (L.n + (d == 2));
       mark_stage(P, S_FACE_FLUX0 + d);
-      k_face_flux<<<nblocks(n, 128), 128, 0, s>>>(c, d);
+      if (P.mixed)
+        k_face_flux_fp32<<<nblocks(n, 128), 128, 0, s>>>(c, d);
+      else
+        k_face_flux<<<nblocks(n, 128), 128, 0, s>>>(c, d);
     }
     mark_stage(P, S_SYNC_FLUX);
     k_sync_flux<<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams);
@@ -1291,12 +1520,18 @@ int enqueue_phase(Problem &P, Phase phase) {
       const int act[3] = {L.nr, L.n, L.n};
       for (int a = 0; a < 3; ++a)
         n *= a == d ? act[a] + 1 : std::min(L.ghost + act[a] + 4, ext[a]) - (L.ghost - 4);
-      k_face_velocity<<<nblocks(n, T), T, 0, s>>>(c, d);
+      if (P.mixed)
+        k_face_velocity_fp32<<<nblocks(n, T), T, 0, s>>>(c, d);
+      else
+        k_face_velocity<<<nblocks(n, T), T, 0, s>>>(c, d);
     }
     for (int e = 0; e < 3; ++e) {
       const long n = 6L * (L.nr + (e != 0)) * (L.n + (e != 1)) * (L.n + (e != 2));
       mark_stage(P, S_EDGE_EMF0 + e);
-      k_edge_emf<<<nblocks(n, 128), 128, 0, s>>>(c, e);
+      if (P.mixed)
+        k_edge_emf_fp32<<<nblocks(n, 128), 128, 0, s>>>(c, e);
+      else
+        k_edge_emf<<<nblocks(n, 128), 128, 0, s>>>(c, e);
     }
     if (bd.earth_emf) {
       mark_stage(P, S_EARTH_EMF);

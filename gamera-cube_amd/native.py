@@ -128,3 +128,14 @@ def _mi_field(self, what):
 
 GpuProblem.mi_report = _mi_report
 GpuProblem.mi_field = _mi_field
+
+
+def _set_precision(self, mixed):
+    """False: FP64 throughout. True: FP32 reconstruction/fluxes/EMF, FP64
+    update, CT, CFL and ionosphere."""
+    self._lib.gcs_problem_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    if self._lib.gcs_problem_set_precision(self._handle, 1 if mixed else 0) != 0:
+        raise RuntimeError("gcs_problem_set_precision failed")
+
+
+GpuProblem.set_precision = _set_precision

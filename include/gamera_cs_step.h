@@ -68,6 +68,13 @@ void gcs_problem_counters(gcs_problem *problem, int face, double out[2]);
  * the active domain are unchanged. No reference counterpart. */
 int gcs_problem_set_fidelity(gcs_problem *problem, int on);
 
+/* 0 (default): FP64 throughout, the drop-in's arithmetic. 1: the mixed
+ * form: Up7/PDM reconstruction, face fluxes, face velocities and edge EMFs
+ * computed in FP32 (results stored in the FP64 flux/EMF arrays), with the
+ * conserved-variable update, CT, CFL reduction and ionosphere solve in FP64.
+ * No reference counterpart. */
+int gcs_problem_set_precision(gcs_problem *problem, int mixed);
+
 /* Timing of `steps` driver steps with HIP events on the problem's stream:
  * total milliseconds, and with profile != 0 the milliseconds of each step
  * stage summed over the steps (gcs_stage_count() entries named by

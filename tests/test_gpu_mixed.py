@@ -1,0 +1,53 @@
+"""The mixed-precision step (FP32 reconstruction, fluxes and EMF; FP64
+update, CT and CFL) against the FP64 step on the same problem. The FP64
+step is the oracle's to 1e-12 (test_gpu_parity.py); the mixed form must
+stay within a stated tolerance of it and keep div B at FP64 roundoff."""
+import numpy as np
+import pytest
+
+from conftest import gcs_config, max_divergence, problem
+
+pytestmark = pytest.mark.gpu
+
+# Relative L2 error of rho, p, |v| and B after the steps below (FP32 has
+# ~6e-8 unit roundoff; the differences grow slowly with the steps).
+MIXED_TOL = 1e-4
+
+
+def primitives(p, cfg):
+    ni, nj, nk = cfg.extent
+    g = cfg.ghost
+    out = {"rho": [], "p": [], "v": [], "b": []}
+    for f in range(gcs_config.FACE_COUNT):
+        u = p.field(f, problem.CONSERVED).reshape(ni, nj, nk, 5)[
+            g:g + cfg.radial_cells, g:g + cfg.angular_cells, g:g + cfg.angular_cells]
+        b = p.field(f, problem.CELL_MAGNETIC).reshape(ni, nj, nk, 3)[
+            g:g + cfg.radial_cells, g:g + cfg.angular_cells, g:g + cfg.angular_cells]
+        rho = u[..., 0]
+        v = u[..., 1:4] / rho[..., None]
+        pr = (5.0 / 3.0 - 1.0) * (u[..., 4] - 0.5 * rho * (v * v).sum(-1))
+        out["rho"].append(rho.ravel())
+        out["p"].append(pr.ravel())
+        out["v"].append(np.linalg.norm(v, axis=-1).ravel())
+        out["b"].append(np.linalg.norm(b, axis=-1).ravel())
+    return {k: np.concatenate(v) for k, v in out.items()}
+
+
+@pytest.mark.parametrize("name,steps", [("small", 40), ("config0", 100)])
+def test_mixed_step_close_to_fp64(gpu, name, steps):
+    cfg = gcs_config.workload(name)
+    print(f"mixed vs fp64 {name} x{steps}", flush=True)
+    a = gpu.GpuProblem(cfg)
+    b = gpu.GpuProblem(cfg)
+    b.set_precision(True)
+    a.advance(steps)
+    b.advance(steps)
+    pa, pb = primitives(a, cfg), primitives(b, cfg)
+    for key in ("rho", "p", "v", "b"):
+        l2 = np.linalg.norm(pb[key] - pa[key]) / np.linalg.norm(pa[key])
+        linf = np.abs(pb[key] - pa[key]).max() / np.abs(pa[key]).max()
+        print(f"  {key}: rel L2 {l2:.2e} rel Linf {linf:.2e}", flush=True)
+        assert l2 <= MIXED_TOL
+    div = max_divergence(b, cfg)
+    print(f"  max |div B| (mixed) {div:.2e}", flush=True)
+    assert div < 1e-9
