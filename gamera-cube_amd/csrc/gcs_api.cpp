@@ -65,6 +65,7 @@ void free_problem(Problem *P) {
   (void)hipFree(P->planes);
   (void)hipFree(P->bytes);
   (void)hipFree(P->scalars);
+  (void)hipFree(P->cfl_partial);
   (void)hipFree(P->halo_cells);
   (void)hipFree(P->halo_faces);
   (void)hipFree(P->seams);
@@ -84,6 +85,9 @@ int allocate_problem(Problem &P) {
       hipMalloc(&P.planes, nbytes) != hipSuccess ||
       hipMalloc(&P.bytes, nflags) != hipSuccess ||
       hipMalloc(&P.scalars, sizeof(StepScalars)) != hipSuccess ||
+      hipMalloc(&P.cfl_partial,
+                ((6L * P.L.nr * P.L.n * P.L.n + 255) / 256 + 1) * sizeof(unsigned long long)) !=
+          hipSuccess ||
       hipMemset(P.planes, 0, nbytes) != hipSuccess ||
       hipMemset(P.bytes, 0, nflags) != hipSuccess ||
       hipEventCreate(&P.ev_start) != hipSuccess ||

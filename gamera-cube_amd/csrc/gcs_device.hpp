@@ -127,6 +127,7 @@ struct Problem {
   double *planes = nullptr;        // PLANES * L.total doubles
   uint8_t *bytes = nullptr;        // BYTE_PLANES * L.total
   StepScalars *scalars = nullptr;  // device
+  unsigned long long *cfl_partial = nullptr;  // per-block CFL minima
   int level = 0;                   // which U/B/PHI level is current
   long steps = 0;
   double time = 0.0, dt = 0.0, previous_dt = 0.0;
@@ -154,6 +155,9 @@ struct Problem {
   WindSeries wind_series;
   bool fidelity = true;            // reproduce ghost values of the storage
   bool mixed = false;              // FP32 reconstruction, fluxes and EMF
+  // Cell B of both levels equals the recovery from their face fluxes (true
+  // after a completed step; false at creation and after a host upload).
+  bool b_recovered = false;
   // Stage timing: when `profiling`, enqueue_step records an event before
   // every stage into `stage_events` (one row of kMaxStages per step).
   bool profiling = false;

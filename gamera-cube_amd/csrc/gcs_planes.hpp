@@ -47,7 +47,11 @@ constexpr int P_FLUID = P_PHIPRED + 3;            // +15 (5 per dir)
 constexpr int P_MAXWELL = P_FLUID + 15;           // +9 (3 per dir)
 constexpr int P_EMF = P_MAXWELL + 9;              // +3
 constexpr int P_FVEL = P_EMF + 3;                 // +9 face velocity, 3 per dir
-constexpr int PLANES = P_FVEL + 9;
+constexpr int P_WPRED = P_FVEL + 9;                // +5 rho v p of the predicted state
+                                                   // (sweep floors; NaN: no conversion)
+constexpr int P_WV = P_WPRED + 5;                  // +3 vol * m / max(rho, EMF floor)
+constexpr int P_BN = P_WV + 3;                     // +3 Phi* / area per direction
+constexpr int PLANES = P_BN + 3;
 // Setup reuses the flux planes as scratch before the first step.
 constexpr int P_SCRATCH_STRESS = P_FLUID;         // +9 background face stress
 constexpr int P_SCRATCH_AREA = P_FLUID + 9;       // +3 uncalibrated face area

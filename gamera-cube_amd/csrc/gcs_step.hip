@@ -74,11 +74,13 @@ GCS_DEV bool box(long t, const int lo[3], const int hi[3], int &f, int &i,
 
 GCS_DEV long tid() { return blockIdx.x * (long)blockDim.x + threadIdx.x; }
 
+constexpr int kCflThreads = 256;
+
 // ---- dt rule ---------------------------------------------------------------
 
 // Local CFL dt of every active cell, min-reduced (nonorthogonal_step.c:
 // 449-525). Positive doubles order like their bit patterns.
-__global__ void k_cfl(K c) {
+__global__ void __launch_bounds__(kCflThreads) k_cfl(K c, unsigned long long *partial) {
   const Layout &L = c.L;
   const int lo[3] = {L.ghost, L.ghost, L.ghost};
   const int hi[3] = {L.ghost + L.nr, L.ghost + L.n, L.ghost + L.n};
@@ -106,20 +108,41 @@ __global__ void k_cfl(K c) {
       else best = cand;
     }
   }
-  // wavefront min, then one atomic per wave
+  // wavefront min, block min through LDS, one partial per block (min is
+  // exact, so the order of the reduction does not matter)
   unsigned long long bits = __double_as_longlong(best);
   for (int off = 32; off > 0; off >>= 1) {
     const unsigned long long other = __shfl_down(bits, off, 64);
     bits = other < bits ? other : bits;
   }
-  if ((threadIdx.x & 63) == 0) atomicMin(&c.sc->cfl_min_bits, bits);
+  __shared__ unsigned long long wave_min[kCflThreads / 64];
+  if ((threadIdx.x & 63) == 0) wave_min[threadIdx.x >> 6] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = wave_min[0];
+    for (int w = 1; w < kCflThreads / 64; ++w) m = wave_min[w] < m ? wave_min[w] : m;
+    partial[blockIdx.x] = m;
+  }
 }
 
-// Driver dt rule (test_cubedsphere_dipole_background.c:593-616, :669-670).
-__global__ void k_dt(K c, double fraction) {
+// Driver dt rule (test_cubedsphere_dipole_background.c:593-616, :669-670),
+// after the min over the CFL kernel's per-block partials (one block).
+__global__ void __launch_bounds__(kCflThreads) k_dt(K c, double fraction,
+                                                    const unsigned long long *partial,
+                                                    int nblock) {
+  unsigned long long m = __double_as_longlong(DBL_MAX);
+  for (int b = threadIdx.x; b < nblock; b += kCflThreads) m = partial[b] < m ? partial[b] : m;
+  __shared__ unsigned long long red[kCflThreads];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kCflThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w && red[threadIdx.x + w] < red[threadIdx.x])
+      red[threadIdx.x] = red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
   StepScalars *s = c.sc;
-  const double limit = __longlong_as_double(s->cfl_min_bits);
-  s->cfl_min_bits = __double_as_longlong(DBL_MAX);
+  const double limit = __longlong_as_double(red[0]);
   if (s->steps == 0) {
     s->dt = fraction * limit;
     s->ratio = 0.0;
@@ -640,7 +663,11 @@ __global__ void k_predict(K c) {
     if (!face) continue;
     const double cf = c.p(plane_phi(c.cur, d))[at];
     const double of = c.p(plane_phi(c.old, d))[at];
-    c.p(P_PHIPRED + d)[at] = cf + r * (cf - of);
+    const double pred = cf + r * (cf - of);
+    c.p(P_PHIPRED + d)[at] = pred;
+    // Face-normal B of the predicted flux, read by the face fluxes and the
+    // edge EMF (both divide this flux by this area).
+    c.p(P_BN + d)[at] = pred / c.p(P_FACE + FACE_PLANES * d + FO_AREA)[at];
   }
   if (!(in[0] && in[1] && in[2])) return;
   double uo[5], uc[5], up[5];
@@ -649,10 +676,29 @@ __global__ void k_predict(K c) {
     uc[q] = c.p(plane_u(c.cur, q))[at];
   }
   if (!predict_cell(uo, uc, r, c.o.upd_gamma, c.o.upd_rho_floor, c.o.upd_p_floor, up)) {
+    for (int q = 0; q < 5; ++q) c.p(P_WPRED + q)[at] = __longlong_as_double(0x7ff8000000000000ll);
     c.fail(31, at);
     return;
   }
   for (int q = 0; q < 5; ++q) c.p(P_UPRED + q)[at] = up[q];
+  // The sweep's primitive conversion of this cell (nonorthogonal_state.c:60-93
+  // with the sweep's floors), once per cell instead of once per stencil use;
+  // the face fluxes read it back. NaN marks a failed conversion.
+  {
+    Prim w;
+    const bool ok = cons_to_prim(up, c.o.gamma, c.o.rho_floor, c.o.p_floor, w);
+    const double nan = __longlong_as_double(0x7ff8000000000000ll);
+    c.p(P_WPRED + 0)[at] = ok ? w.rho : nan;
+    for (int e = 0; e < 3; ++e) c.p(P_WPRED + 1 + e)[at] = ok ? w.v[e] : nan;
+    c.p(P_WPRED + 4)[at] = ok ? w.p : nan;
+  }
+  // Volume-weighted velocity with the EMF's density floor, the summand of the
+  // face velocity's central8 (nonorthogonal_sweep.c:223-259).
+  {
+    const double vol = c.p(P_VOL)[at];
+    const double rho = fmax(up[0], c.o.emf_rho_floor);
+    for (int e = 0; e < 3; ++e) c.p(P_WV + e)[at] = vol * up[1 + e] / rho;
+  }
   // B of the predicted fluxes; the upper faces' predictions are recomputed
   // here with the same expression rather than read back.
   const int piv = c.b(Q_PIV)[at];
@@ -690,35 +736,33 @@ __global__ void __launch_bounds__(128) k_face_flux(K c, int d) {
   const long at = L.at(f, i, j, k);
   const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
   const StepOptions &o = c.o;
-  double vol[8];
-  Prim ws[8];
+  // Primitives of the eight stencil cells come from the predictor's cache;
+  // one variable at a time keeps a single 8-point column live.
+  double q8[8];
+  bool converted = true;
   for (int n = 0; n < 8; ++n) {
-    const long cell = at + (n - 4) * st;
-    vol[n] = c.p(P_VOL)[cell];
-    double u[5];
-    for (int q = 0; q < 5; ++q) u[q] = c.p(P_UPRED + q)[cell];
-    if (!cons_to_prim(u, o.gamma, o.rho_floor, o.p_floor, ws[n])) {
-      c.fail(41, at);
-      return;
-    }
+    q8[n] = c.p(P_WPRED)[at + (n - 4) * st];
+    converted &= isfinite(q8[n]);
   }
+  if (!converted) {
+    c.fail(41, at);
+    return;
+  }
+  double vol[8];
+  for (int n = 0; n < 8; ++n) vol[n] = c.p(P_VOL)[at + (n - 4) * st];
   const double iv = central8(vol);
   if (!isfinite(iv) || fabs(iv) <= DBL_MIN) {
     c.fail(42, at);
     return;
   }
   Prim w[2];
-  {
-    double q8[8];
-    for (int n = 0; n < 8; ++n) q8[n] = ws[n].rho;
-    up7_pdm(vol, iv, q8, o.pdm, w[0].rho, w[1].rho);
-    for (int e = 0; e < 3; ++e) {
-      for (int n = 0; n < 8; ++n) q8[n] = ws[n].v[e];
-      up7_pdm(vol, iv, q8, o.pdm, w[0].v[e], w[1].v[e]);
-    }
-    for (int n = 0; n < 8; ++n) q8[n] = ws[n].p;
-    up7_pdm(vol, iv, q8, o.pdm, w[0].p, w[1].p);
+  up7_pdm(vol, iv, q8, o.pdm, w[0].rho, w[1].rho);
+  for (int e = 0; e < 3; ++e) {
+    for (int n = 0; n < 8; ++n) q8[n] = c.p(P_WPRED + 1 + e)[at + (n - 4) * st];
+    up7_pdm(vol, iv, q8, o.pdm, w[0].v[e], w[1].v[e]);
   }
+  for (int n = 0; n < 8; ++n) q8[n] = c.p(P_WPRED + 4)[at + (n - 4) * st];
+  up7_pdm(vol, iv, q8, o.pdm, w[0].p, w[1].p);
   double bb[2][3];
   for (int e = 0; e < 3; ++e) {
     double q8[8];
@@ -740,7 +784,7 @@ __global__ void __launch_bounds__(128) k_face_flux(K c, int d) {
   }
   FluidFlux fl;
   fluid_flux(w, o.gamma, fr, fl);
-  const double bn = c.p(P_PHIPRED + d)[at] / area;
+  const double bn = c.p(P_BN + d)[at];
   const double b0[3] = {c.p(P_B0F + 3 * d)[at], c.p(P_B0F + 3 * d + 1)[at],
                         c.p(P_B0F + 3 * d + 2)[at]};
   MaxwellFlux mx;
@@ -820,8 +864,7 @@ __global__ void k_face_velocity(K c, int d) {
   for (int n = 0; n < 8; ++n) {
     const long cell = at + (n - 4) * st;
     vol[n] = c.p(P_VOL)[cell];
-    const double rho = fmax(c.p(P_UPRED)[cell], c.o.emf_rho_floor);
-    for (int e = 0; e < 3; ++e) wv[e][n] = vol[n] * c.p(P_UPRED + 1 + e)[cell] / rho;
+    for (int e = 0; e < 3; ++e) wv[e][n] = c.p(P_WV + e)[cell];
   }
   const double iv = central8(vol);
   const bool ok = isfinite(iv) && fabs(iv) > DBL_MIN;
@@ -873,17 +916,17 @@ __global__ void __launch_bounds__(128) k_edge_emf(K c, int e) {
   double fl, fr, sl, sr;
   {
     double fld[8];
-    const double *ph = c.p(P_PHIPRED + t1);
-    for (int n = 0; n < 8; ++n) fld[n] = ph[at + (n - 4) * step[t2]] / a1[n];
+    const double *bn = c.p(P_BN + t1);
+    for (int n = 0; n < 8; ++n) fld[n] = bn[at + (n - 4) * step[t2]];
     up7_pdm(a1, central8(a1), fld, o.emf_pdm, fl, fr);
   }
   {
     double a2[8], fld[8];
-    const double *ph = c.p(P_PHIPRED + t2);
+    const double *bn = c.p(P_BN + t2);
     for (int n = 0; n < 8; ++n) {
       const long fa = at + (n - 4) * step[t1];
       a2[n] = c.p(P_FACE + FACE_PLANES * t2 + FO_AREA)[fa];
-      fld[n] = ph[fa] / a2[n];
+      fld[n] = bn[fa];
     }
     const double ia = central8(a2);
     if (!isfinite(ia) || fabs(ia) <= DBL_MIN) {
@@ -934,36 +977,34 @@ __global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
   const long at = L.at(f, i, j, k);
   const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
   const StepOptions &o = c.o;
-  const R gamma = (R)o.gamma, rf = (R)o.rho_floor, pf = (R)o.p_floor, pdm = (R)o.pdm;
-  R vol[8];
-  PrimR<R> ws[8];
+  const R gamma = (R)o.gamma, pdm = (R)o.pdm;
+  // Primitives of the stencil from the predictor's FP64 cache, rounded to R.
+  R q8[8];
+  bool converted = true;
   for (int n = 0; n < 8; ++n) {
-    const long cell = at + (n - 4) * st;
-    vol[n] = (R)c.p(P_VOL)[cell];
-    R u[5];
-    for (int q = 0; q < 5; ++q) u[q] = (R)c.p(P_UPRED + q)[cell];
-    if (!cons_to_prim_r(u, gamma, rf, pf, ws[n])) {
-      c.fail(41, at);
-      return;
-    }
+    const double x = c.p(P_WPRED)[at + (n - 4) * st];
+    converted &= isfinite(x);
+    q8[n] = (R)x;
   }
+  if (!converted) {
+    c.fail(41, at);
+    return;
+  }
+  R vol[8];
+  for (int n = 0; n < 8; ++n) vol[n] = (R)c.p(P_VOL)[at + (n - 4) * st];
   const R iv = central8_r(vol);
   if (!isfinite(iv) || fabs(iv) <= tiny<R>()) {
     c.fail(42, at);
     return;
   }
   PrimR<R> w[2];
-  {
-    R q8[8];
-    for (int n = 0; n < 8; ++n) q8[n] = ws[n].rho;
-    up7_pdm_r(vol, iv, q8, pdm, w[0].rho, w[1].rho);
-    for (int e = 0; e < 3; ++e) {
-      for (int n = 0; n < 8; ++n) q8[n] = ws[n].v[e];
-      up7_pdm_r(vol, iv, q8, pdm, w[0].v[e], w[1].v[e]);
-    }
-    for (int n = 0; n < 8; ++n) q8[n] = ws[n].p;
-    up7_pdm_r(vol, iv, q8, pdm, w[0].p, w[1].p);
+  up7_pdm_r(vol, iv, q8, pdm, w[0].rho, w[1].rho);
+  for (int e = 0; e < 3; ++e) {
+    for (int n = 0; n < 8; ++n) q8[n] = (R)c.p(P_WPRED + 1 + e)[at + (n - 4) * st];
+    up7_pdm_r(vol, iv, q8, pdm, w[0].v[e], w[1].v[e]);
   }
+  for (int n = 0; n < 8; ++n) q8[n] = (R)c.p(P_WPRED + 4)[at + (n - 4) * st];
+  up7_pdm_r(vol, iv, q8, pdm, w[0].p, w[1].p);
   R bb[2][3];
   for (int e = 0; e < 3; ++e) {
     R q8[8];
@@ -985,7 +1026,7 @@ __global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
   }
   FluidFluxR<R> fl;
   fluid_flux_r(w, gamma, fr, fl);
-  const R bn = (R)(c.p(P_PHIPRED + d)[at] / area);
+  const R bn = (R)c.p(P_BN + d)[at];
   const R b0[3] = {(R)c.p(P_B0F + 3 * d)[at], (R)c.p(P_B0F + 3 * d + 1)[at],
                    (R)c.p(P_B0F + 3 * d + 2)[at]};
   const R cl = (R)o.c;
@@ -1042,13 +1083,11 @@ __global__ void k_face_velocity_fp32(K c, int d) {
   if (!box(tid(), lo, hi, f, i, j, k)) return;
   const long at = L.at(f, i, j, k);
   const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
-  const R rf = (R)c.o.emf_rho_floor;
   R vol[8], wv[3][8];
   for (int n = 0; n < 8; ++n) {
     const long cell = at + (n - 4) * st;
     vol[n] = (R)c.p(P_VOL)[cell];
-    const R rho = fmax((R)c.p(P_UPRED)[cell], rf);
-    for (int e = 0; e < 3; ++e) wv[e][n] = vol[n] * (R)c.p(P_UPRED + 1 + e)[cell] / rho;
+    for (int e = 0; e < 3; ++e) wv[e][n] = (R)c.p(P_WV + e)[cell];
   }
   const R iv = central8_r(vol);
   const bool ok = isfinite(iv) && fabs(iv) > tiny<R>();
@@ -1099,17 +1138,17 @@ __global__ void __launch_bounds__(128) k_edge_emf_fp32(K c, int e) {
   R fl, fr, sl, sr;
   {
     R fld[8];
-    const double *ph = c.p(P_PHIPRED + t1);
-    for (int n = 0; n < 8; ++n) fld[n] = (R)ph[at + (n - 4) * step[t2]] / a1[n];
+    const double *bn = c.p(P_BN + t1);
+    for (int n = 0; n < 8; ++n) fld[n] = (R)bn[at + (n - 4) * step[t2]];
     up7_pdm_r(a1, central8_r(a1), fld, pdm, fl, fr);
   }
   {
     R a2[8], fld[8];
-    const double *ph = c.p(P_PHIPRED + t2);
+    const double *bn = c.p(P_BN + t2);
     for (int n = 0; n < 8; ++n) {
       const long fa = at + (n - 4) * step[t1];
       a2[n] = (R)c.p(P_FACE + FACE_PLANES * t2 + FO_AREA)[fa];
-      fld[n] = (R)ph[fa] / a2[n];
+      fld[n] = (R)bn[fa];
     }
     const R ia = central8_r(a2);
     if (!isfinite(ia) || fabs(ia) <= tiny<R>()) {
@@ -1490,8 +1529,15 @@ int enqueue_phase(Problem &P, Phase phase) {
         k_earth_faces<<<nblocks(n, T), T, 0, s>>>(c, d, w);
       }
       mark_stage(P, S_EARTH_RECOVER);
-      k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.cur, 0), plane_b(c.cur, 0), 1, 0);
-      k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.old, 0), plane_b(c.old, 0), 1, 0);
+      // cubedsphere_earth_boundary.c:169 recovers every cell of both levels.
+      // After a completed step the six face fluxes of an active cell are
+      // those its B was last recovered from (the step ends with the seam
+      // sync and a recovery), so only the non-active cells change.
+      const int ghost_only = P.b_recovered ? 1 : 0;
+      k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.cur, 0), plane_b(c.cur, 0), 1,
+                                                  ghost_only);
+      k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.old, 0), plane_b(c.old, 0), 1,
+                                                  ghost_only);
     }
   } else if (phase == PHASE_FLUX) {
     // cubedsphere_step.c:67-98
@@ -1568,6 +1614,7 @@ int enqueue_phase(Problem &P, Phase phase) {
     }
     P.level = 1 - P.level;
     P.host_dirty = true;
+    P.b_recovered = true;
   }
   GCS_HIP(hipGetLastError());
   return 0;
@@ -1581,9 +1628,10 @@ int enqueue_step(Problem &P, bool explicit_dt) {
   const long n_active = 6L * L.nr * L.n * L.n;
   if (!explicit_dt) {
     mark_stage(P, S_CFL);
-    k_cfl<<<nblocks(n_active, T), T, 0, s>>>(c);
+    const int ncfl = (int)nblocks(n_active, kCflThreads);
+    k_cfl<<<ncfl, kCflThreads, 0, s>>>(c, P.cfl_partial);
     mark_stage(P, S_DT);
-    k_dt<<<1, 1, 0, s>>>(c, P.cfg.dt_fraction);
+    k_dt<<<1, kCflThreads, 0, s>>>(c, P.cfg.dt_fraction, P.cfl_partial, ncfl);
   }
   for (Phase ph : {PHASE_GHOSTS, PHASE_FLUX, PHASE_EMF, PHASE_UPDATE})
     if (enqueue_phase(P, ph) != 0) return -1;

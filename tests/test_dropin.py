@@ -14,6 +14,7 @@ update and leaves those host arrays untouched.
 import ctypes
 import os
 
+import numpy as np
 import pytest
 
 from conftest import REF_LIB, gcs_config, problem, scaled_error
@@ -57,11 +58,21 @@ def address(lib, name):
     return ctypes.cast(getattr(lib, name), ctypes.c_void_p).value
 
 
+# Area-scaled face fluxes are intermediates of order 1e3 whose components
+# cancel; their differences are measured against the array's largest value.
+INTERMEDIATE = ([problem.FLUID_FACE_FLUX + d for d in range(3)]
+                + [problem.MAXWELL_FACE_FLUX + d for d in range(3)])
+
+
 def worst(a, b, fields=WRITTEN):
     out = (0.0, None)
     for fid in fields:
         for f in range(gcs_config.FACE_COUNT):
-            e = scaled_error(a.field(f, fid), b.field(f, fid))
+            x, y = a.field(f, fid), b.field(f, fid)
+            if fid in INTERMEDIATE:
+                e = float(np.abs(x - y).max() / max(1.0, np.abs(x).max())) if x.size else 0.0
+            else:
+                e = scaled_error(x, y)
             if e > out[0]:
                 out = (e, (fid, f))
     return out
