@@ -218,4 +218,10 @@ def workload(name):
     if name == "small_mi_hall":
         return earth_problem(16, 12, perturbation=0.01, seed=7, mi=True,
                              hall_siemens=5.0, mi_azimuth=32, mi_colatitude=12)
+    if name == "small_mi_net":  # a 64 x 32 polar grid the warm-start U-Net fits
+        return earth_problem(16, 12, perturbation=0.01, seed=7, mi=True,
+                             hall_siemens=5.0, mi_azimuth=64, mi_colatitude=32)
+    if name == "small_mi_polar1":  # config 1's 2 x 128 x 256 polar grids, small MHD
+        return earth_problem(16, 12, perturbation=0.01, seed=7, mi=True,
+                             hall_siemens=5.0, mi_azimuth=256, mi_colatitude=128)
     raise KeyError(name)

@@ -61,7 +61,8 @@ void make_wind(Problem &P) {
 void free_problem(Problem *P) {
   if (P == nullptr) return;
   mi_destroy(*P);
-  if (P->graph) (void)hipGraphExecDestroy(P->graph);
+  for (auto &g : P->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   (void)hipFree(P->planes);
   (void)hipFree(P->bytes);
   (void)hipFree(P->scalars);
@@ -274,6 +275,52 @@ int download_valid(Problem &P, int face, int dir, std::vector<uint8_t> &out) {
 }  // namespace gcs
 
 namespace gcs {
+// The driver step as one hipGraph launch. The graph of the current level
+// parity is captured from enqueue_step on first use and again whenever a
+// value that the capture fixed in the kernel arguments has changed.
+static int launch_step_graph(Problem &P) {
+  Problem::StepGraph &g = P.graphs[P.level];
+  const bool match = g.exec != nullptr && g.mixed == P.mixed && g.fidelity == P.fidelity &&
+                     g.b_recovered == P.b_recovered &&
+                     g.mi_potential == P.mi_view.potential;
+  if (!match) {
+    if (g.exec != nullptr) {
+      GCS_HIP(hipGraphExecDestroy(g.exec));
+      g.exec = nullptr;
+    }
+    const int level = P.level;
+    const bool recovered = P.b_recovered;
+    hipGraph_t graph = nullptr;
+    GCS_HIP(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_step(P, false);
+    const hipError_t end = hipStreamEndCapture(P.stream, &graph);
+    // Capturing ran no kernel: the state is still that of the step's start.
+    P.level = level;
+    P.b_recovered = recovered;
+    if (rc != 0 || end != hipSuccess) {
+      if (graph != nullptr) (void)hipGraphDestroy(graph);
+      if (rc == 0) set_error(std::string("step capture: ") + hipGetErrorString(end));
+      return -1;
+    }
+    const hipError_t inst = hipGraphInstantiateWithFlags(&g.exec, graph, 0);
+    (void)hipGraphDestroy(graph);
+    if (inst != hipSuccess) {
+      g.exec = nullptr;
+      set_error(std::string("step graph instantiation: ") + hipGetErrorString(inst));
+      return -1;
+    }
+    g.mixed = P.mixed;
+    g.fidelity = P.fidelity;
+    g.b_recovered = P.b_recovered;
+    g.mi_potential = P.mi_view.potential;
+  }
+  GCS_HIP(hipGraphLaunch(g.exec, P.stream));
+  P.level = 1 - P.level;
+  P.host_dirty = true;
+  P.b_recovered = true;
+  return 0;
+}
+
 // One driver iteration: the MI cadence check on the clock of the previous
 // step (test_cubedsphere_dipole_background.c:617-625), then the step.
 int driver_step(Problem &P) {
@@ -283,6 +330,7 @@ int driver_step(Problem &P) {
     if (s.failed) return check_failure(P);
     if (mi_maybe_update(P, s.time) < 0) return -1;
   }
+  if (P.use_graph && !P.profiling) return launch_step_graph(P);
   return enqueue_step(P, false);
 }
 }  // namespace gcs
@@ -389,6 +437,12 @@ int gcs_problem_edge_valid(Problem *P, int face, int dir, unsigned char **data,
 int gcs_problem_set_precision(Problem *P, int mixed) {
   if (P == nullptr || (mixed != 0 && mixed != 1)) return -1;
   P->mixed = mixed == 1;
+  return 0;
+}
+
+int gcs_problem_set_graph(Problem *P, int on) {
+  if (P == nullptr) return -1;
+  P->use_graph = on != 0;
   return 0;
 }
 

@@ -147,7 +147,15 @@ struct Problem {
   std::vector<std::vector<uint8_t>> host_valid;
   bool host_dirty = true;
   hipStream_t stream = nullptr;
-  hipGraphExec_t graph = nullptr;
+  // Captured driver steps, one per time-level parity, with what the capture
+  // baked into its kernel arguments (recaptured when any of it changes).
+  struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    bool mixed = false, fidelity = false, b_recovered = false;
+    const double *mi_potential = nullptr;
+  };
+  bool use_graph = false;
+  StepGraph graphs[2];
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   std::vector<double> kernel_ms;   // per-stage timing (profiling mode)
   int *seam_cells = nullptr;       // active cells with a seam face

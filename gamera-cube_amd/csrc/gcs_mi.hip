@@ -23,13 +23,16 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <random>
 #include <vector>
 
 #include "gcs_device.hpp"
 #include "gcs_math.hpp"
+#include "gcs_unet.hpp"
 
 namespace gcs {
 
@@ -74,6 +77,10 @@ struct MiState {
   Bilinear *bfringe = nullptr, *ffringe = nullptr, *samples = nullptr;
   int n_bfringe = 0, n_ffringe = 0, n_samples = 0;
   int *active = nullptr;  // per-hemisphere "still iterating" flags
+  // learned warm start (not owned): the initial guess becomes the previous
+  // potential plus the network's answer for the current residual
+  unet::Net *net = nullptr;
+  double *warm_scale = nullptr;  // [2] RMS of the residual's current equivalent
 };
 
 namespace {
@@ -270,6 +277,36 @@ GCS_DEV double theta_coeff(double sigma, double half, double center, double dt) 
   return sin(half) * sigma / (incl2(half) * sin(center) * dt * dt);
 }
 
+// One interior row (colatitude index i) of the five-point tensor stencil
+// (:777-833) for uniform conductances: the averages of neighbouring Pedersen
+// values are the value itself; the Hall term enters through
+// d(-SigmaH / cos I)/dtheta. y = diag x + lo x(i-1) + hi x(i+1)
+// + llo x(j-1) + lhi x(j+1).
+struct TensorRow {
+  double diag, lo, hi, llo, lhi;
+};
+GCS_DEV TensorRow tensor_row(double sp, double hall, int i, double dt, double dp, int hemi) {
+  const double th = (double)i * dt;
+  const double w = sin(th);
+  const double inc = cos_incl(th, hemi);
+  const double lt = th - 0.5 * dt, ut = th + 0.5 * dt;
+  const double lo = sin(lt) * 0.5 * (sp + sp) / (incl2(lt) * dt * dt);
+  const double hi = sin(ut) * 0.5 * (sp + sp) / (incl2(ut) * dt * dt);
+  const double llo = 0.5 * (sp + sp) / (w * dp * dp);
+  const double lhi = 0.5 * (sp + sp) / (w * dp * dp);
+  const double f_up = -hall / cos_incl((double)(i + 1) * dt, hemi);
+  const double f_dn = -hall / cos_incl((double)(i - 1) * dt, hemi);
+  const double dft = (f_up - f_dn) / (2.0 * dt);
+  const double dfp = (-hall / inc - -hall / inc) / (2.0 * dp);
+  TensorRow t;
+  t.diag = lo + hi + llo + lhi;
+  t.lo = -lo - dfp / (2.0 * dt);
+  t.hi = -hi + dfp / (2.0 * dt);
+  t.llo = -llo + dft / (2.0 * dp);
+  t.lhi = -lhi - dft / (2.0 * dp);
+  return t;
+}
+
 // Right-hand side (:428-448, weighted by sin theta), initial guess from the
 // previous potential (:449-464), and the diagonal (Pedersen, :322-344) or the
 // five-point tensor stencil (:777-833).
@@ -317,23 +354,12 @@ __global__ void k_mi_setup(MiK c, int tensor, double hall) {
     diag[q] = w * (lo + hi + 2.0 * lon);
     return;
   }
-  // uniform conductances: the averages of neighbouring Pedersen values are
-  // the value itself; the Hall term enters through d(-SigmaH / cos I)/dtheta
-  const double sp = c.pedersen;
-  const double lt = th - 0.5 * dt, ut = th + 0.5 * dt;
-  const double lo = sin(lt) * 0.5 * (sp + sp) / (incl2(lt) * dt * dt);
-  const double hi = sin(ut) * 0.5 * (sp + sp) / (incl2(ut) * dt * dt);
-  const double llo = 0.5 * (sp + sp) / (w * dp * dp);
-  const double lhi = 0.5 * (sp + sp) / (w * dp * dp);
-  diag[q] = lo + hi + llo + lhi;
-  const double f_up = -hall / cos_incl((double)(i + 1) * dt, hemi_sign(h));
-  const double f_dn = -hall / cos_incl((double)(i - 1) * dt, hemi_sign(h));
-  const double dft = (f_up - f_dn) / (2.0 * dt);
-  const double dfp = (-hall / inc - -hall / inc) / (2.0 * dp);
-  c.vec(h, V_LO)[q] = -lo - dfp / (2.0 * dt);
-  c.vec(h, V_HI)[q] = -hi + dfp / (2.0 * dt);
-  c.vec(h, V_LLO)[q] = -llo + dft / (2.0 * dp);
-  c.vec(h, V_LHI)[q] = -lhi - dft / (2.0 * dp);
+  const TensorRow t = tensor_row(c.pedersen, hall, i, dt, dp, hemi_sign(h));
+  diag[q] = t.diag;
+  c.vec(h, V_LO)[q] = t.lo;
+  c.vec(h, V_HI)[q] = t.hi;
+  c.vec(h, V_LLO)[q] = t.llo;
+  c.vec(h, V_LHI)[q] = t.lhi;
 }
 
 // y = A x on the reduced unknowns: the sin(theta)-weighted Pedersen operator
@@ -595,6 +621,109 @@ MiK make_mik(Problem &P) {
   return c;
 }
 
+// ---- learned warm start ----------------------------------------------------------
+
+// Current density equivalent to a residual row: the right-hand side of
+// interior row (i, j) is -R^2 FAC cos(I) sin(theta) (:428-448), so a residual
+// r there is what a current -r / (sin(theta) R^2 cos I) would add. The pole
+// and outer rows carry none.
+GCS_DEV double equivalent_current(double r, int i, int nt, double dt, double r2, int hemi) {
+  if (i <= 0 || i >= nt - 1) return 0.0;
+  const double th = (double)i * dt;
+  return -(r / sin(th)) / (r2 * cos_incl(th, hemi));
+}
+
+// Network features of one point (16 floats): current / scale, SigmaP / 10,
+// SigmaH / 10, colatitude index / (nt - 1), hemisphere sign, zeros.
+GCS_DEV void write_features(float *f, double current, double scale, double sp, double sh,
+                            int i, int nt, int hemi) {
+  f[0] = scale > 0.0 ? (float)(current / scale) : 0.f;
+  f[1] = (float)(sp / 10.0);
+  f[2] = (float)(sh / 10.0);
+  f[3] = (float)((double)i / (double)(nt - 1));
+  f[4] = (float)hemi;
+  for (int k = 5; k < unet::kInChannels; ++k) f[k] = 0.f;
+}
+
+// RMS of the residual's equivalent current over the interior rows, per
+// hemisphere (one block each).
+__global__ void k_warm_scale(MiK c, double *scale) {
+  __shared__ double red[kT];
+  const int h = blockIdx.y;
+  const double dt = c.max_colat / (double)(c.nt - 1);
+  const long count = (long)(c.nt - 2) * c.np;
+  double sum = 0.0;
+  for (long q = threadIdx.x; q < count; q += kT) {
+    const int i = 1 + (int)(q / c.np), j = (int)(q % c.np);
+    const double v = equivalent_current(c.vec(h, V_R)[red_at(c.np, i, j)], i, c.nt, dt, c.r2,
+                                        hemi_sign(h));
+    sum += v * v;
+  }
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int w = kT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) scale[h] = sqrt(red[0] / (double)count);
+}
+
+__global__ void k_warm_features(MiK c, const double *scale, double hall, float *feat) {
+  const int h = blockIdx.y;
+  const long q = gtid();
+  if (q >= (long)c.nt * c.np) return;
+  const int i = (int)(q / c.np), j = (int)(q % c.np);
+  const double dt = c.max_colat / (double)(c.nt - 1);
+  const double r = (i > 0 && i < c.nt - 1) ? c.vec(h, V_R)[red_at(c.np, i, j)] : 0.0;
+  write_features(feat + ((long)h * c.nt * c.np + q) * unet::kInChannels,
+                 equivalent_current(r, i, c.nt, dt, c.r2, hemi_sign(h)), scale[h], c.pedersen,
+                 hall, i, c.nt, hemi_sign(h));
+}
+
+// x += the network's potential correction (out * out_scale * R^2 * scale /
+// SigmaP on the interior rows; the pole unknown takes the mean of row 0).
+__global__ void k_warm_correct(MiK c, const double *scale, const float *out, double out_scale) {
+  const int h = blockIdx.y;
+  const long q = gtid();
+  if (q >= c.n) return;
+  const double s = scale[h];
+  if (!(s > 0.0) || !isfinite(s)) return;
+  const double k = out_scale * c.r2 * s / c.pedersen;
+  const float *o = out + (long)h * c.nt * c.np;
+  double *x = c.vec(h, V_X);
+  if (q == 0) {
+    double mean = 0.0;
+    for (int j = 0; j < c.np; ++j) mean += (double)o[j];
+    x[0] += k * (mean / (double)c.np);
+    return;
+  }
+  const int i = 1 + (int)((q - 1) / c.np), j = (int)((q - 1) % c.np);
+  x[q] += k * (double)o[(long)i * c.np + j];
+}
+
+// The guess in V_X (previous potential) becomes guess + N(residual): one
+// operator product, the network's forward pass on both hemispheres, one
+// update. The solve that follows recomputes the residual of the new guess.
+int warm_start(Problem &P, const MiK &c, bool tensor) {
+  MiState &m = *P.mi;
+  unet::Net &n = *m.net;
+  hipStream_t s = P.stream;
+  const dim3 g2(blocks_of(m.n), 2);
+  k_mi_apply<<<g2, kT, 0, s>>>(c, tensor ? 1 : 0, V_X, V_AP, m.active);
+  k_mi_vec<<<g2, kT, 0, s>>>(c, OP_RESIDUAL, 0.0, m.active);
+  k_warm_scale<<<dim3(1, 2), kT, 0, s>>>(c, m.warm_scale);
+  k_warm_features<<<dim3(blocks_of((long)m.nt * m.np), 2), kT, 0, s>>>(c, m.warm_scale, m.hall,
+                                                                       n.input);
+  hipStream_t keep = n.stream;
+  n.stream = s;
+  const int rc = unet::forward_device(n, 2);
+  n.stream = keep;
+  if (rc != 0) return -1;
+  k_warm_correct<<<g2, kT, 0, s>>>(c, m.warm_scale, n.out, n.out_scale);
+  GCS_HIP(hipGetLastError());
+  return 0;
+}
+
 // The polar solves of both hemispheres; 0 when both converge.
 int solve(Problem &P, bool tensor) {
   MiState &m = *P.mi;
@@ -634,6 +763,7 @@ int solve(Problem &P, bool tensor) {
   const auto t0 = std::chrono::steady_clock::now();
   if (set_active() != 0) return -1;
   k_mi_setup<<<g2, kT, 0, s>>>(c, tensor ? 1 : 0, m.hall);
+  if (m.net != nullptr && warm_start(P, c, tensor) != 0) return -1;
   apply(V_X, V_AP);
   vec(OP_RESIDUAL);
   dot(V_R, V_R, 1, SL_FIRST);
@@ -1017,6 +1147,7 @@ void mi_destroy(Problem &P) {
   (void)hipFree(m->samples);
   (void)hipFree(m->active);
   if (m->host_scal) (void)hipHostFree(m->host_scal);
+  (void)hipFree(m->warm_scale);
   delete m;
   P.mi = nullptr;
 }
@@ -1081,6 +1212,317 @@ int mi_download(Problem &P, int what, std::vector<double> &out) {
   GCS_HIP(hipStreamSynchronize(P.stream));
   GCS_HIP(hipMemcpy(out.data(), what == 0 ? m.potential : m.fac,
                     out.size() * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+
+// ---- training pairs for the warm-start network ------------------------------------
+//
+// A pair is a potential and the field-aligned current for which it solves the
+// polar system exactly: a random smooth potential (Gaussian bumps in
+// colatitude times von Mises bumps in longitude, tapered to zero at the
+// low-latitude edge, the pole row set to the mean of the first ring as the
+// reduced system's pole equation requires) is multiplied by the tensor
+// operator of a uniform (SigmaP, SigmaH), and the interior rows of the product
+// are turned back into the current that produces them.
+
+namespace {
+
+constexpr int kBumps = 6;
+
+struct GenSample {
+  int hemi;
+  double sp, sh;
+  double amp[kBumps], th[kBumps], sth[kBumps], ph[kBumps], kap[kBumps];
+};
+
+__global__ void k_gen_potential(const GenSample *smp, int nt, int np, double max_colat,
+                                double *phi) {
+  const int b = blockIdx.y;
+  const long q = gtid();
+  if (q >= (long)nt * np) return;
+  const int i = (int)(q / np), j = (int)(q % np);
+  const double dt = max_colat / (double)(nt - 1), dp = 2.0 * kPi / (double)np;
+  const double th = (double)i * dt, ph = (double)j * dp;
+  const GenSample &g = smp[b];
+  double v = 0.0;
+  for (int k = 0; k < kBumps; ++k) {
+    const double d = (th - g.th[k]) / g.sth[k];
+    v += g.amp[k] * exp(-0.5 * d * d) * exp(g.kap[k] * (cos(ph - g.ph[k]) - 1.0));
+  }
+  const double taper = 1.0 - (th / max_colat) * (th / max_colat);
+  phi[(long)b * nt * np + q] = i == nt - 1 ? 0.0 : taper * v;
+}
+
+// Pole row = mean of ring 1 (one block per sample).
+__global__ void k_gen_pole(int nt, int np, double *phi) {
+  __shared__ double red[kT];
+  double *p = phi + (long)blockIdx.x * nt * np;
+  double sum = 0.0;
+  for (int j = threadIdx.x; j < np; j += kT) sum += p[np + j];
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int w = kT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double mean = red[0] / (double)np;
+  for (int j = threadIdx.x; j < np; j += kT) p[j] = mean;
+}
+
+__global__ void k_gen_current(const GenSample *smp, int nt, int np, double max_colat,
+                              double r2, const double *phi, double *fac) {
+  const int b = blockIdx.y;
+  const long q = gtid();
+  if (q >= (long)nt * np) return;
+  const int i = (int)(q / np), j = (int)(q % np);
+  const long base = (long)b * nt * np;
+  if (i == 0 || i == nt - 1) {
+    fac[base + q] = 0.0;
+    return;
+  }
+  const GenSample &g = smp[b];
+  const double dt = max_colat / (double)(nt - 1), dp = 2.0 * kPi / (double)np;
+  const TensorRow t = tensor_row(g.sp, g.sh, i, dt, dp, g.hemi);
+  const double *p = phi + base;
+  const int jm = j == 0 ? np - 1 : j - 1, jp = j + 1 == np ? 0 : j + 1;
+  const double below = p[(long)(i - 1) * np + (i == 1 ? 0 : j)];
+  const double above = i + 2 < nt ? p[(long)(i + 1) * np + j] : 0.0;
+  const double y = t.diag * p[q] + t.lo * below + t.hi * above + t.llo * p[(long)i * np + jm] +
+                   t.lhi * p[(long)i * np + jp];
+  fac[base + q] = equivalent_current(y, i, nt, dt, r2, g.hemi);
+}
+
+// RMS of the current over the interior rows (one block per sample).
+__global__ void k_gen_scale(int nt, int np, const double *fac, double *scale) {
+  __shared__ double red[kT];
+  const double *f = fac + (long)blockIdx.x * nt * np;
+  const long count = (long)(nt - 2) * np;
+  double sum = 0.0;
+  for (long q = threadIdx.x; q < count; q += kT) sum += f[np + q] * f[np + q];
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int w = kT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) scale[blockIdx.x] = sqrt(red[0] / (double)count);
+}
+
+__global__ void k_gen_features(const GenSample *smp, int nt, int np, double r2,
+                               const double *fac, const double *phi, const double *scale,
+                               double out_scale, float *feat, float *target) {
+  const int b = blockIdx.y;
+  const long q = gtid();
+  if (q >= (long)nt * np) return;
+  const int i = (int)(q / np);
+  const GenSample &g = smp[b];
+  const long at = (long)b * nt * np + q;
+  write_features(feat + at * unet::kInChannels, fac[at], scale[b], g.sp, g.sh, i, nt, g.hemi);
+  target[at] = (float)(phi[at] * g.sp / (r2 * scale[b] * out_scale));
+}
+
+// A batch of random pair parameters (host side, seeded).
+void draw_samples(std::mt19937_64 &rng, int count, double max_colat, double sp_lo,
+                  double sp_hi, double sh_lo, double sh_hi, GenSample *out) {
+  std::uniform_real_distribution<double> u(0.0, 1.0);
+  std::normal_distribution<double> z(0.0, 1.0);
+  for (int b = 0; b < count; ++b) {
+    GenSample &g = out[b];
+    g.hemi = u(rng) < 0.5 ? GAMERA_MI_NORTH : GAMERA_MI_SOUTH;
+    g.sp = sp_lo + (sp_hi - sp_lo) * u(rng);
+    g.sh = sh_lo + (sh_hi - sh_lo) * u(rng);
+    for (int k = 0; k < kBumps; ++k) {
+      g.amp[k] = z(rng);
+      g.th[k] = max_colat * (0.05 + 0.85 * u(rng));
+      g.sth[k] = max_colat * (0.03 + 0.22 * u(rng));
+      g.ph[k] = 2.0 * kPi * u(rng);
+      g.kap[k] = 0.3 + 8.0 * u(rng);
+    }
+  }
+}
+
+}  // namespace
+
+// Device buffers of a pair generator for `batch` samples on an nt x np grid.
+struct PairBatch {
+  GenSample *host = nullptr, *dev = nullptr;
+  double *phi = nullptr, *fac = nullptr, *scale = nullptr;
+  int batch = 0;
+  ~PairBatch() {
+    if (host) (void)hipHostFree(host);
+    (void)hipFree(dev);
+    (void)hipFree(phi);
+    (void)hipFree(fac);
+    (void)hipFree(scale);
+  }
+  int allocate(int b, int nt, int np) {
+    batch = b;
+    GCS_HIP(hipHostMalloc(&host, b * sizeof(GenSample)));
+    GCS_HIP(hipMalloc(&dev, b * sizeof(GenSample)));
+    GCS_HIP(hipMalloc(&phi, (size_t)b * nt * np * sizeof(double)));
+    GCS_HIP(hipMalloc(&fac, (size_t)b * nt * np * sizeof(double)));
+    GCS_HIP(hipMalloc(&scale, (size_t)b * sizeof(double)));
+    return 0;
+  }
+};
+
+// Potentials and currents of host->dev samples on stream s.
+static int make_pairs(PairBatch &pb, int nt, int np, double max_colat, double r2,
+                      hipStream_t s) {
+  GCS_HIP(hipMemcpyAsync(pb.dev, pb.host, pb.batch * sizeof(GenSample), hipMemcpyHostToDevice,
+                         s));
+  const dim3 g(blocks_of((long)nt * np), pb.batch);
+  k_gen_potential<<<g, kT, 0, s>>>(pb.dev, nt, np, max_colat, pb.phi);
+  k_gen_pole<<<pb.batch, kT, 0, s>>>(nt, np, pb.phi);
+  k_gen_current<<<g, kT, 0, s>>>(pb.dev, nt, np, max_colat, r2, pb.phi, pb.fac);
+  k_gen_scale<<<pb.batch, kT, 0, s>>>(nt, np, pb.fac, pb.scale);
+  GCS_HIP(hipGetLastError());
+  return 0;
+}
+
+int mi_attach_warm_start(Problem &P, unet::Net *net) {
+  if (P.mi == nullptr) {
+    set_error("MI is not enabled");
+    return -1;
+  }
+  MiState &m = *P.mi;
+  if (net != nullptr && (net->H != m.nt || net->W != m.np || net->max_batch < 2)) {
+    set_error("warm-start network grid does not match the MI grid (colatitude x azimuth), "
+              "or its batch is below 2");
+    return -1;
+  }
+  if (m.warm_scale == nullptr) GCS_HIP(hipMalloc(&m.warm_scale, 2 * sizeof(double)));
+  m.net = net;
+  return 0;
+}
+
+int mi_make_pairs(Problem &P, uint64_t seed, int count, double sp, double sh, double *fac_out,
+                  double *phi_out) {
+  if (P.mi == nullptr || count <= 0) {
+    set_error(P.mi == nullptr ? "MI is not enabled" : "count must be positive");
+    return -1;
+  }
+  const MiState &m = *P.mi;
+  const double rm = m.cfg.ionosphere_radius_re * m.cfg.length_norm_m;
+  PairBatch pb;
+  if (pb.allocate(count, m.nt, m.np) != 0) return -1;
+  std::mt19937_64 rng(seed);
+  draw_samples(rng, count, m.max_colat, sp, sp, sh, sh, pb.host);
+  for (int b = 0; b < count; ++b) pb.host[b].hemi = b % 2 == 0 ? GAMERA_MI_NORTH : GAMERA_MI_SOUTH;
+  if (make_pairs(pb, m.nt, m.np, m.max_colat, rm * rm, P.stream) != 0) return -1;
+  GCS_HIP(hipStreamSynchronize(P.stream));
+  const size_t bytes = (size_t)count * m.nt * m.np * sizeof(double);
+  GCS_HIP(hipMemcpy(fac_out, pb.fac, bytes, hipMemcpyDeviceToHost));
+  GCS_HIP(hipMemcpy(phi_out, pb.phi, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int mi_train_warm_start(Problem &P, unet::Net &n, const gcs_warm_train_config &t,
+                        float *loss_history) {
+  if (P.mi == nullptr) {
+    set_error("MI is not enabled");
+    return -1;
+  }
+  const MiState &m = *P.mi;
+  if (n.H != m.nt || n.W != m.np || unet::check_batch(n, t.batch) != 0 || t.steps <= 0 ||
+      !(t.learning_rate > 0.0) || !(t.pedersen_min > 0.0) ||
+      !(t.pedersen_max >= t.pedersen_min) || !(t.hall_min >= 0.0) ||
+      !(t.hall_max >= t.hall_min)) {
+    set_error("invalid warm-start training configuration");
+    return -1;
+  }
+  const double rm = m.cfg.ionosphere_radius_re * m.cfg.length_norm_m;
+  const double r2 = rm * rm;
+  hipStream_t s = n.stream;
+  PairBatch pb;
+  if (pb.allocate(t.batch, m.nt, m.np) != 0) return -1;
+  float *host_loss = nullptr;
+  GCS_HIP(hipHostMalloc(&host_loss, sizeof(float)));
+  std::mt19937_64 rng(t.seed);
+  const dim3 g(blocks_of((long)m.nt * m.np), t.batch);
+  int rc = 0;
+  for (long step = 0; step < t.steps && rc == 0; ++step) {
+    draw_samples(rng, t.batch, m.max_colat, t.pedersen_min, t.pedersen_max, t.hall_min,
+                 t.hall_max, pb.host);
+    if (make_pairs(pb, m.nt, m.np, m.max_colat, r2, s) != 0) {
+      rc = -1;
+      break;
+    }
+    if (n.adam_t == 0 && step == 0) {
+      // output scale: unit RMS of the first batch's targets
+      k_gen_features<<<g, kT, 0, s>>>(pb.dev, m.nt, m.np, r2, pb.fac, pb.phi, pb.scale, 1.0,
+                                      n.input, n.target);
+      std::vector<float> tg((size_t)t.batch * m.nt * m.np);
+      if (hipMemcpyAsync(tg.data(), n.target, tg.size() * sizeof(float),
+                         hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        set_error("warm-start training: target readback failed");
+        rc = -1;
+        break;
+      }
+      double ss = 0.0;
+      for (float v : tg) ss += (double)v * v;
+      n.out_scale = std::sqrt(ss / (double)tg.size());
+      if (!(n.out_scale > 0.0) || !std::isfinite(n.out_scale)) n.out_scale = 1.0;
+    }
+    k_gen_features<<<g, kT, 0, s>>>(pb.dev, m.nt, m.np, r2, pb.fac, pb.phi, pb.scale,
+                                    n.out_scale, n.input, n.target);
+    // cosine decay to 5% of the rate after a short linear warmup
+    const double warm = std::min(1.0, (double)(step + 1) / 50.0);
+    const double frac = (double)step / (double)std::max(1L, t.steps - 1);
+    const double lr = t.learning_rate * warm * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(kPi * frac)));
+    if (unet::forward_device(n, t.batch) != 0 || unet::loss_device(n, t.batch) != 0 ||
+        unet::backward_device(n, t.batch) != 0 || unet::adam_device(n, (float)lr) != 0) {
+      rc = -1;
+      break;
+    }
+    if (loss_history != nullptr) {
+      if (hipMemcpyAsync(host_loss, n.loss, sizeof(float), hipMemcpyDeviceToHost, s) !=
+              hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        set_error("warm-start training: loss readback failed");
+        rc = -1;
+        break;
+      }
+      loss_history[step] = *host_loss;
+    }
+  }
+  (void)hipHostFree(host_loss);
+  if (rc == 0 && hipStreamSynchronize(s) != hipSuccess) {
+    set_error("warm-start training: stream error");
+    rc = -1;
+  }
+  return rc;
+}
+
+// One polar solve of both hemispheres from host arrays ([2][nt][np] current
+// and guess; the solution replaces the guess), with or without the attached
+// network. Leaves the problem's MI state at the solved potential.
+int mi_solve_current(Problem &P, const double *fac, double *potential, int use_net,
+                     int iterations[2], double *ms) {
+  if (P.mi == nullptr) {
+    set_error("MI is not enabled");
+    return -1;
+  }
+  MiState &m = *P.mi;
+  if (use_net && m.net == nullptr) {
+    set_error("no warm-start network attached");
+    return -1;
+  }
+  const size_t bytes = 2UL * m.nt * m.np * sizeof(double);
+  GCS_HIP(hipStreamSynchronize(P.stream));
+  GCS_HIP(hipMemcpy(m.fac, fac, bytes, hipMemcpyHostToDevice));
+  GCS_HIP(hipMemcpy(m.potential, potential, bytes, hipMemcpyHostToDevice));
+  unet::Net *keep = m.net;
+  if (!use_net) m.net = nullptr;
+  const int rc = solve(P, m.hall > 0.0);
+  m.net = keep;
+  iterations[0] = m.iterations[0];
+  iterations[1] = m.iterations[1];
+  *ms = m.solve_ms;
+  if (rc != 0) return -1;
+  GCS_HIP(hipMemcpy(potential, m.potential, bytes, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -55,20 +55,64 @@ struct K {
 };
 
 // (f, i, j, k) of thread t in the box lo <= (i, j, k) < hi on every face.
+// Launch index -> (face, i, j, k) of a box. Every box here has fewer than
+// 2^31 points over the six faces, so the decomposition runs in 32-bit
+// unsigned arithmetic (64-bit division is a long software sequence).
 GCS_DEV bool box(long t, const int lo[3], const int hi[3], int &f, int &i,
                  int &j, int &k) {
-  const int e0 = hi[0] - lo[0], e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
-  const long per = (long)e0 * e1 * e2;
-  if (t >= 6 * per) return false;
-  f = (int)(t / per);
-  long r = t - f * per;
-  i = (int)(r / ((long)e1 * e2));
-  r -= (long)i * e1 * e2;
-  j = (int)(r / e2);
-  k = (int)(r - (long)j * e2);
-  i += lo[0];
-  j += lo[1];
-  k += lo[2];
+  const unsigned e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
+  const unsigned e12 = e1 * e2;
+  const unsigned per = (unsigned)(hi[0] - lo[0]) * e12;
+  if (t >= 6L * per) return false;
+  const unsigned u = (unsigned)t;
+  const unsigned fu = u / per;
+  unsigned r = u - fu * per;
+  const unsigned iu = r / e12;
+  r -= iu * e12;
+  const unsigned ju = r / e2;
+  f = (int)fu;
+  i = (int)iu + lo[0];
+  j = (int)ju + lo[1];
+  k = (int)(r - ju * e2) + lo[2];
+  return true;
+}
+
+// Launch index -> a cell outside the active box (radial shells of `ghost`
+// cells at both ends, then the tangential frame of every active shell).
+GCS_DEV bool ghost_cell(long t, const Layout &L, int &f, int &i, int &j, int &k) {
+  const unsigned g = L.ghost, NJ = L.NJ, NK = L.NK, n = L.n;
+  const unsigned plane = NJ * NK;
+  const unsigned shell = 2 * g * plane;
+  const unsigned frame = plane - n * n;
+  const unsigned per = shell + (unsigned)L.nr * frame;
+  if (t >= 6L * per) return false;
+  const unsigned u = (unsigned)t;
+  const unsigned fu = u / per;
+  unsigned r = u - fu * per;
+  f = (int)fu;
+  if (r < shell) {
+    const unsigned s = r / plane;
+    r -= s * plane;
+    i = (int)(s < g ? s : L.nr + s);
+    j = (int)(r / NK);
+    k = (int)(r - (unsigned)j * NK);
+    return true;
+  }
+  r -= shell;
+  const unsigned si = r / frame;
+  r -= si * frame;
+  i = (int)(g + si);
+  const unsigned rows = 2 * g * NK;  // full rows j < g and j >= g + n
+  if (r < rows) {
+    const unsigned row = r / NK;
+    j = (int)(row < g ? row : n + row);
+    k = (int)(r - row * NK);
+  } else {
+    r -= rows;
+    const unsigned mj = r / (2 * g), col = r - mj * (2 * g);
+    j = (int)(g + mj);
+    k = (int)(col < g ? col : n + col);
+  }
   return true;
 }
 
@@ -617,6 +661,24 @@ __global__ void k_recover(K c, int phi_plane, int b_plane, int inner_copy,
   if (ghost_only && i >= L.ghost && i < L.ghost + L.nr && j >= L.ghost &&
       j < L.ghost + L.n && k >= L.ghost && k < L.ghost + L.n)
     return;
+  const long at = L.at(f, i, j, k);
+  double b[3];
+  if (!recover_at(c, at, phi_plane, b)) {
+    c.fail(21, at);
+    return;
+  }
+  if (inner_copy && i < L.ghost && !recover_at(c, L.at(f, L.ghost, j, k), phi_plane, b)) {
+    c.fail(22, at);
+    return;
+  }
+  for (int d = 0; d < 3; ++d) c.p(b_plane + d)[at] = b[d];
+}
+
+// k_recover over the cells outside the active box only.
+__global__ void k_recover_ghosts(K c, int phi_plane, int b_plane, int inner_copy) {
+  const Layout &L = c.L;
+  int f, i, j, k;
+  if (!ghost_cell(tid(), L, f, i, j, k)) return;
   const long at = L.at(f, i, j, k);
   double b[3];
   if (!recover_at(c, at, phi_plane, b)) {
@@ -1533,11 +1595,14 @@ int enqueue_phase(Problem &P, Phase phase) {
       // After a completed step the six face fluxes of an active cell are
       // those its B was last recovered from (the step ends with the seam
       // sync and a recovery), so only the non-active cells change.
-      const int ghost_only = P.b_recovered ? 1 : 0;
-      k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.cur, 0), plane_b(c.cur, 0), 1,
-                                                  ghost_only);
-      k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(c.old, 0), plane_b(c.old, 0), 1,
-                                                  ghost_only);
+      for (int lev : {c.cur, c.old}) {
+        if (P.b_recovered)
+          k_recover_ghosts<<<nblocks(n_cells - n_active, T), T, 0, s>>>(c, plane_phi(lev, 0),
+                                                                        plane_b(lev, 0), 1);
+        else
+          k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(lev, 0), plane_b(lev, 0), 1,
+                                                      0);
+      }
     }
   } else if (phase == PHASE_FLUX) {
     // cubedsphere_step.c:67-98
@@ -1610,7 +1675,8 @@ int enqueue_phase(Problem &P, Phase phase) {
     if (P.fidelity) {
       mark_stage(P, S_FIDELITY);
       k_keep_ghosts<<<nblocks(n_box, T), T, 0, s>>>(c);
-      k_recover<<<nblocks(n_cells, T), T, 0, s>>>(n, plane_phi(n.cur, 0), plane_b(n.cur, 0), 0, 1);
+      k_recover_ghosts<<<nblocks(n_cells - n_active, T), T, 0, s>>>(n, plane_phi(n.cur, 0),
+                                                                    plane_b(n.cur, 0), 0);
     }
     P.level = 1 - P.level;
     P.host_dirty = true;

@@ -139,3 +139,13 @@ def _set_precision(self, mixed):
 
 
 GpuProblem.set_precision = _set_precision
+
+
+def _set_graph(self, on):
+    """True: each driver step is one hipGraph launch (same results)."""
+    self._lib.gcs_problem_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    if self._lib.gcs_problem_set_graph(self._handle, 1 if on else 0) != 0:
+        raise RuntimeError("gcs_problem_set_graph failed")
+
+
+GpuProblem.set_graph = _set_graph

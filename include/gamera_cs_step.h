@@ -75,6 +75,13 @@ int gcs_problem_set_fidelity(gcs_problem *problem, int on);
  * No reference counterpart. */
 int gcs_problem_set_precision(gcs_problem *problem, int mixed);
 
+/* 1: run each driver step (outside profiled timing) as one hipGraph launch,
+ * captured per time-level parity on first use and recaptured when the
+ * precision, fidelity, ghost-recovery state or ionosphere view changes.
+ * 0 (default): launch the step's kernels one by one. Results are identical
+ * either way. No reference counterpart. */
+int gcs_problem_set_graph(gcs_problem *problem, int on);
+
 /* Timing of `steps` driver steps with HIP events on the problem's stream:
  * total milliseconds, and with profile != 0 the milliseconds of each step
  * stage summed over the steps (gcs_stage_count() entries named by

@@ -1,0 +1,181 @@
+"""The warm-start network's device kernels against the numpy oracle
+(oracle/unet_oracle.py), its training, and what it gains in the MI solve.
+
+Tolerances: the device and the oracle round the same bf16 values but sum
+their fp32 products in different orders, so a layer output near a bf16
+rounding boundary can differ by one bf16 step (2^-8 relative) and carry
+that forward. Forward outputs are compared in relative L2 (<= 2e-2),
+gradients per parameter tensor in relative L2 (<= 5e-2) and as one
+flattened vector by cosine similarity (>= 0.999).
+"""
+import importlib
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import gcs_config
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                "oracle"))
+import unet_oracle as U  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 2e-2
+GRAD_TOL = 5e-2
+
+
+@pytest.fixture(scope="module")
+def ws(gpu):
+    return importlib.import_module("gamera-cube_amd.warm_start")
+
+
+def bf16_bits(a):
+    u = U.bf16(a).view(np.uint32)
+    return (u >> 16).astype(np.uint16)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def flat_to_dict(flat):
+    out, off = {}, 0
+    for name, cin, cout, _ in U.LAYERS:
+        n = cout * 9 * cin
+        out[name + ".w"] = flat[off:off + n].reshape(cout, 3, 3, cin)
+        off += n
+        out[name + ".b"] = flat[off:off + cout]
+        off += cout
+    out["head.w"] = flat[off:off + U.HEAD_IN]
+    off += U.HEAD_IN
+    out["head.b"] = flat[off:off + 1]
+    off += 1
+    assert off == flat.size
+    return out
+
+
+def test_mfma_operand_layout(ws):
+    rng = np.random.default_rng(0)
+    a = U.bf16(rng.standard_normal((32, 48)).astype(np.float32))
+    b = U.bf16(rng.standard_normal((48, 48)).astype(np.float32))
+    c = ws.gemm_check(bf16_bits(a), bf16_bits(b))
+    ref = a.astype(np.float64) @ b.astype(np.float64)
+    assert np.abs(c - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+def make_case(seed, n=2, h=16, w=32):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, h, w, U.IN_CHANNELS)).astype(np.float32)
+    target = rng.standard_normal((n, h, w)).astype(np.float32)
+    params = U.init_params(seed + 10)
+    # nonzero biases so the bias paths are exercised
+    for k in params:
+        if k.endswith(".b"):
+            params[k][...] = (0.05 * rng.standard_normal(params[k].shape)).astype(np.float32)
+    return x, target, params
+
+
+def pack(params):
+    return np.concatenate([params[k].ravel() for k in params]).astype(np.float32)
+
+
+def test_forward_matches_oracle(ws):
+    x, _, params = make_case(1)
+    net = ws.WarmStartNet(16, 32, 2, seed=0)
+    net.set_params(pack(params))
+    assert np.array_equal(net.params(), pack(params))
+    out = net.forward(x)
+    ref, _ = U.forward(params, x)
+    err = rel_l2(out, ref)
+    print(f"forward rel L2 {err:.2e}", flush=True)
+    assert err <= FWD_TOL
+
+
+def test_gradients_match_oracle(ws):
+    x, target, params = make_case(2)
+    net = ws.WarmStartNet(16, 32, 2, seed=0)
+    net.set_params(pack(params))
+    loss = net.gradients(x, target)
+    out, acts = U.forward(params, x)
+    ref_loss, g_out = U.loss_and_grad_out(out, target)
+    assert abs(loss - ref_loss) <= 1e-2 * ref_loss
+    ref = U.backward(params, acts, g_out)
+    got = flat_to_dict(net.grads())
+    worst = 0.0
+    for k in ref:
+        e = rel_l2(got[k], ref[k])
+        worst = max(worst, e)
+        assert e <= GRAD_TOL, (k, e)
+    a, b = pack(got), pack(ref)
+    cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+    print(f"gradients: worst tensor rel L2 {worst:.2e}, cosine {cos:.6f}", flush=True)
+    assert cos >= 0.999
+
+
+def test_adam_step_matches_oracle(ws):
+    x, target, params = make_case(3)
+    net = ws.WarmStartNet(16, 32, 2, seed=0)
+    net.set_params(pack(params))
+    net.gradients(x, target)
+    grads = flat_to_dict(net.grads())
+    net.adam_step(1e-3)
+    state = {}
+    ref = {k: v.copy() for k, v in params.items()}
+    U.adam_step(ref, grads, state, lr=1e-3)
+    got = net.params()
+    np.testing.assert_allclose(got, pack(ref), rtol=1e-5, atol=1e-7)
+
+
+def test_training_lowers_the_loss(ws, gpu):
+    cfg = gcs_config.workload("small_mi_net")
+    p = gpu.GpuProblem(cfg)
+    net = ws.WarmStartNet(cfg.mi.colatitude_points, cfg.mi.azimuth_points, 8, seed=1)
+    losses = ws.train(p, net, steps=300, batch=8, learning_rate=1e-3, seed=5)
+    first, last = losses[:10].mean(), losses[-20:].mean()
+    print(f"training loss {first:.3e} -> {last:.3e} (output scale {net.output_scale:.3e})",
+          flush=True)
+    assert np.isfinite(losses).all()
+    assert last <= 0.2 * first
+
+
+def test_pairs_solve_to_their_potential(ws, gpu):
+    """The generated current's solve returns the generated potential."""
+    cfg = gcs_config.workload("small_mi_net")
+    p = gpu.GpuProblem(cfg)
+    cur, pot = ws.make_pairs(p, seed=3, count=2, pedersen=cfg.mi.pedersen_siemens,
+                             hall=cfg.hall_siemens)
+    sol, iters, _ = ws.solve_current(p, cur, np.zeros_like(pot), use_network=False)
+    print(f"pair solve iterations {iters}", flush=True)
+    scale = np.abs(pot).max()
+    assert np.abs(sol - pot).max() <= 1e-5 * scale
+
+
+def test_warm_start_saves_iterations(ws, gpu):
+    """After training, the network-corrected guess needs fewer Krylov
+    iterations than the previous potential (the reference's guess) for a
+    current that changed between updates; both solves reach the tolerance
+    and agree."""
+    cfg = gcs_config.workload("small_mi_net")
+    p = gpu.GpuProblem(cfg)
+    nt, np_ = cfg.mi.colatitude_points, cfg.mi.azimuth_points
+    net = ws.WarmStartNet(nt, np_, 8, seed=2)
+    ws.train(p, net, steps=600, batch=8, learning_rate=1e-3, seed=9)
+    ws.attach(p, net)
+    cur_a, pot_a = ws.make_pairs(p, seed=21, count=2, pedersen=cfg.mi.pedersen_siemens,
+                                 hall=cfg.hall_siemens)
+    cur_d, _ = ws.make_pairs(p, seed=22, count=2, pedersen=cfg.mi.pedersen_siemens,
+                             hall=cfg.hall_siemens)
+    cur_b = cur_a + 0.3 * cur_d * (np.abs(cur_a).max() / np.abs(cur_d).max())
+    plain, it_plain, _ = ws.solve_current(p, cur_b, pot_a, use_network=False)
+    learned, it_net, _ = ws.solve_current(p, cur_b, pot_a, use_network=True)
+    cold_plain = ws.solve_current(p, cur_b, np.zeros_like(pot_a), use_network=False)[1]
+    cold_net = ws.solve_current(p, cur_b, np.zeros_like(pot_a), use_network=True)[1]
+    print(f"iterations from the previous potential: plain {it_plain}, learned {it_net}; "
+          f"from zero: plain {cold_plain}, learned {cold_net}", flush=True)
+    assert np.abs(learned - plain).max() <= 1e-5 * np.abs(plain).max()
+    assert sum(it_net) < sum(it_plain)
+    assert sum(cold_net) < sum(cold_plain)
+    ws.attach(p, None)
