@@ -80,6 +80,20 @@ def cpu_baseline(gpu_problem, cfg, steps=2):
                       f"{seconds:.1f} s"}
 
 
+def measured_traffic(workload, stage):
+    """HBM bytes per launch of the stage's kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload (calibrated against a copy
+    kernel), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"step_{workload}_hbm_traffic.json")
+    kernel = "k_" + stage.rsplit("_", 1)[0] if stage.endswith(("_i", "_j", "_k")) else "k_" + stage
+    try:
+        with open(path) as f:
+            entry = json.load(f)["kernels"][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return entry["read_bytes"] + entry["write_bytes"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,7 +185,8 @@ def main():
                    "hbm_fraction_step_model": STEP_BYTES_PER_CELL * value / (HBM_PEAK_GBS * 1e9)},
         "roofline": {"bound": "hbm", "kernel": top,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, top),
                      "launch_ms": launch_ms, "bytes_per_unit": bytes_per_unit,
                      "units_per_launch": units(nr, n)},
         "stage_ms_per_step": {k: round(v / args.steps, 4) for k, v in stages.items() if v > 0},

@@ -78,8 +78,13 @@ def make_case(seed, n=2, h=16, w=32):
     return x, target, params
 
 
+def param_names():
+    return [n + s for n, _, _, _ in U.LAYERS for s in (".w", ".b")] + ["head.w", "head.b"]
+
+
 def pack(params):
-    return np.concatenate([params[k].ravel() for k in params]).astype(np.float32)
+    """Flat vector in the device's packing order (whatever the dict order)."""
+    return np.concatenate([params[k].ravel() for k in param_names()]).astype(np.float32)
 
 
 def test_forward_matches_oracle(ws):
