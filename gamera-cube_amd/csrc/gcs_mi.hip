@@ -49,7 +49,7 @@ struct Bilinear {
 
 enum Slot {  // solver scalars per hemisphere
   SL_RHS2, SL_FIRST, SL_TARGET, SL_RZ, SL_PAP, SL_NORM, SL_RHO, SL_DEN, SL_WNUM,
-  SL_WDEN, SL_ALPHA, SL_OMEGA, SL_RHOPREV, SL_POLE, SL_COUNT = 16
+  SL_WDEN, SL_ALPHA, SL_OMEGA, SL_RHOPREV, SL_POLE, SL_ITER, SL_OK, SL_COUNT = 16
 };
 
 }  // namespace
@@ -72,7 +72,6 @@ struct MiState {
   double *potential = nullptr;  // [2][nt][np]
   double *work = nullptr;       // [2][kVectors][n] + ring scratch
   double *scal = nullptr;       // [2][SL_COUNT]
-  double *partial = nullptr;    // [2][kMaxBlocks]
   double *host_scal = nullptr;  // pinned mirror of scal
   Bilinear *bfringe = nullptr, *ffringe = nullptr, *samples = nullptr;
   int n_bfringe = 0, n_ffringe = 0, n_samples = 0;
@@ -86,7 +85,6 @@ struct MiState {
 namespace {
 
 constexpr int kVectors = 18;   // solver vectors per hemisphere
-constexpr int kMaxBlocks = 512;
 
 struct MiK {
   Layout L;
@@ -96,7 +94,7 @@ struct MiK {
   int cur;
   int np, nt, n, S, si;
   double ri, max_colat, current_norm, pedersen, r2;
-  double *shell_fac, *fac, *potential, *work, *scal, *partial;
+  double *shell_fac, *fac, *potential, *work, *scal;
   __device__ double *p(int q) const { return pl + (long)q * L.total; }
   __device__ const uint8_t *b(int q) const { return by + (long)q * L.total; }
   __device__ double *vec(int h, int v) const { return work + ((long)h * kVectors + v) * n; }
@@ -410,67 +408,16 @@ __global__ void k_mi_apply(MiK c, int tensor, int xin, int yout, const int *acti
   }
 }
 
-// Two-pass dot products: per-block partial sums in a fixed order, then one
-// block sums the partials in order. mode 0: a.b; mode 1: the residual norm of
-// the unweighted equations (:394-410), sqrt taken in the finish.
-__global__ void k_mi_dot(MiK c, int a, int b, int mode, const int *active) {
-  const int h = blockIdx.y;
-  if (!active[h]) return;
-  __shared__ double red[kT];
-  const int np = c.np;
-  const double dt = c.max_colat / (double)(c.nt - 1);
-  double s = 0.0;
-  for (long q = blockIdx.x * (long)kT + threadIdx.x; q < c.n; q += (long)gridDim.x * kT) {
-    if (mode == 0) {
-      s += c.vec(h, a)[q] * c.vec(h, b)[q];
-    } else if (q == 0) {
-      const double half = 0.5 * dt;
-      const double ps = (double)np * (sin(half) * c.pedersen / (incl2(half) * dt * dt));
-      const double pr = c.vec(h, a)[0] / ps;
-      s += (double)np * pr * pr;
-    } else {
-      const int i = 1 + (int)((q - 1) / np);
-      const double v = c.vec(h, a)[q] / sin((double)i * dt);
-      s += v * v;
-    }
-  }
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = kT / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) c.partial[h * kMaxBlocks + blockIdx.x] = red[0];
-}
-
-__global__ void k_mi_finish(MiK c, int nblocks, int slot, int mode, const int *active) {
-  const int h = blockIdx.x;
-  if (!active[h] || threadIdx.x != 0) return;
-  double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += c.partial[h * kMaxBlocks + b];
-  c.sl(h)[slot] = mode == 1 ? sqrt(fmax(0.0, s)) : s;
-}
-
-// Elementwise steps of the Krylov loops; scalars come from device slots.
+// Elementwise steps outside the fused solves.
 enum Op {
-  OP_RESIDUAL,      // r = rhs - ap; shadow = r
-  OP_JACOBI,        // z = r / diag
-  OP_COPY_ZP,       // p = z
-  OP_CG_XR,         // alpha = rz / pap; x += alpha p; r -= alpha ap
-  OP_CG_P,          // p = z + (rz_new / rz) p   (rz_new in SL_NORM slot holder)
-  OP_BI_P,          // p = r + beta (p - omega ap)
-  OP_BI_S,          // alpha = rho / den; s = r - alpha ap
-  OP_BI_XR,         // omega = wnum / wden; x += alpha pp + omega ps; r = s - omega as
-  OP_BI_X_ONLY,     // x += alpha pp
-  OP_ZERO_PAP,      // p = 0, ap = 0
-  OP_EXPAND         // potential from x
+  OP_RESIDUAL,  // r = rhs - ap; shadow = r
+  OP_EXPAND     // potential from x (outer ring = extra)
 };
 
 __global__ void k_mi_vec(MiK c, int op, double extra, const int *active) {
   const int h = blockIdx.y;
   if (!active[h]) return;
   const long q = gtid();
-  const double *sl = c.sl(h);
   if (op == OP_EXPAND) {
     const long np = c.np, nt = c.nt;
     if (q >= np * nt) return;
@@ -481,105 +428,9 @@ __global__ void k_mi_vec(MiK c, int op, double extra, const int *active) {
     return;
   }
   if (q >= c.n) return;
-  double *x = c.vec(h, V_X), *r = c.vec(h, V_R), *z = c.vec(h, V_Z), *p = c.vec(h, V_P),
-         *ap = c.vec(h, V_AP);
-  switch (op) {
-    case OP_RESIDUAL:
-      r[q] = c.vec(h, V_RHS)[q] - ap[q];
-      c.vec(h, V_SH)[q] = r[q];
-      break;
-    case OP_JACOBI: z[q] = r[q] / c.vec(h, V_DIAG)[q]; break;
-    case OP_COPY_ZP: p[q] = z[q]; break;
-    case OP_CG_XR: {
-      const double alpha = sl[SL_RZ] / sl[SL_PAP];
-      x[q] += alpha * p[q];
-      r[q] -= alpha * ap[q];
-      break;
-    }
-    case OP_CG_P: p[q] = z[q] + (sl[SL_RHO] / sl[SL_RZ]) * p[q]; break;
-    case OP_BI_P: {
-      const double beta = (sl[SL_RHO] / sl[SL_RHOPREV]) * (sl[SL_ALPHA] / sl[SL_OMEGA]);
-      p[q] = r[q] + beta * (p[q] - sl[SL_OMEGA] * ap[q]);
-      break;
-    }
-    case OP_BI_S: c.vec(h, V_S)[q] = r[q] - sl[SL_ALPHA] * ap[q]; break;
-    case OP_BI_XR: {
-      const double omega = sl[SL_OMEGA];
-      x[q] += sl[SL_ALPHA] * c.vec(h, V_PP)[q] + omega * c.vec(h, V_PS)[q];
-      r[q] = c.vec(h, V_S)[q] - omega * c.vec(h, V_AS)[q];
-      break;
-    }
-    case OP_BI_X_ONLY: x[q] += sl[SL_ALPHA] * c.vec(h, V_PP)[q]; break;
-    case OP_ZERO_PAP:
-      p[q] = 0.0;
-      ap[q] = 0.0;
-      break;
-    default: break;
-  }
-}
-
-// Scalar updates of BiCGStab that must follow a reduction.
-__global__ void k_mi_scalar(MiK c, int what, const int *active) {
-  const int h = threadIdx.x;
-  if (h >= 2 || !active[h]) return;
-  double *sl = c.sl(h);
-  if (what == 0) sl[SL_ALPHA] = sl[SL_RHO] / sl[SL_DEN];
-  else if (what == 1) sl[SL_OMEGA] = sl[SL_WNUM] / sl[SL_WDEN];
-  else if (what == 2) sl[SL_RHOPREV] = sl[SL_RHO];
-  else if (what == 3) {  // restart
-    sl[SL_RHOPREV] = 1.0;
-    sl[SL_ALPHA] = 1.0;
-    sl[SL_OMEGA] = 1.0;
-  } else if (what == 4) {
-    sl[SL_RZ] = sl[SL_RHO];  // CG: rz <- rz_new
-  }
-}
-
-// Longitude-line preconditioner of the tensor system: out = M^-1 in with M
-// the pole diagonal plus, on each colatitude ring, the ring's own periodic
-// tridiagonal coupling (cyclic Thomas solve with a Sherman-Morrison
-// correction). One thread per ring.
-__global__ void k_mi_ring_solve(MiK c, int vin, int vout, const int *active) {
-  const int h = blockIdx.y;
-  if (!active[h]) return;
-  const int ring = (int)gtid();
-  const int np = c.np, rings = c.nt - 2;
-  const double *in = c.vec(h, vin);
-  double *out = c.vec(h, vout);
-  const double *diag = c.vec(h, V_DIAG);
-  if (ring == 0) out[0] = in[0] / diag[0];
-  if (ring >= rings) return;
-  const int base = 1 + ring * np;
-  const double *a = c.vec(h, V_LLO) + base;  // coefficient of x[j-1]
-  const double *b = diag + base;
-  const double *cc = c.vec(h, V_LHI) + base; // coefficient of x[j+1]
-  double *g = c.vec(h, V_SCR) + base;   // forward-sweep factors
-  double *z = c.vec(h, V_SCR2) + base;  // correction vector
-  double *x = out + base;
-  const double *rhs = in + base;
-  // Sherman-Morrison split of the periodic corners: gamma = -b[0]
-  const double gam = -b[0];
-  const double alpha = cc[np - 1];  // row np-1's coupling to x[0]
-  const double beta = a[0];         // row 0's coupling to x[np-1]
-  // T y = rhs into x[], T z = (gamma, 0, ..., alpha) into z[]; T has the
-  // corner-corrected first and last diagonal entries
-  double bet = b[0] - gam;
-  x[0] = rhs[0] / bet;
-  z[0] = gam / bet;
-  g[0] = 0.0;
-  for (int j = 1; j < np; ++j) {
-    g[j] = cc[j - 1] / bet;
-    const double bj = j == np - 1 ? b[j] - alpha * beta / gam : b[j];
-    bet = bj - a[j] * g[j];
-    x[j] = (rhs[j] - a[j] * x[j - 1]) / bet;
-    z[j] = ((j == np - 1 ? alpha : 0.0) - a[j] * z[j - 1]) / bet;
-  }
-  for (int j = np - 2; j >= 0; --j) {
-    x[j] -= g[j + 1] * x[j + 1];
-    z[j] -= g[j + 1] * z[j + 1];
-  }
-  const double fact = (x[0] + beta * x[np - 1] / gam) / (1.0 + z[0] + beta * z[np - 1] / gam);
-  for (int j = 0; j < np; ++j) x[j] -= fact * z[j];
+  double *r = c.vec(h, V_R);
+  r[q] = c.vec(h, V_RHS)[q] - c.vec(h, V_AP)[q];
+  c.vec(h, V_SH)[q] = r[q];
 }
 
 template <typename T>
@@ -617,8 +468,346 @@ MiK make_mik(Problem &P) {
   c.potential = m.potential;
   c.work = m.work;
   c.scal = m.scal;
-  c.partial = m.partial;
   return c;
+}
+
+// ---- fused solves: one workgroup per hemisphere runs the whole Krylov loop -------
+//
+// The same iterations as the reference's solvers (CG: :516-600, BiCGStab:
+// :1029-1131, with the ring preconditioner above), but every scalar stays on
+// the device: the workgroup's threads stride over the unknowns, dot products
+// are workgroup reductions in a fixed order, and the loop, its breakdown and
+// convergence tests and the periodic true-residual checks run in the kernel.
+// The host launches once and reads the iteration count and final residual.
+// Every branch depends only on values all threads share, so the whole
+// workgroup leaves the loop together, at the latest after maximum_iterations.
+
+constexpr int kSolveThreads = 1024;
+
+// Sum over the workgroup, returned to every thread (fixed order).
+GCS_DEV double block_sum(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();  // red[] may still be read from the previous call
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < kSolveThreads / 64; ++w) s += red[w];
+  return s;
+}
+
+// One term of the residual norm of the unweighted equations (:394-410).
+GCS_DEV double orig_term(const MiK &c, double a, long q, double ps, double dt) {
+  if (q == 0) {
+    const double pr = a / ps;
+    return (double)c.np * pr * pr;
+  }
+  const int i = 1 + (int)((q - 1) / c.np);
+  const double v = a / sin((double)i * dt);
+  return v * v;
+}
+
+// Rows of the Pedersen operator (w, lo, hi, lon per colatitude index),
+// computed with k_mi_apply's expressions.
+GCS_DEV void pedersen_rows(const MiK &c, double *rows) {
+  const double dt = c.max_colat / (double)(c.nt - 1);
+  const double dp = 2.0 * kPi / (double)c.np;
+  for (int i = 1 + threadIdx.x; i + 1 < c.nt; i += blockDim.x) {
+    const double th = (double)i * dt;
+    rows[4 * i + 0] = sin(th);
+    rows[4 * i + 1] = theta_coeff(c.pedersen, th - 0.5 * dt, th, dt);
+    rows[4 * i + 2] = theta_coeff(c.pedersen, th + 0.5 * dt, th, dt);
+    rows[4 * i + 3] = c.pedersen / (sin(th) * sin(th) * dp * dp);
+  }
+}
+
+// (A x)[q] with k_mi_apply's arithmetic (Pedersen rows from `rows`).
+GCS_DEV double apply_at(const MiK &c, int h, bool tensor, const double *x, long q,
+                        const double *rows, double pe) {
+  const int np = c.np, nt = c.nt;
+  if (q == 0) {
+    if (!tensor) {
+      double ring = 0.0;
+      for (int j = 0; j < np; ++j) ring += x[red_at(np, 1, j)];
+      return pe * ((double)np * x[0] - ring);
+    }
+    double v = c.vec(h, V_DIAG)[0] * x[0];
+    for (int j = 0; j < np; ++j) v -= pe * x[red_at(np, 1, j)];
+    return v;
+  }
+  const int i = 1 + (int)((q - 1) / np), j = (int)((q - 1) % np);
+  const int jm = j == 0 ? np - 1 : j - 1, jp = j + 1 == np ? 0 : j + 1;
+  const double below = i == 1 ? x[0] : x[red_at(np, i - 1, j)];
+  const double above = i + 2 < nt ? x[red_at(np, i + 1, j)] : 0.0;
+  if (!tensor) {
+    const double w = rows[4 * i], lo = rows[4 * i + 1], hi = rows[4 * i + 2],
+                 lon = rows[4 * i + 3];
+    return w * (lo * (x[q] - below) + hi * (x[q] - above) +
+                lon * (2.0 * x[q] - x[red_at(np, i, jm)] - x[red_at(np, i, jp)]));
+  }
+  return c.vec(h, V_DIAG)[q] * x[q] + c.vec(h, V_LO)[q] * below +
+         c.vec(h, V_HI)[q] * above + c.vec(h, V_LLO)[q] * x[red_at(np, i, jm)] +
+         c.vec(h, V_LHI)[q] * x[red_at(np, i, jp)];
+}
+
+// Longitude-line preconditioner of the tensor system: out = M^-1 in with M
+// the pole diagonal plus, on each colatitude ring, the ring's own periodic
+// tridiagonal coupling (cyclic Thomas solve with a Sherman-Morrison
+// correction), one thread per ring; the recurrence is carried in registers.
+GCS_DEV void ring_solve(const MiK &c, int h, const double *__restrict__ in,
+                        double *__restrict__ out, int ring) {
+  const int np = c.np;
+  const int base = 1 + ring * np;
+  const double *__restrict__ a = c.vec(h, V_LLO) + base;
+  const double *__restrict__ b = c.vec(h, V_DIAG) + base;
+  const double *__restrict__ cc = c.vec(h, V_LHI) + base;
+  double *__restrict__ g = c.vec(h, V_SCR) + base;
+  double *__restrict__ z = c.vec(h, V_SCR2) + base;
+  double *__restrict__ x = out + base;
+  const double *__restrict__ rhs = in + base;
+  const double gam = -b[0];
+  const double alpha = cc[np - 1];
+  const double beta = a[0];
+  double ib = 1.0 / (b[0] - gam);
+  double xp = rhs[0] * ib, zp = gam * ib;
+  x[0] = xp;
+  z[0] = zp;
+  g[0] = 0.0;
+  for (int j = 1; j < np; ++j) {
+    const double gj = cc[j - 1] * ib;
+    g[j] = gj;
+    const double bj = j == np - 1 ? b[j] - alpha * beta / gam : b[j];
+    ib = 1.0 / (bj - a[j] * gj);
+    xp = (rhs[j] - a[j] * xp) * ib;
+    zp = ((j == np - 1 ? alpha : 0.0) - a[j] * zp) * ib;
+    x[j] = xp;
+    z[j] = zp;
+  }
+  const double xl = xp, zl = zp;
+  for (int j = np - 2; j >= 0; --j) {
+    xp = x[j] - g[j + 1] * xp;
+    zp = z[j] - g[j + 1] * zp;
+    x[j] = xp;
+    z[j] = zp;
+  }
+  const double fact = (xp + beta * xl / gam) / (1.0 + zp + beta * zl / gam);
+  for (int j = 0; j < np; ++j) x[j] -= fact * z[j];
+}
+
+GCS_DEV void precondition_all(const MiK &c, int h, const double *in, double *out) {
+  __syncthreads();  // `in` complete
+  if (threadIdx.x == 0) out[0] = in[0] / c.vec(h, V_DIAG)[0];
+  for (int ring = threadIdx.x; ring < c.nt - 2; ring += blockDim.x) ring_solve(c, h, in, out, ring);
+  __syncthreads();  // `out` complete
+}
+
+struct SolveResult {
+  int iters;
+  bool ok;
+  double last;
+};
+
+GCS_DEV void store_result(const MiK &c, int h, const SolveResult &r, double first) {
+  if (threadIdx.x != 0) return;
+  double *sl = c.sl(h);
+  sl[SL_ITER] = (double)r.iters;
+  sl[SL_OK] = r.ok ? 1.0 : 0.0;
+  sl[SL_NORM] = r.last;
+  sl[SL_FIRST] = first;
+}
+
+// Jacobi-preconditioned CG on the reduced Pedersen system (:516-600).
+__global__ void __launch_bounds__(kSolveThreads) k_mi_cg(MiK c, double rel_tol, int maxit) {
+  __shared__ double red[kSolveThreads / 64];
+  extern __shared__ double rows[];
+  const int h = blockIdx.x;
+  const long n = c.n;
+  const long t0 = threadIdx.x, st = blockDim.x;
+  const double dt = c.max_colat / (double)(c.nt - 1);
+  const double half = 0.5 * dt;
+  const double pe = sin(half) * c.pedersen / (incl2(half) * dt * dt);
+  const double ps = (double)c.np * pe;
+  pedersen_rows(c, rows);
+  __syncthreads();
+  double *x = c.vec(h, V_X), *r = c.vec(h, V_R), *z = c.vec(h, V_Z), *p = c.vec(h, V_P),
+         *ap = c.vec(h, V_AP);
+  const double *rhs = c.vec(h, V_RHS), *diag = c.vec(h, V_DIAG);
+  double a = 0.0, b = 0.0;
+  for (long q = t0; q < n; q += st) {
+    r[q] = rhs[q] - apply_at(c, h, false, x, q, rows, pe);
+    a += orig_term(c, r[q], q, ps, dt);
+    b += q == 0 ? 0.0 : orig_term(c, rhs[q], q, ps, dt);
+  }
+  double norm = sqrt(fmax(0.0, block_sum(a, red)));
+  const double first = norm;
+  const double target = rel_tol * fmax(sqrt(fmax(0.0, block_sum(b, red))), 1.0);
+  SolveResult res{0, norm <= target, norm};
+  if (!res.ok) {
+    a = 0.0;
+    for (long q = t0; q < n; q += st) {
+      z[q] = r[q] / diag[q];
+      p[q] = z[q];
+      a += r[q] * z[q];
+    }
+    double rz = block_sum(a, red);
+    for (int it = 1; it <= maxit; ++it) {
+      a = 0.0;
+      for (long q = t0; q < n; q += st) {
+        ap[q] = apply_at(c, h, false, p, q, rows, pe);
+        a += p[q] * ap[q];
+      }
+      const double pap = block_sum(a, red);
+      if (!isfinite(rz) || rz <= DBL_MIN || !isfinite(pap) || pap <= DBL_MIN) break;
+      const double alpha = rz / pap;
+      a = 0.0;
+      for (long q = t0; q < n; q += st) {
+        x[q] += alpha * p[q];
+        r[q] -= alpha * ap[q];
+        a += orig_term(c, r[q], q, ps, dt);
+      }
+      norm = sqrt(fmax(0.0, block_sum(a, red)));
+      res.iters = it;
+      res.last = norm;
+      if (norm <= target || it % 1024 == 0) {
+        __syncthreads();  // x complete
+        a = 0.0;
+        for (long q = t0; q < n; q += st) {
+          r[q] = rhs[q] - apply_at(c, h, false, x, q, rows, pe);
+          a += orig_term(c, r[q], q, ps, dt);
+        }
+        norm = sqrt(fmax(0.0, block_sum(a, red)));
+        res.last = norm;
+        if (norm <= target) {
+          res.ok = true;
+          break;
+        }
+        a = 0.0;
+        for (long q = t0; q < n; q += st) {
+          z[q] = r[q] / diag[q];
+          p[q] = z[q];
+          a += r[q] * z[q];
+        }
+        rz = block_sum(a, red);
+        continue;
+      }
+      a = 0.0;
+      for (long q = t0; q < n; q += st) {
+        z[q] = r[q] / diag[q];
+        a += r[q] * z[q];
+      }
+      const double rzn = block_sum(a, red);
+      for (long q = t0; q < n; q += st) p[q] = z[q] + (rzn / rz) * p[q];
+      rz = rzn;
+      __syncthreads();  // p complete before the next product
+    }
+  }
+  store_result(c, h, res, first);
+}
+
+// Right-preconditioned BiCGStab on the tensor system (:1029-1131).
+__global__ void __launch_bounds__(kSolveThreads) k_mi_bicgstab(MiK c, double rel_tol,
+                                                               int maxit) {
+  __shared__ double red[kSolveThreads / 64];
+  const int h = blockIdx.x;
+  const long n = c.n;
+  const long t0 = threadIdx.x, st = blockDim.x;
+  const double dt = c.max_colat / (double)(c.nt - 1);
+  const double half = 0.5 * dt;
+  const double pe = sin(half) * c.pedersen / (incl2(half) * dt * dt);
+  const double ps = (double)c.np * pe;
+  double *x = c.vec(h, V_X), *r = c.vec(h, V_R), *p = c.vec(h, V_P), *ap = c.vec(h, V_AP),
+         *sh = c.vec(h, V_SH), *pp = c.vec(h, V_PP), *s = c.vec(h, V_S), *psv = c.vec(h, V_PS),
+         *as = c.vec(h, V_AS);
+  const double *rhs = c.vec(h, V_RHS);
+  double a = 0.0, b = 0.0;
+  for (long q = t0; q < n; q += st) {
+    r[q] = rhs[q] - apply_at(c, h, true, x, q, nullptr, pe);
+    sh[q] = r[q];
+    p[q] = 0.0;
+    ap[q] = 0.0;
+    a += orig_term(c, r[q], q, ps, dt);
+    b += q == 0 ? 0.0 : orig_term(c, rhs[q], q, ps, dt);
+  }
+  double norm = sqrt(fmax(0.0, block_sum(a, red)));
+  const double first = norm;
+  const double target = rel_tol * fmax(sqrt(fmax(0.0, block_sum(b, red))), 1.0);
+  SolveResult res{0, norm <= target, norm};
+  double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+  for (int it = 1; it <= maxit && !res.ok; ++it) {
+    a = 0.0;
+    for (long q = t0; q < n; q += st) a += sh[q] * r[q];
+    const double rho = block_sum(a, red);
+    if (!isfinite(rho) || fabs(rho) <= DBL_MIN || !isfinite(omega) || fabs(omega) <= DBL_MIN)
+      break;
+    const double beta = (rho / rho_prev) * (alpha / omega);
+    for (long q = t0; q < n; q += st) p[q] = r[q] + beta * (p[q] - omega * ap[q]);
+    precondition_all(c, h, p, pp);
+    a = 0.0;
+    for (long q = t0; q < n; q += st) {
+      ap[q] = apply_at(c, h, true, pp, q, nullptr, pe);
+      a += sh[q] * ap[q];
+    }
+    const double den = block_sum(a, red);
+    alpha = rho / den;
+    a = 0.0;
+    for (long q = t0; q < n; q += st) {
+      s[q] = r[q] - alpha * ap[q];
+      a += orig_term(c, s[q], q, ps, dt);
+    }
+    const double snorm = sqrt(fmax(0.0, block_sum(a, red)));
+    if (!isfinite(den) || fabs(den) <= DBL_MIN) break;
+    if (snorm <= target) {
+      res.last = snorm;
+      res.iters = it;
+      res.ok = true;
+      for (long q = t0; q < n; q += st) x[q] += alpha * pp[q];
+      break;
+    }
+    precondition_all(c, h, s, psv);
+    a = 0.0;
+    b = 0.0;
+    for (long q = t0; q < n; q += st) {
+      as[q] = apply_at(c, h, true, psv, q, nullptr, pe);
+      a += as[q] * as[q];
+      b += as[q] * s[q];
+    }
+    const double wden = block_sum(a, red);
+    const double wnum = block_sum(b, red);
+    omega = wnum / wden;
+    if (!isfinite(wden) || wden <= DBL_MIN || !isfinite(omega) || fabs(omega) <= DBL_MIN)
+      break;
+    a = 0.0;
+    for (long q = t0; q < n; q += st) {
+      x[q] += alpha * pp[q] + omega * psv[q];
+      r[q] = s[q] - omega * as[q];
+      a += orig_term(c, r[q], q, ps, dt);
+    }
+    norm = sqrt(fmax(0.0, block_sum(a, red)));
+    res.iters = it;
+    res.last = norm;
+    if (norm <= target || it % 256 == 0) {
+      __syncthreads();  // x complete
+      a = 0.0;
+      for (long q = t0; q < n; q += st) {
+        r[q] = rhs[q] - apply_at(c, h, true, x, q, nullptr, pe);
+        sh[q] = r[q];
+        a += orig_term(c, r[q], q, ps, dt);
+      }
+      norm = sqrt(fmax(0.0, block_sum(a, red)));
+      res.last = norm;
+      if (norm <= target) {
+        res.ok = true;
+        break;
+      }
+      for (long q = t0; q < n; q += st) {
+        p[q] = 0.0;
+        ap[q] = 0.0;
+      }
+      rho_prev = alpha = omega = 1.0;
+      continue;
+    }
+    rho_prev = rho;
+  }
+  store_result(c, h, res, first);
 }
 
 // ---- learned warm start ----------------------------------------------------------
@@ -724,234 +913,40 @@ int warm_start(Problem &P, const MiK &c, bool tensor) {
   return 0;
 }
 
-// The polar solves of both hemispheres; 0 when both converge.
+// The polar solves of both hemispheres; 0 when both converge. Setup (right-
+// hand side, guess, stencil), the optional learned correction of the guess,
+// then one fused Krylov kernel per solve (both hemispheres in one launch).
 int solve(Problem &P, bool tensor) {
   MiState &m = *P.mi;
   MiK c = make_mik(P);
   hipStream_t s = P.stream;
-  int *active = m.active;
-  int act[2] = {1, 1};
-  auto set_active = [&]() -> int {
-    GCS_HIP(hipMemcpyAsync(active, act, sizeof(act), hipMemcpyHostToDevice, s));
-    return 0;
-  };
-  const int nb = (int)std::min<long>(kMaxBlocks, blocks_of(m.n));
-  const dim3 g2(blocks_of(m.n), 2);
-  auto dot = [&](int a, int b, int mode, int slot) {
-    k_mi_dot<<<dim3(nb, 2), kT, 0, s>>>(c, a, b, mode, active);
-    k_mi_finish<<<2, 64, 0, s>>>(c, nb, slot, mode, active);
-  };
-  auto vec = [&](int op, double extra = 0.0) {
-    const long count = op == OP_EXPAND ? (long)m.np * m.nt : m.n;
-    k_mi_vec<<<dim3(blocks_of(count), 2), kT, 0, s>>>(c, op, extra, active);
-  };
-  auto apply = [&](int xin, int yout) {
-    k_mi_apply<<<g2, kT, 0, s>>>(c, tensor ? 1 : 0, xin, yout, active);
-  };
-  auto precondition = [&](int vin, int vout) {
-    if (tensor)
-      k_mi_ring_solve<<<dim3(blocks_of(m.nt - 2, 64), 2), 64, 0, s>>>(c, vin, vout, active);
-    else
-      vec(OP_JACOBI);  // z = r / diag (the CG's only use)
-  };
-  auto read = [&]() -> int {
-    GCS_HIP(hipMemcpyAsync(m.host_scal, m.scal, 2 * SL_COUNT * sizeof(double),
-                           hipMemcpyDeviceToHost, s));
-    GCS_HIP(hipStreamSynchronize(s));
-    return 0;
-  };
   const auto t0 = std::chrono::steady_clock::now();
-  if (set_active() != 0) return -1;
+  const int act[2] = {1, 1};
+  GCS_HIP(hipMemcpyAsync(m.active, act, sizeof(act), hipMemcpyHostToDevice, s));
+  const dim3 g2(blocks_of(m.n), 2);
   k_mi_setup<<<g2, kT, 0, s>>>(c, tensor ? 1 : 0, m.hall);
   if (m.net != nullptr && warm_start(P, c, tensor) != 0) return -1;
-  apply(V_X, V_AP);
-  vec(OP_RESIDUAL);
-  dot(V_R, V_R, 1, SL_FIRST);
-  // right-hand side norm: sum of squares of the unweighted rows
-  dot(V_RHS, V_RHS, 1, SL_RHS2);
-  if (read() != 0) return -1;
-  double target[2], last[2];
-  int iters[2] = {0, 0};
-  bool ok[2] = {false, false};
-  for (int h = 0; h < 2; ++h) {
-    const double *sl = m.host_scal + h * SL_COUNT;
-    // SL_RHS2 holds sqrt(sum (rhs/w)^2) except for the pole row (zero rhs)
-    const double rhs_norm = sl[SL_RHS2];
-    target[h] = m.cfg.relative_tolerance * std::fmax(rhs_norm, 1.0);
-    last[h] = sl[SL_FIRST];
-    if (last[h] <= target[h]) {
-      ok[h] = true;
-      act[h] = 0;
-    }
-  }
+  const double rel = m.cfg.relative_tolerance;
   const int maxit = m.cfg.maximum_iterations;
-  if (!tensor) {
-    // :516-600
-    if (set_active() != 0) return -1;
-    precondition(V_R, V_Z);
-    vec(OP_COPY_ZP);
-    dot(V_R, V_Z, 0, SL_RZ);
-    for (int it = 1; it <= maxit && (act[0] || act[1]); ++it) {
-      apply(V_P, V_AP);
-      dot(V_P, V_AP, 0, SL_PAP);
-      if (read() != 0) return -1;
-      for (int h = 0; h < 2; ++h) {
-        if (!act[h]) continue;
-        const double *sl = m.host_scal + h * SL_COUNT;
-        if (!std::isfinite(sl[SL_RZ]) || sl[SL_RZ] <= DBL_MIN ||
-            !std::isfinite(sl[SL_PAP]) || sl[SL_PAP] <= DBL_MIN)
-          act[h] = 0;
-      }
-      if (set_active() != 0) return -1;
-      vec(OP_CG_XR);
-      dot(V_R, V_R, 1, SL_NORM);
-      if (read() != 0) return -1;
-      bool verify = false;
-      for (int h = 0; h < 2; ++h)
-        if (act[h]) {
-          iters[h] = it;
-          last[h] = m.host_scal[h * SL_COUNT + SL_NORM];
-          if (last[h] <= target[h] || it % 1024 == 0) verify = true;
-        }
-      if (verify) {
-        // true residual for every active hemisphere (restart when drifted)
-        apply(V_X, V_AP);
-        vec(OP_RESIDUAL);
-        dot(V_R, V_R, 1, SL_NORM);
-        if (read() != 0) return -1;
-        for (int h = 0; h < 2; ++h)
-          if (act[h]) {
-            last[h] = m.host_scal[h * SL_COUNT + SL_NORM];
-            if (last[h] <= target[h]) {
-              ok[h] = true;
-              act[h] = 0;
-            }
-          }
-        if (set_active() != 0) return -1;
-        precondition(V_R, V_Z);
-        vec(OP_COPY_ZP);
-        dot(V_R, V_Z, 0, SL_RZ);
-        continue;
-      }
-      precondition(V_R, V_Z);
-      dot(V_R, V_Z, 0, SL_RHO);  // rz_new
-      vec(OP_CG_P);
-      k_mi_scalar<<<1, 64, 0, s>>>(c, 4, active);
-    }
-  } else {
-    // :1029-1131 with the ring preconditioner
-    for (int h = 0; h < 2; ++h) {
-      double *sl = m.host_scal + h * SL_COUNT;
-      sl[SL_RHOPREV] = 1.0;
-      sl[SL_ALPHA] = 1.0;
-      sl[SL_OMEGA] = 1.0;
-    }
-    k_mi_scalar<<<1, 64, 0, s>>>(c, 3, active);
-    vec(OP_ZERO_PAP);
-    for (int it = 1; it <= maxit && (act[0] || act[1]); ++it) {
-      dot(V_SH, V_R, 0, SL_RHO);
-      if (read() != 0) return -1;
-      for (int h = 0; h < 2; ++h) {
-        if (!act[h]) continue;
-        const double *sl = m.host_scal + h * SL_COUNT;
-        if (!std::isfinite(sl[SL_RHO]) || std::fabs(sl[SL_RHO]) <= DBL_MIN ||
-            !std::isfinite(sl[SL_OMEGA]) || std::fabs(sl[SL_OMEGA]) <= DBL_MIN)
-          act[h] = 0;
-      }
-      if (set_active() != 0) return -1;
-      vec(OP_BI_P);
-      precondition(V_P, V_PP);
-      apply(V_PP, V_AP);
-      dot(V_SH, V_AP, 0, SL_DEN);
-      k_mi_scalar<<<1, 64, 0, s>>>(c, 0, active);
-      vec(OP_BI_S);
-      dot(V_S, V_S, 1, SL_NORM);
-      if (read() != 0) return -1;
-      for (int h = 0; h < 2; ++h) {
-        if (!act[h]) continue;
-        const double *sl = m.host_scal + h * SL_COUNT;
-        if (!std::isfinite(sl[SL_DEN]) || std::fabs(sl[SL_DEN]) <= DBL_MIN) {
-          act[h] = 0;
-          continue;
-        }
-        if (sl[SL_NORM] <= target[h]) {
-          last[h] = sl[SL_NORM];
-          iters[h] = it;
-          ok[h] = true;
-        }
-      }
-      // converged hemispheres take x += alpha pp and stop
-      {
-        int conv[2] = {act[0] && ok[0], act[1] && ok[1]};
-        if (conv[0] || conv[1]) {
-          int keep[2] = {act[0], act[1]};
-          act[0] = conv[0];
-          act[1] = conv[1];
-          if (set_active() != 0) return -1;
-          vec(OP_BI_X_ONLY);
-          act[0] = keep[0] && !conv[0];
-          act[1] = keep[1] && !conv[1];
-        }
-        if (set_active() != 0) return -1;
-        if (!act[0] && !act[1]) break;
-      }
-      precondition(V_S, V_PS);
-      apply(V_PS, V_AS);
-      dot(V_AS, V_AS, 0, SL_WDEN);
-      dot(V_AS, V_S, 0, SL_WNUM);
-      k_mi_scalar<<<1, 64, 0, s>>>(c, 1, active);
-      if (read() != 0) return -1;
-      for (int h = 0; h < 2; ++h) {
-        if (!act[h]) continue;
-        const double *sl = m.host_scal + h * SL_COUNT;
-        if (!std::isfinite(sl[SL_WDEN]) || sl[SL_WDEN] <= DBL_MIN ||
-            !std::isfinite(sl[SL_OMEGA]) || std::fabs(sl[SL_OMEGA]) <= DBL_MIN)
-          act[h] = 0;
-      }
-      if (set_active() != 0) return -1;
-      vec(OP_BI_XR);
-      dot(V_R, V_R, 1, SL_NORM);
-      if (read() != 0) return -1;
-      bool verify = false;
-      for (int h = 0; h < 2; ++h)
-        if (act[h]) {
-          iters[h] = it;
-          last[h] = m.host_scal[h * SL_COUNT + SL_NORM];
-          if (last[h] <= target[h] || it % 256 == 0) verify = true;
-        }
-      if (verify) {
-        // true residual r = rhs - A x (the shadow residual restarts from it)
-        apply(V_X, V_AP);
-        vec(OP_RESIDUAL);  // also resets the shadow residual to r
-        dot(V_R, V_R, 1, SL_NORM);
-        if (read() != 0) return -1;
-        for (int h = 0; h < 2; ++h)
-          if (act[h]) {
-            last[h] = m.host_scal[h * SL_COUNT + SL_NORM];
-            if (last[h] <= target[h]) {
-              ok[h] = true;
-              act[h] = 0;
-            }
-          }
-        if (set_active() != 0) return -1;
-        vec(OP_ZERO_PAP);
-        k_mi_scalar<<<1, 64, 0, s>>>(c, 3, active);
-        continue;
-      }
-      k_mi_scalar<<<1, 64, 0, s>>>(c, 2, active);
-    }
-  }
-  act[0] = act[1] = 1;
-  if (set_active() != 0) return -1;
-  vec(OP_EXPAND, 0.0);
+  if (tensor)
+    k_mi_bicgstab<<<2, kSolveThreads, 0, s>>>(c, rel, maxit);
+  else
+    k_mi_cg<<<2, kSolveThreads, 4 * (size_t)m.nt * sizeof(double), s>>>(c, rel, maxit);
+  k_mi_vec<<<dim3(blocks_of((long)m.np * m.nt), 2), kT, 0, s>>>(c, OP_EXPAND, 0.0, m.active);
+  GCS_HIP(hipGetLastError());
+  GCS_HIP(hipMemcpyAsync(m.host_scal, m.scal, 2 * SL_COUNT * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
   GCS_HIP(hipStreamSynchronize(s));
   m.solve_ms = std::chrono::duration<double, std::milli>(
                    std::chrono::steady_clock::now() - t0).count();
+  bool ok = true;
   for (int h = 0; h < 2; ++h) {
-    m.iterations[h] = iters[h];
-    m.final_residual[h] = last[h];
+    const double *sl = m.host_scal + h * SL_COUNT;
+    m.iterations[h] = (int)sl[SL_ITER];
+    m.final_residual[h] = sl[SL_NORM];
+    ok = ok && sl[SL_OK] != 0.0;
   }
-  if (!ok[0] || !ok[1]) {
+  if (!ok) {
     set_error("ionosphere potential solve did not converge");
     return -1;
   }
@@ -1108,7 +1103,6 @@ int mi_create(Problem &P) {
   GCS_HIP(hipMalloc(&m.potential, polar * sizeof(double)));
   GCS_HIP(hipMalloc(&m.work, 2L * kVectors * m.n * sizeof(double)));
   GCS_HIP(hipMalloc(&m.scal, 2 * SL_COUNT * sizeof(double)));
-  GCS_HIP(hipMalloc(&m.partial, 2 * kMaxBlocks * sizeof(double)));
   GCS_HIP(hipHostMalloc(&m.host_scal, 2 * SL_COUNT * sizeof(double)));
   GCS_HIP(hipMalloc(&m.active, 2 * sizeof(int)));
   GCS_HIP(hipMemset(m.shell_fac, 0, 6L * S * S * sizeof(double)));
@@ -1141,7 +1135,6 @@ void mi_destroy(Problem &P) {
   (void)hipFree(m->potential);
   (void)hipFree(m->work);
   (void)hipFree(m->scal);
-  (void)hipFree(m->partial);
   (void)hipFree(m->bfringe);
   (void)hipFree(m->ffringe);
   (void)hipFree(m->samples);
