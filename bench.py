@@ -101,6 +101,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="config1_nomi")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="fp64: the drop-in's arithmetic; mixed: FP32 reconstruction, "
+                         "fluxes and EMF with FP64 update, CT, CFL and ionosphere")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
                     help="only run a plain copy kernel over N doubles (read N, "
                          "write N) three times: the known-size reference for "
@@ -130,6 +133,7 @@ def main():
     t0 = time.perf_counter()
     g = native.GpuProblem(cfg)
     g.set_fidelity(False)
+    g.set_precision(args.precision == "mixed")
     print(f"bench: setup {time.perf_counter() - t0:.1f} s, warmup {args.warmup}",
           file=sys.stderr, flush=True)
     # Every driver step is one hipGraph launch; the warmup steps capture the
@@ -139,7 +143,9 @@ def main():
         g.advance(args.warmup)
     mi_before = g.mi_report()["updates"] if cfg.mi_enabled else 0
     # Timed run: one hipGraph launch per step, no stage events.
+    t_before = g.clock()["time"]
     ms, _ = g.time_steps(args.steps, profile=False)
+    sim_per_step = (g.clock()["time"] - t_before) / args.steps * cfgm.TIME_NORM_S
     mi_timed = (g.mi_report()["updates"] - mi_before) if cfg.mi_enabled else 0
     print(f"bench: {args.steps} steps in {ms:.1f} ms", file=sys.stderr, flush=True)
     # Stage breakdown: the same number of steps again, kernel by kernel with
@@ -176,12 +182,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 flux / f64 update",
         "data": "synthetic",
         "config": {"workload": args.workload,
                    "cells": f"6x{n}^2x{nr}",
                    "active_cells": cfg.active_cells,
                    "mi": mi_info,
+                   "precision": args.precision,
+                   "simulated_s_per_step": sim_per_step,
                    "hbm_fraction_step_model": STEP_BYTES_PER_CELL * value / (HBM_PEAK_GBS * 1e9)},
         "roofline": {"bound": "hbm", "kernel": top,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -84,7 +84,7 @@ struct MiState {
 
 namespace {
 
-constexpr int kVectors = 18;   // solver vectors per hemisphere
+constexpr int kVectors = 20;   // solver vectors per hemisphere
 
 struct MiK {
   Layout L;
@@ -108,8 +108,10 @@ struct MiK {
 };
 
 // vectors of one hemisphere's solve
+// V_SCR, V_SCR2, V_IB, V_RDEN: the ring preconditioner's factors (forward
+// factors g, Sherman-Morrison vector z, 1 / pivots, per-ring denominator).
 enum Vec { V_RHS, V_DIAG, V_X, V_R, V_Z, V_P, V_AP, V_SH, V_PP, V_S, V_PS, V_AS,
-           V_LO, V_HI, V_LLO, V_LHI, V_SCR, V_SCR2 };
+           V_LO, V_HI, V_LLO, V_LHI, V_SCR, V_SCR2, V_IB, V_RDEN };
 
 GCS_DEV long gtid() { return blockIdx.x * (long)blockDim.x + threadIdx.x; }
 
@@ -520,20 +522,10 @@ GCS_DEV void pedersen_rows(const MiK &c, double *rows) {
   }
 }
 
-// (A x)[q] with k_mi_apply's arithmetic (Pedersen rows from `rows`).
+// (A x)[q], q >= 1, with k_mi_apply's arithmetic (Pedersen rows from `rows`).
 GCS_DEV double apply_at(const MiK &c, int h, bool tensor, const double *x, long q,
                         const double *rows, double pe) {
   const int np = c.np, nt = c.nt;
-  if (q == 0) {
-    if (!tensor) {
-      double ring = 0.0;
-      for (int j = 0; j < np; ++j) ring += x[red_at(np, 1, j)];
-      return pe * ((double)np * x[0] - ring);
-    }
-    double v = c.vec(h, V_DIAG)[0] * x[0];
-    for (int j = 0; j < np; ++j) v -= pe * x[red_at(np, 1, j)];
-    return v;
-  }
   const int i = 1 + (int)((q - 1) / np), j = (int)((q - 1) % np);
   const int jm = j == 0 ? np - 1 : j - 1, jp = j + 1 == np ? 0 : j + 1;
   const double below = i == 1 ? x[0] : x[red_at(np, i - 1, j)];
@@ -551,53 +543,157 @@ GCS_DEV double apply_at(const MiK &c, int h, bool tensor, const double *x, long 
 
 // Longitude-line preconditioner of the tensor system: out = M^-1 in with M
 // the pole diagonal plus, on each colatitude ring, the ring's own periodic
-// tridiagonal coupling (cyclic Thomas solve with a Sherman-Morrison
-// correction), one thread per ring; the recurrence is carried in registers.
-GCS_DEV void ring_solve(const MiK &c, int h, const double *__restrict__ in,
-                        double *__restrict__ out, int ring) {
+// tridiagonal coupling, solved by the Thomas algorithm with a
+// Sherman-Morrison correction for the periodic corners.
+//
+// The factors depend on the matrix only, so each solve computes them once
+// (one thread per ring): g (forward factors), 1 / pivots, the correction
+// vector z and the correction's denominator. Each application is then two
+// first-order linear recurrences per ring, which one wavefront evaluates as
+// a parallel prefix of affine maps (x -> A + B x) across its 64 lanes.
+GCS_DEV void ring_factor(const MiK &c, int h, int ring) {
   const int np = c.np;
   const int base = 1 + ring * np;
-  const double *__restrict__ a = c.vec(h, V_LLO) + base;
+  const double *__restrict__ a = c.vec(h, V_LLO) + base;  // coefficient of x[j-1]
   const double *__restrict__ b = c.vec(h, V_DIAG) + base;
-  const double *__restrict__ cc = c.vec(h, V_LHI) + base;
+  const double *__restrict__ cc = c.vec(h, V_LHI) + base; // coefficient of x[j+1]
   double *__restrict__ g = c.vec(h, V_SCR) + base;
   double *__restrict__ z = c.vec(h, V_SCR2) + base;
-  double *__restrict__ x = out + base;
-  const double *__restrict__ rhs = in + base;
+  double *__restrict__ ib = c.vec(h, V_IB) + base;
   const double gam = -b[0];
-  const double alpha = cc[np - 1];
-  const double beta = a[0];
-  double ib = 1.0 / (b[0] - gam);
-  double xp = rhs[0] * ib, zp = gam * ib;
-  x[0] = xp;
-  z[0] = zp;
+  const double alpha = cc[np - 1];  // row np-1's coupling to x[0]
+  const double beta = a[0];         // row 0's coupling to x[np-1]
+  double inv = 1.0 / (b[0] - gam);
+  double zp = gam * inv;
+  ib[0] = inv;
   g[0] = 0.0;
+  z[0] = zp;
   for (int j = 1; j < np; ++j) {
-    const double gj = cc[j - 1] * ib;
+    const double gj = cc[j - 1] * inv;
     g[j] = gj;
     const double bj = j == np - 1 ? b[j] - alpha * beta / gam : b[j];
-    ib = 1.0 / (bj - a[j] * gj);
-    xp = (rhs[j] - a[j] * xp) * ib;
-    zp = ((j == np - 1 ? alpha : 0.0) - a[j] * zp) * ib;
-    x[j] = xp;
+    inv = 1.0 / (bj - a[j] * gj);
+    ib[j] = inv;
+    zp = ((j == np - 1 ? alpha : 0.0) - a[j] * zp) * inv;
     z[j] = zp;
   }
-  const double xl = xp, zl = zp;
+  const double zl = zp;
   for (int j = np - 2; j >= 0; --j) {
-    xp = x[j] - g[j + 1] * xp;
     zp = z[j] - g[j + 1] * zp;
-    x[j] = xp;
     z[j] = zp;
   }
-  const double fact = (xp + beta * xl / gam) / (1.0 + zp + beta * zl / gam);
-  for (int j = 0; j < np; ++j) x[j] -= fact * z[j];
+  c.vec(h, V_RDEN)[base] = 1.0 + zp + beta * zl / gam;
+}
+
+GCS_DEV void factor_rings(const MiK &c, int h) {
+  for (int ring = threadIdx.x; ring < c.nt - 2; ring += blockDim.x) ring_factor(c, h, ring);
+}
+
+// One ring's solve by one wavefront (lane = threadIdx.x & 63): each lane
+// holds kRingChunk consecutive points of the ring in registers, so a ring
+// costs one load phase and one store phase.
+constexpr int kRingChunk = 8;  // rings up to 64 * 8 = 512 points
+
+GCS_DEV void ring_apply(const MiK &c, int h, const double *in, double *out, int ring,
+                        int lane) {
+  const int np = c.np;
+  const int base = 1 + ring * np;
+  const double *a = c.vec(h, V_LLO) + base;
+  const double *g = c.vec(h, V_SCR) + base;
+  const double *z = c.vec(h, V_SCR2) + base;
+  const double *ib = c.vec(h, V_IB) + base;
+  const double *rhs = in + base;
+  double *x = out + base;
+  const int k = (np + 63) / 64;
+  const int j0 = min(np, lane * k);
+  const int cnt = min(np, j0 + k) - j0;
+  double fa[kRingChunk], fb[kRingChunk], gd[kRingChunk], zz[kRingChunk], v[kRingChunk];
+#pragma unroll
+  for (int t = 0; t < kRingChunk; ++t) {
+    const int j = j0 + t;
+    const bool in_range = t < cnt;
+    const double ibj = in_range ? ib[j] : 0.0;
+    fa[t] = in_range ? ibj * rhs[j] : 0.0;
+    fb[t] = in_range && j > 0 ? -a[j] * ibj : 0.0;
+    gd[t] = in_range && j < np - 1 ? -g[j + 1] : 0.0;
+    zz[t] = in_range ? z[j] : 0.0;
+  }
+  // forward: x_j = fa_j + fb_j x_{j-1}, as a prefix of affine maps
+  double A = 0.0, B = 1.0;
+#pragma unroll
+  for (int t = 0; t < kRingChunk; ++t)
+    if (t < cnt) {
+      A = fa[t] + fb[t] * A;
+      B = fb[t] * B;
+    }
+  for (int off = 1; off < 64; off <<= 1) {
+    const double ao = __shfl_up(A, off, 64), bo = __shfl_up(B, off, 64);
+    if (lane >= off) {
+      A = A + B * ao;
+      B = B * bo;
+    }
+  }
+  double w = __shfl_up(A, 1, 64);
+#pragma unroll
+  for (int t = 0; t < kRingChunk; ++t)
+    if (t < cnt) {
+      w = fa[t] + fb[t] * w;
+      v[t] = w;
+    }
+  // backward: y_j = x_j + gd_j y_{j+1} (gd_j = -g_{j+1}; 0 for the last point)
+  double C = 0.0, D = 1.0;
+#pragma unroll
+  for (int t = kRingChunk - 1; t >= 0; --t)
+    if (t < cnt) {
+      C = v[t] + gd[t] * C;
+      D = gd[t] * D;
+    }
+  for (int off = 1; off < 64; off <<= 1) {
+    const double co = __shfl_down(C, off, 64), dd = __shfl_down(D, off, 64);
+    if (lane + off < 64) {
+      C = C + D * co;
+      D = D * dd;
+    }
+  }
+  w = __shfl_down(C, 1, 64);
+#pragma unroll
+  for (int t = kRingChunk - 1; t >= 0; --t)
+    if (t < cnt) {
+      w = v[t] + gd[t] * w;
+      v[t] = w;
+    }
+  // Sherman-Morrison correction with y_0 and y_{np-1}
+  const double y0 = __shfl(v[0], 0, 64);
+  const int owner = (np - 1) / k;
+  double mine_last = v[0];
+#pragma unroll
+  for (int t = 0; t < kRingChunk; ++t)
+    if (t == (np - 1) - owner * k) mine_last = v[t];
+  const double yl = __shfl(mine_last, owner, 64);
+  const double gam = -c.vec(h, V_DIAG)[base];
+  const double fact = (y0 + a[0] * yl / gam) / c.vec(h, V_RDEN)[base];
+#pragma unroll
+  for (int t = 0; t < kRingChunk; ++t)
+    if (t < cnt) x[j0 + t] = v[t] - fact * zz[t];
 }
 
 GCS_DEV void precondition_all(const MiK &c, int h, const double *in, double *out) {
   __syncthreads();  // `in` complete
   if (threadIdx.x == 0) out[0] = in[0] / c.vec(h, V_DIAG)[0];
-  for (int ring = threadIdx.x; ring < c.nt - 2; ring += blockDim.x) ring_solve(c, h, in, out, ring);
+  const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+  for (int ring = threadIdx.x >> 6; ring < c.nt - 2; ring += waves)
+    ring_apply(c, h, in, out, ring, lane);
   __syncthreads();  // `out` complete
+}
+
+// (A x)[0], the pole row, by wavefront 0 (every lane returns it).
+GCS_DEV double pole_row(const MiK &c, int h, bool tensor, const double *x, double pe) {
+  const int np = c.np;
+  double ring = 0.0;
+  for (int j = threadIdx.x & 63; j < np; j += 64) ring += x[red_at(np, 1, j)];
+  for (int off = 32; off > 0; off >>= 1) ring += __shfl_xor(ring, off, 64);
+  if (!tensor) return pe * ((double)np * x[0] - ring);
+  return c.vec(h, V_DIAG)[0] * x[0] - pe * ring;
 }
 
 struct SolveResult {
@@ -632,8 +728,9 @@ __global__ void __launch_bounds__(kSolveThreads) k_mi_cg(MiK c, double rel_tol, 
          *ap = c.vec(h, V_AP);
   const double *rhs = c.vec(h, V_RHS), *diag = c.vec(h, V_DIAG);
   double a = 0.0, b = 0.0;
+  const double pole = threadIdx.x < 64 ? pole_row(c, h, false, x, pe) : 0.0;
   for (long q = t0; q < n; q += st) {
-    r[q] = rhs[q] - apply_at(c, h, false, x, q, rows, pe);
+    r[q] = rhs[q] - (q == 0 ? pole : apply_at(c, h, false, x, q, rows, pe));
     a += orig_term(c, r[q], q, ps, dt);
     b += q == 0 ? 0.0 : orig_term(c, rhs[q], q, ps, dt);
   }
@@ -651,8 +748,9 @@ __global__ void __launch_bounds__(kSolveThreads) k_mi_cg(MiK c, double rel_tol, 
     double rz = block_sum(a, red);
     for (int it = 1; it <= maxit; ++it) {
       a = 0.0;
+      const double pole = threadIdx.x < 64 ? pole_row(c, h, false, p, pe) : 0.0;
       for (long q = t0; q < n; q += st) {
-        ap[q] = apply_at(c, h, false, p, q, rows, pe);
+        ap[q] = (q == 0 ? pole : apply_at(c, h, false, p, q, rows, pe));
         a += p[q] * ap[q];
       }
       const double pap = block_sum(a, red);
@@ -670,8 +768,9 @@ __global__ void __launch_bounds__(kSolveThreads) k_mi_cg(MiK c, double rel_tol, 
       if (norm <= target || it % 1024 == 0) {
         __syncthreads();  // x complete
         a = 0.0;
+        const double pole = threadIdx.x < 64 ? pole_row(c, h, false, x, pe) : 0.0;
         for (long q = t0; q < n; q += st) {
-          r[q] = rhs[q] - apply_at(c, h, false, x, q, rows, pe);
+          r[q] = rhs[q] - (q == 0 ? pole : apply_at(c, h, false, x, q, rows, pe));
           a += orig_term(c, r[q], q, ps, dt);
         }
         norm = sqrt(fmax(0.0, block_sum(a, red)));
@@ -718,9 +817,12 @@ __global__ void __launch_bounds__(kSolveThreads) k_mi_bicgstab(MiK c, double rel
          *sh = c.vec(h, V_SH), *pp = c.vec(h, V_PP), *s = c.vec(h, V_S), *psv = c.vec(h, V_PS),
          *as = c.vec(h, V_AS);
   const double *rhs = c.vec(h, V_RHS);
+  factor_rings(c, h);
+  __syncthreads();  // ring factors complete
   double a = 0.0, b = 0.0;
+  const double pole = threadIdx.x < 64 ? pole_row(c, h, true, x, pe) : 0.0;
   for (long q = t0; q < n; q += st) {
-    r[q] = rhs[q] - apply_at(c, h, true, x, q, nullptr, pe);
+    r[q] = rhs[q] - (q == 0 ? pole : apply_at(c, h, true, x, q, nullptr, pe));
     sh[q] = r[q];
     p[q] = 0.0;
     ap[q] = 0.0;
@@ -742,8 +844,9 @@ __global__ void __launch_bounds__(kSolveThreads) k_mi_bicgstab(MiK c, double rel
     for (long q = t0; q < n; q += st) p[q] = r[q] + beta * (p[q] - omega * ap[q]);
     precondition_all(c, h, p, pp);
     a = 0.0;
+    const double pole = threadIdx.x < 64 ? pole_row(c, h, true, pp, pe) : 0.0;
     for (long q = t0; q < n; q += st) {
-      ap[q] = apply_at(c, h, true, pp, q, nullptr, pe);
+      ap[q] = (q == 0 ? pole : apply_at(c, h, true, pp, q, nullptr, pe));
       a += sh[q] * ap[q];
     }
     const double den = block_sum(a, red);
@@ -765,8 +868,9 @@ __global__ void __launch_bounds__(kSolveThreads) k_mi_bicgstab(MiK c, double rel
     precondition_all(c, h, s, psv);
     a = 0.0;
     b = 0.0;
+    const double pole_s = threadIdx.x < 64 ? pole_row(c, h, true, psv, pe) : 0.0;
     for (long q = t0; q < n; q += st) {
-      as[q] = apply_at(c, h, true, psv, q, nullptr, pe);
+      as[q] = (q == 0 ? pole_s : apply_at(c, h, true, psv, q, nullptr, pe));
       a += as[q] * as[q];
       b += as[q] * s[q];
     }
@@ -787,8 +891,9 @@ __global__ void __launch_bounds__(kSolveThreads) k_mi_bicgstab(MiK c, double rel
     if (norm <= target || it % 256 == 0) {
       __syncthreads();  // x complete
       a = 0.0;
+      const double pole = threadIdx.x < 64 ? pole_row(c, h, true, x, pe) : 0.0;
       for (long q = t0; q < n; q += st) {
-        r[q] = rhs[q] - apply_at(c, h, true, x, q, nullptr, pe);
+        r[q] = rhs[q] - (q == 0 ? pole : apply_at(c, h, true, x, q, nullptr, pe));
         sh[q] = r[q];
         a += orig_term(c, r[q], q, ps, dt);
       }
@@ -967,8 +1072,8 @@ int mi_create(Problem &P) {
       mc.maximum_iterations <= 0 || !(mc.relative_tolerance > 0.0) ||
       L.nr < 3 || (int)mc.current_shell + 1 >= L.nr ||
       !(mc.ionosphere_radius_re < cfg.startup.inner_radius) ||
-      !(cfg.hall_siemens >= 0.0)) {
-    set_error("invalid MI configuration");
+      !(cfg.hall_siemens >= 0.0) || mc.azimuth_points > 64 * kRingChunk) {
+    set_error("invalid MI configuration (azimuth points must be 8..512)");
     return -1;
   }
   MiState *mp = new MiState();
@@ -1465,7 +1570,7 @@ int mi_train_warm_start(Problem &P, unet::Net &n, const gcs_warm_train_config &t
     const double warm = std::min(1.0, (double)(step + 1) / 50.0);
     const double frac = (double)step / (double)std::max(1L, t.steps - 1);
     const double lr = t.learning_rate * warm * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(kPi * frac)));
-    if (unet::forward_device(n, t.batch) != 0 || unet::loss_device(n, t.batch) != 0 ||
+    if (unet::forward_device(n, t.batch) != 0 || unet::loss_relative_device(n, t.batch) != 0 ||
         unet::backward_device(n, t.batch) != 0 || unet::adam_device(n, (float)lr) != 0) {
       rc = -1;
       break;

@@ -334,6 +334,43 @@ __global__ void k_mse(const float *out, const float *target, int M, float *g, fl
   if (threadIdx.x == 0) atomicAdd(loss, red[0]);
 }
 
+// Mean square of each sample's target (one block per sample).
+__global__ void k_sample_energy(const float *target, int P, float *energy) {
+  __shared__ float red[kT];
+  const float *t = target + (long)blockIdx.x * P;
+  float s = 0.f;
+  for (int m = threadIdx.x; m < P; m += kT) s += t[m] * t[m];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) energy[blockIdx.x] = fmaxf(red[0] / (float)P, 1e-30f);
+}
+
+// Relative error: loss += mean over samples of mean_pixels (out - t)^2 /
+// energy_b; g = 2 (out - t) / (B P energy_b).
+__global__ void k_mse_rel(const float *out, const float *target, int B, int P,
+                          const float *energy, float *g, float *loss) {
+  __shared__ float red[kT];
+  const long m = gtid();
+  float d2 = 0.f;
+  if (m < (long)B * P) {
+    const float e = energy[m / P];
+    const float d = out[m] - target[m];
+    g[m] = 2.f * d / ((float)B * (float)P * e);
+    d2 = d * d / ((float)B * (float)P * e);
+  }
+  red[threadIdx.x] = d2;
+  __syncthreads();
+  for (int s = kT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+}
+
 __global__ void k_adam(float *p, const float *g, float *mo, float *ve, long count, float lr,
                        float b1, float b2, float c1, float c2, float eps) {
   const long t = gtid();
@@ -491,6 +528,7 @@ Net *create(int H, int W, int max_batch, uint64_t seed) {
   n.target = dalloc<float>(n, pix0);
   n.input = dalloc<float>(n, pix0 * kInChannels);
   n.loss = dalloc<float>(n, 1);
+  n.energy = dalloc<float>(n, max_batch);
   for (void *p : n.allocations)
     if (p == nullptr) {
       set_error("warm-start network: device allocation failed");
@@ -511,7 +549,7 @@ Net *create(int H, int W, int max_batch, uint64_t seed) {
     const size_t cnt = (size_t)kLayers[l].cout * 9 * kLayers[l].cin;
     for (size_t i = 0; i < cnt; ++i) host[n.w_off[l] + i] = normal(rng) * sd;
   }
-  for (int c = 0; c < kHeadIn; ++c) host[n.head_w_off + c] = normal(rng) * 0.1f;
+  for (int c = 0; c < kHeadIn; ++c) host[n.head_w_off + c] = normal(rng) * 0.01f;
   if (set_params(n, host.data(), host.size()) != 0) {
     delete net;
     return nullptr;
@@ -662,6 +700,17 @@ int loss_device(Net &n, int batch) {
   const long pix0 = (long)batch * n.H * n.W;
   k_fill<<<1, 64, 0, n.stream>>>(n.loss, 1, 0.f);
   k_mse<<<nb(pix0), kT, 0, n.stream>>>(n.out, n.target, (int)pix0, n.gout, n.loss);
+  GCS_HIP(hipGetLastError());
+  return 0;
+}
+
+// Per-sample relative error of n.out against n.target (training loss).
+int loss_relative_device(Net &n, int batch) {
+  const int P = n.H * n.W;
+  k_fill<<<1, 64, 0, n.stream>>>(n.loss, 1, 0.f);
+  k_sample_energy<<<batch, kT, 0, n.stream>>>(n.target, P, n.energy);
+  k_mse_rel<<<nb((long)batch * P), kT, 0, n.stream>>>(n.out, n.target, batch, P, n.energy,
+                                                      n.gout, n.loss);
   GCS_HIP(hipGetLastError());
   return 0;
 }

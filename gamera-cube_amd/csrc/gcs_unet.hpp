@@ -47,6 +47,7 @@ struct Net {
   float *input = nullptr;                     // fp32 features [N][H][W][16]
   float *out = nullptr, *gout = nullptr, *target = nullptr;
   float *loss = nullptr;
+  float *energy = nullptr;                    // per-sample target mean square
   // potential = out * out_scale * R^2 * scale / SigmaP (set on the first
   // training batch so that targets have unit RMS)
   double out_scale = 1.0;
@@ -62,6 +63,7 @@ int get_array(Net &n, int which, float *host, size_t count);  // 0 params, 1 gra
 int check_batch(const Net &n, int batch);
 int forward_device(Net &n, int batch);
 int loss_device(Net &n, int batch);
+int loss_relative_device(Net &n, int batch);
 int backward_device(Net &n, int batch);
 int adam_device(Net &n, float lr);
 int gemm_check(const uint16_t *A, const uint16_t *B, int M, int N, int K, float *C);

@@ -35,7 +35,8 @@ typedef struct gcs_unet gcs_unet;
 /* Training on generated pairs: each step draws `batch` random smooth
  * potentials with uniform conductances in the given ranges, computes the
  * current for which each solves the problem's polar system, and takes one
- * Adam step on the mean square error of the network's potential. The
+ * Adam step on the network's relative error (per sample, the mean square
+ * error over the mean square target, averaged over the batch). The
  * learning rate decays by a cosine to 5% after a 50-step linear warmup. */
 typedef struct {
   long steps;

@@ -139,7 +139,7 @@ def init_params(seed):
         std = np.sqrt(2.0 / (9.0 * cin))
         params[name + ".w"] = (rng.standard_normal((cout, 3, 3, cin)) * std).astype(np.float32)
         params[name + ".b"] = np.zeros(cout, dtype=np.float32)
-    params["head.w"] = (rng.standard_normal(HEAD_IN) * 0.1).astype(np.float32)
+    params["head.w"] = (rng.standard_normal(HEAD_IN) * 0.01).astype(np.float32)
     params["head.b"] = np.zeros(1, dtype=np.float32)
     return params
 
