@@ -80,6 +80,120 @@ def cpu_baseline(gpu_problem, cfg, steps=2):
                       f"{seconds:.1f} s"}
 
 
+def active_state(p, cfg, prob):
+    """(rho, p, |v|, |B|) of the active cells, and max |div B| (net total
+    face flux over volume, test_cubedsphere_dipole_background.c:808-845,
+    computed on the device)."""
+    import numpy as np
+    ni, nj, nk = cfg.extent
+    g, nr, n = cfg.ghost, cfg.radial_cells, cfg.angular_cells
+    act = (slice(g, g + nr), slice(g, g + n), slice(g, g + n))
+    out = {"rho": [], "p": [], "v": [], "b": []}
+    gamma = cfg.startup.gamma
+    for f in range(6):
+        u = p.field(f, prob.CONSERVED).reshape(ni, nj, nk, 5)[act]
+        b = p.field(f, prob.CELL_MAGNETIC).reshape(ni, nj, nk, 3)[act]
+        rho = u[..., 0]
+        v = u[..., 1:4] / rho[..., None]
+        out["rho"].append(rho.ravel())
+        out["p"].append(((gamma - 1.0) * (u[..., 4] - 0.5 * rho * (v * v).sum(-1))).ravel())
+        out["v"].append(np.linalg.norm(v, axis=-1).ravel())
+        out["b"].append(np.linalg.norm(b, axis=-1).ravel())
+    return {k: np.concatenate(v) for k, v in out.items()}, p.max_div_b()
+
+
+def mixed_error(args, cfgm, native, prob):
+    """The mixed form against the FP64 step on the same problem after
+    `args.mixed_error` driver steps: relative L2 / Linf of rho, p, |v|, |B|
+    and max |div B| of both. The FP64 step is the oracle's to 1e-12 (parity
+    tests), so it stands for the oracle at this size."""
+    import numpy as np
+    cfg = cfgm.workload(args.workload)
+    runs = {}
+    for precision in ("fp64", "mixed"):
+        g = native.GpuProblem(cfg)
+        g.set_fidelity(False)
+        g.set_graph(True)
+        g.set_precision(precision == "mixed")
+        t0 = time.perf_counter()
+        done = 0
+        while done < args.mixed_error:
+            k = min(100, args.mixed_error - done)
+            g.advance(k)
+            done += k
+            print(f"bench: {precision} {done} steps ({time.perf_counter() - t0:.1f} s)",
+                  file=sys.stderr, flush=True)
+        state, div = active_state(g, cfg, prob)
+        mi = g.mi_report() if cfg.mi_enabled else None
+        runs[precision] = (state, div, g.clock()["time"], mi)
+        g.close()
+    ref, mix = runs["fp64"][0], runs["mixed"][0]
+    err = {}
+    for k in ("rho", "p", "v", "b"):
+        d = mix[k] - ref[k]
+        err[k] = {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(ref[k])),
+                  "rel_linf": float(np.abs(d).max() / np.abs(ref[k]).max())}
+    print(json.dumps({"mixed_error": {"workload": args.workload, "steps": args.mixed_error,
+                                      "errors": err,
+                                      "max_div_b": {"fp64": runs["fp64"][1],
+                                                    "mixed": runs["mixed"][1]},
+                                      "time_code": {"fp64": runs["fp64"][2],
+                                                    "mixed": runs["mixed"][2]},
+                                      "mi_updates": {k: (v[3]["updates"] if v[3] else 0)
+                                                     for k, v in runs.items()}}}))
+    return 0
+
+
+def large_run(args, cfgm, native, prob):
+    """Config 4: 6 x 192^2 x 384 with MI, Bz switching from +5 to -5 nT at
+    about step 500 (the code time of 500 first-step dts), run for
+    `args.large_run` hipGraph-launched driver steps in FP64 and in the mixed
+    form; cell-updates/s of each and the mixed form's error against FP64."""
+    import numpy as np
+    probe = native.GpuProblem(cfgm.workload("config4"))
+    probe.advance(1)
+    t_switch = 500.0 * probe.clock()["dt"]
+    probe.close()
+    results, states = {}, {}
+    for precision in ("fp64", "mixed"):
+        cfg = cfgm.bz_switch(cfgm.workload("config4"), t_switch, -5.0)
+        t0 = time.perf_counter()
+        g = native.GpuProblem(cfg)
+        setup_s = time.perf_counter() - t0
+        g.set_fidelity(False)
+        g.set_graph(True)
+        g.set_precision(precision == "mixed")
+        total_ms, done = 0.0, 0
+        while done < args.large_run:
+            k = min(100, args.large_run - done)
+            ms, _ = g.time_steps(k, profile=False)
+            total_ms += ms
+            done += k
+            print(f"bench: config4 {precision} {done} steps, {total_ms / done:.2f} ms/step",
+                  file=sys.stderr, flush=True)
+        mi = g.mi_report()
+        states[precision] = active_state(g, cfg, prob)
+        results[precision] = {
+            "cell_updates_per_s": cfg.active_cells * done / (total_ms * 1e-3),
+            "ms_per_step": total_ms / done, "setup_s": setup_s,
+            "time_code": g.clock()["time"], "mi_updates": mi["updates"],
+            "last_mi_iterations": mi["iterations"], "last_mi_ms": mi["solve_ms"]}
+        g.close()
+    ref, mix = states["fp64"][0], states["mixed"][0]
+    err = {}
+    for k in ("rho", "p", "v", "b"):
+        d = mix[k] - ref[k]
+        err[k] = {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(ref[k])),
+                  "rel_linf": float(np.abs(d).max() / np.abs(ref[k]).max())}
+    print(json.dumps({"large_run": {
+        "workload": "config4 (6x192^2x384, MI 2x128x256 Hall 5 S, Bz +5 -> -5 nT)",
+        "active_cells": cfgm.workload("config4").active_cells, "steps": args.large_run,
+        "t_switch_code": t_switch, "runs": results, "mixed_vs_fp64": err,
+        "max_div_b": {"fp64": states["fp64"][1], "mixed": states["mixed"][1]},
+        "step_launch": "hipGraph"}}))
+    return 0
+
+
 def measured_traffic(workload, stage):
     """HBM bytes per launch of the stage's kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary of this workload (calibrated against a copy
@@ -108,6 +222,11 @@ def main():
                     help="only run a plain copy kernel over N doubles (read N, "
                          "write N) three times: the known-size reference for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--mixed-error", type=int, default=0, metavar="STEPS",
+                    help="only compare the mixed form with the FP64 step after STEPS "
+                         "driver steps of the workload and print the errors")
+    ap.add_argument("--large-run", type=int, default=0, metavar="STEPS",
+                    help="only run config 4 (FP64 and mixed, Bz switch) for STEPS steps")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -127,6 +246,10 @@ def main():
         return 2
     cfgm = importlib.import_module("gamera-cube_amd.config")
     native = importlib.import_module("gamera-cube_amd.native")
+    if args.mixed_error:
+        return mixed_error(args, cfgm, native, importlib.import_module("gamera-cube_amd.problem"))
+    if args.large_run:
+        return large_run(args, cfgm, native, importlib.import_module("gamera-cube_amd.problem"))
     cfg = cfgm.workload(args.workload)
     print(f"bench: workload {args.workload} {cfg.active} x 6, setup ...",
           file=sys.stderr, flush=True)

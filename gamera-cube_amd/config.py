@@ -195,6 +195,18 @@ def earth_problem(radial_cells=48, angular_cells=24, *, n_cm3=5.0,
     return c
 
 
+def bz_switch(c, t_switch, bz_after_nt):
+    """A piecewise-constant wind Bz series: the startup IMF until code time
+    t_switch, then bz_after_nt."""
+    c.wind_count = 2
+    c.wind_linear = 0
+    c.wind_time[0] = 0.0
+    c.wind_bz[0] = c.startup.imf.value[2]
+    c.wind_time[1] = t_switch
+    c.wind_bz[1] = nt_to_code(bz_after_nt)
+    return c
+
+
 def workload(name):
     """The named workloads: config0 .. config4 and small parity cases."""
     if name == "config0":      # 6 x 24^2 x 48, 200 steps, potential 0
@@ -204,10 +216,9 @@ def workload(name):
                              hall_siemens=5.0)
     if name == "config1_nomi":
         return earth_problem(192, 96, perturbation=0.01, seed=1)
-    if name == "config4":      # 6 x 192^2 x 384
-        c = earth_problem(384, 192, perturbation=0.01, seed=1, mi=True,
-                          hall_siemens=5.0)
-        return c
+    if name == "config4":      # 6 x 192^2 x 384, Bz +5 nT (bz_switch -> -5)
+        return earth_problem(384, 192, bz_nt=5.0, perturbation=0.01, seed=1, mi=True,
+                             hall_siemens=5.0)
     if name == "tiny":         # 4 ghosts need n > 8 (cubedsphere_mesh.c:191)
         return earth_problem(8, 12)
     if name == "small":

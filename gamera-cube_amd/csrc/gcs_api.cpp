@@ -440,6 +440,11 @@ int gcs_problem_set_precision(Problem *P, int mixed) {
   return 0;
 }
 
+int gcs_problem_max_div_b(Problem *P, double *out) {
+  if (P == nullptr || out == nullptr) return -1;
+  return gcs::max_div_b(*P, out);
+}
+
 int gcs_problem_set_graph(Problem *P, int on) {
   if (P == nullptr) return -1;
   P->use_graph = on != 0;

@@ -236,6 +236,7 @@ int enqueue_step(Problem &P, bool explicit_dt);
 // edge_emf_sync), then seam EMF average, update and final seam sync. The
 // last phase swaps the time levels.
 enum Phase { PHASE_GHOSTS, PHASE_FLUX, PHASE_EMF, PHASE_UPDATE };
+int max_div_b(Problem &P, double *out);
 int enqueue_phase(Problem &P, Phase phase);
 // Constants derived from uploaded geometry and background: recovery
 // pivots, edge-B solve, B0 normal and edge-tangential components.

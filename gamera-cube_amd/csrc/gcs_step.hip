@@ -22,6 +22,8 @@
 //    zeros for finite donors, cubedsphere_halo_fastpath.h:1-24).
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -117,6 +119,19 @@ GCS_DEV bool ghost_cell(long t, const Layout &L, int &f, int &i, int &j, int &k)
 }
 
 GCS_DEV long tid() { return blockIdx.x * (long)blockDim.x + threadIdx.x; }
+
+// Launch index with an XCD-aware block order: workgroups are dispatched
+// round-robin over the 8 XCDs, each with its own L2, so block b runs on XCD
+// b % 8; giving each XCD one contiguous range of logical blocks keeps the
+// stencil neighbours of a block in the same L2. (A bijection of [0, nb).)
+// Used by the face-velocity and edge-EMF sweeps, whose 8-point stencils
+// reach across k-rows; plan-driven kernels keep the plain order.
+GCS_DEV long tid_xcd() {
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned q = nb >> 3, r = nb & 7, x = b & 7, slot = b >> 3;
+  const unsigned lb = x * q + min(x, r) + slot;
+  return lb * (long)blockDim.x + threadIdx.x;
+}
 
 constexpr int kCflThreads = 256;
 
@@ -919,7 +934,7 @@ __global__ void k_face_velocity(K c, int d) {
     }
   }
   int f, i, j, k;
-  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  if (!box(tid_xcd(), lo, hi, f, i, j, k)) return;
   const long at = L.at(f, i, j, k);
   const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
   double vol[8], wv[3][8];
@@ -943,7 +958,7 @@ __global__ void __launch_bounds__(128) k_edge_emf(K c, int e) {
   const int hi[3] = {L.ghost + L.nr + (e != 0), L.ghost + L.n + (e != 1),
                      L.ghost + L.n + (e != 2)};
   int f, i, j, k;
-  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  if (!box(tid_xcd(), lo, hi, f, i, j, k)) return;
   const long at = L.at(f, i, j, k);
   if (!c.b(Q_VALID + e)[at] || !c.b(Q_EBOK + e)[at]) {
     c.fail(51, at);
@@ -1142,7 +1157,7 @@ __global__ void k_face_velocity_fp32(K c, int d) {
     }
   }
   int f, i, j, k;
-  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  if (!box(tid_xcd(), lo, hi, f, i, j, k)) return;
   const long at = L.at(f, i, j, k);
   const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
   R vol[8], wv[3][8];
@@ -1166,7 +1181,7 @@ __global__ void __launch_bounds__(128) k_edge_emf_fp32(K c, int e) {
   const int hi[3] = {L.ghost + L.nr + (e != 0), L.ghost + L.n + (e != 1),
                      L.ghost + L.n + (e != 2)};
   int f, i, j, k;
-  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  if (!box(tid_xcd(), lo, hi, f, i, j, k)) return;
   const long at = L.at(f, i, j, k);
   if (!c.b(Q_VALID + e)[at] || !c.b(Q_EBOK + e)[at]) {
     c.fail(51, at);
@@ -1683,6 +1698,44 @@ int enqueue_phase(Problem &P, Phase phase) {
     P.b_recovered = true;
   }
   GCS_HIP(hipGetLastError());
+  return 0;
+}
+
+// max over active cells of |net total (B1 + B0) face flux| / volume, the
+// reference driver's divergence gate (test_cubedsphere_dipole_background.c:
+// 808-845); non-negative doubles order like their bit patterns.
+__global__ void k_div_b(K c, unsigned long long *out) {
+  const Layout &L = c.L;
+  const int lo[3] = {L.ghost, L.ghost, L.ghost};
+  const int hi[3] = {L.ghost + L.nr, L.ghost + L.n, L.ghost + L.n};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  const long step[3] = {L.di(), (long)L.SK, 1};
+  double net = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const double *phi = c.p(plane_phi(c.cur, d)), *b0 = c.p(P_B0FLUX + d);
+    net += (phi[at + step[d]] + b0[at + step[d]]) - (phi[at] + b0[at]);
+  }
+  const double v = fabs(net) / c.p(P_VOL)[at];
+  atomicMax(out, (unsigned long long)__double_as_longlong(isfinite(v) ? v : DBL_MAX));
+}
+
+int max_div_b(Problem &P, double *out) {
+  unsigned long long *dev = nullptr, host = 0;
+  GCS_HIP(hipMalloc(&dev, sizeof(host)));
+  GCS_HIP(hipMemsetAsync(dev, 0, sizeof(host), P.stream));
+  K c = make_k(P);
+  k_div_b<<<nblocks(6L * P.L.nr * P.L.n * P.L.n, 256), 256, 0, P.stream>>>(c, dev);
+  const hipError_t e = hipGetLastError();
+  const hipError_t m = hipMemcpyAsync(&host, dev, sizeof(host), hipMemcpyDeviceToHost, P.stream);
+  const hipError_t w = hipStreamSynchronize(P.stream);
+  (void)hipFree(dev);
+  if (e != hipSuccess || m != hipSuccess || w != hipSuccess) {
+    set_error("divergence check failed");
+    return -1;
+  }
+  std::memcpy(out, &host, sizeof(double));
   return 0;
 }
 

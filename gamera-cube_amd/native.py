@@ -149,3 +149,16 @@ def _set_graph(self, on):
 
 
 GpuProblem.set_graph = _set_graph
+
+
+def _max_div_b(self):
+    """max |div B| over the active cells (net total face flux / volume)."""
+    out = ctypes.c_double()
+    self._lib.gcs_problem_max_div_b.argtypes = [ctypes.c_void_p,
+                                                ctypes.POINTER(ctypes.c_double)]
+    if self._lib.gcs_problem_max_div_b(self._handle, ctypes.byref(out)) != 0:
+        raise RuntimeError(f"gcs_problem_max_div_b: {last_error()}")
+    return out.value
+
+
+GpuProblem.max_div_b = _max_div_b

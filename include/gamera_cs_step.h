@@ -82,6 +82,11 @@ int gcs_problem_set_precision(gcs_problem *problem, int mixed);
  * either way. No reference counterpart. */
 int gcs_problem_set_graph(gcs_problem *problem, int on);
 
+/* max over active cells of |net total (B1 + B0) face flux| / volume of the
+ * current level: the divergence gate of the reference's Earth driver
+ * (test_cubedsphere_dipole_background.c:808-845), computed on the device. */
+int gcs_problem_max_div_b(gcs_problem *problem, double *out);
+
 /* Timing of `steps` driver steps with HIP events on the problem's stream:
  * total milliseconds, and with profile != 0 the milliseconds of each step
  * stage summed over the steps (gcs_stage_count() entries named by
