@@ -194,6 +194,49 @@ def large_run(args, cfgm, native, prob):
     return 0
 
 
+def warm_start_report(g, cfg, steps):
+    """Train the warm-start U-Net on the GPU for the problem's MI grid, then
+    compare Krylov iterations and milliseconds with and without it: on a
+    generated current that changes by 30 % between two updates, and on the
+    problem's own current against the potential of its last MI update."""
+    import numpy as np
+    ws = importlib.import_module("gamera-cube_amd.warm_start")
+    nt, np_ = cfg.mi.colatitude_points, cfg.mi.azimuth_points
+    net = ws.WarmStartNet(nt, np_, 4, seed=2)
+    t0 = time.perf_counter()
+    losses = ws.train(g, net, steps=steps, batch=4, learning_rate=1e-3, seed=9)
+    train_s = time.perf_counter() - t0
+    print(f"bench: warm start trained {steps} steps in {train_s:.1f} s, relative loss "
+          f"{losses[:10].mean():.3e} -> {losses[-50:].mean():.3e}", file=sys.stderr, flush=True)
+    ws.attach(g, net)
+    out = {"network": "4-level U-Net, 32 base channels, bf16/fp32 MFMA",
+           "train_steps": steps, "train_batch": 4, "train_s": train_s,
+           "relative_loss_first": float(losses[:10].mean()),
+           "relative_loss_last": float(losses[-50:].mean())}
+    # the problem's own current about 8 simulated seconds after its last MI
+    # update (the update interval is 10 s) against that update's potential
+    g.advance(300)
+    pot_prev = g.mi_field(0).reshape(2, nt, np_)
+    g.mi_sample_current()
+    cur_now = g.mi_field(1).reshape(2, nt, np_)
+    cases = {"mhd_current_vs_last_potential": (cur_now, pot_prev)}
+    sp, sh = cfg.mi.pedersen_siemens, cfg.hall_siemens
+    cur_a, pot_a = ws.make_pairs(g, seed=21, count=2, pedersen=sp, hall=sh)
+    cur_d, _ = ws.make_pairs(g, seed=22, count=2, pedersen=sp, hall=sh)
+    cases["generated_30pct_change"] = (
+        cur_a + 0.3 * cur_d * (np.abs(cur_a).max() / np.abs(cur_d).max()), pot_a)
+    for name, (cur, guess) in cases.items():
+        plain, it_p, ms_p = ws.solve_current(g, cur, guess, use_network=False)
+        learned, it_l, ms_l = ws.solve_current(g, cur, guess, use_network=True)
+        out[name] = {"plain_iterations": it_p, "plain_ms": ms_p,
+                     "learned_iterations": it_l, "learned_ms": ms_l,
+                     "max_potential_difference_rel": float(
+                         np.abs(learned - plain).max() / max(np.abs(plain).max(), 1e-300))}
+    ws.attach(g, None)
+    net.close()
+    return out
+
+
 def measured_traffic(workload, stage):
     """HBM bytes per launch of the stage's kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary of this workload (calibrated against a copy
@@ -225,6 +268,9 @@ def main():
     ap.add_argument("--mixed-error", type=int, default=0, metavar="STEPS",
                     help="only compare the mixed form with the FP64 step after STEPS "
                          "driver steps of the workload and print the errors")
+    ap.add_argument("--warm-start", type=int, default=0, metavar="STEPS",
+                    help="after the timed steps, train the MI warm-start network for "
+                         "STEPS steps and report its iterations/ms against the plain solve")
     ap.add_argument("--large-run", type=int, default=0, metavar="STEPS",
                     help="only run config 4 (FP64 and mixed, Bz switch) for STEPS steps")
     args = ap.parse_args()
@@ -325,6 +371,8 @@ def main():
         "step_launch": "hipGraph (one captured step per launch)",
         "cpu_baseline": None,
     }
+    if args.warm_start and cfg.mi_enabled:
+        result["config"]["warm_start"] = warm_start_report(g, cfg, args.warm_start)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (oracle, same workload) ...",
               file=sys.stderr, flush=True)

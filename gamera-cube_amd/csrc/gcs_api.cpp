@@ -440,6 +440,14 @@ int gcs_problem_set_precision(Problem *P, int mixed) {
   return 0;
 }
 
+int gcs_problem_mi_sample_current(Problem *P) {
+  if (P == nullptr || P->mi == nullptr) {
+    gcs::set_error("MI is not enabled");
+    return -1;
+  }
+  return gcs::mi_sample_current(*P);
+}
+
 int gcs_problem_max_div_b(Problem *P, double *out) {
   if (P == nullptr || out == nullptr) return -1;
   return gcs::max_div_b(*P, out);

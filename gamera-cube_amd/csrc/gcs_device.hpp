@@ -208,6 +208,7 @@ int check_failure(Problem &P);
 // (cubedsphere_mi_dipole.c:372-436). Returns 1 updated, 0 not due, -1.
 int mi_create(Problem &P);
 int mi_maybe_update(Problem &P, double time_code);
+int mi_sample_current(Problem &P);
 void mi_destroy(Problem &P);
 struct MiReport {
   int updates, ready;

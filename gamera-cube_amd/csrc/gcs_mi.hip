@@ -1257,10 +1257,26 @@ int mi_maybe_update(Problem &P, double time_code) {
   MiState &m = *mp;
   const double ts = time_code * m.cfg.length_norm_m / m.cfg.speed_norm_m_s;
   if (m.ready && !(ts + 1e-10 >= m.next_update)) return 0;
+  mark_stage(P, S_MI);
+  if (mi_sample_current(P) != 0) return -1;
+  if (solve(P, m.hall > 0.0) != 0) return -1;
+  m.ready = true;
+  m.epoch = ts;
+  ++m.updates;
+  do {
+    m.next_update += m.cfg.coupling_interval_s;
+  } while (m.next_update <= ts + 1e-10);
+  P.mi_view.potential = m.potential;
+  return 1;
+}
+
+// The field-aligned current of the current state on the polar grids
+// (:223-370): shell FAC, its fringe, bilinear samples.
+int mi_sample_current(Problem &P) {
+  MiState &m = *P.mi;
   MiK c = make_mik(P);
   hipStream_t s = P.stream;
   const Layout &L = P.L;
-  mark_stage(P, S_MI);
   k_mi_copy_shell<<<blocks_of(6L * 3 * L.NJ * L.NK), kT, 0, s>>>(c);
   k_mi_fringe_b<<<blocks_of(m.n_bfringe), kT, 0, s>>>(c, m.bfringe, m.n_bfringe);
   k_mi_shell_fac<<<blocks_of(6L * L.n * L.n), kT, 0, s>>>(c);
@@ -1275,15 +1291,7 @@ int mi_maybe_update(Problem &P, double time_code) {
     set_error("MI field-aligned current failed at stage " + std::to_string(sc.fail_stage));
     return -1;
   }
-  if (solve(P, m.hall > 0.0) != 0) return -1;
-  m.ready = true;
-  m.epoch = ts;
-  ++m.updates;
-  do {
-    m.next_update += m.cfg.coupling_interval_s;
-  } while (m.next_update <= ts + 1e-10);
-  P.mi_view.potential = m.potential;
-  return 1;
+  return 0;
 }
 
 void mi_report(const Problem &P, MiReport &out) {

@@ -126,6 +126,15 @@ def _mi_field(self, what):
     return np.ctypeslib.as_array(ptr, shape=(count.value,)).copy()
 
 
+def _mi_sample_current(self):
+    """Sample the current state's field-aligned current onto the polar grids
+    (no solve); mi_field(1) returns it."""
+    self._lib.gcs_problem_mi_sample_current.argtypes = [ctypes.c_void_p]
+    if self._lib.gcs_problem_mi_sample_current(self._handle) != 0:
+        raise RuntimeError(f"gcs_problem_mi_sample_current: {last_error()}")
+
+
+GpuProblem.mi_sample_current = _mi_sample_current
 GpuProblem.mi_report = _mi_report
 GpuProblem.mi_field = _mi_field
 

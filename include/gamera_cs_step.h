@@ -82,6 +82,12 @@ int gcs_problem_set_precision(gcs_problem *problem, int mixed);
  * either way. No reference counterpart. */
 int gcs_problem_set_graph(gcs_problem *problem, int on);
 
+/* Sample the field-aligned current of the current state onto the MI polar
+ * grids (cubedsphere_mi_dipole.c:223-370) without solving; the potential
+ * and the update clock are unchanged. gcs_problem_mi_field(what = 1) then
+ * returns it. No reference counterpart as a separate call. */
+int gcs_problem_mi_sample_current(gcs_problem *problem);
+
 /* max over active cells of |net total (B1 + B0) face flux| / volume of the
  * current level: the divergence gate of the reference's Earth driver
  * (test_cubedsphere_dipole_background.c:808-845), computed on the device. */
