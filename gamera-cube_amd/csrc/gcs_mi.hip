@@ -928,15 +928,42 @@ GCS_DEV double equivalent_current(double r, int i, int nt, double dt, double r2,
 }
 
 // Network features of one point (16 floats): current / scale, SigmaP / 10,
-// SigmaH / 10, colatitude index / (nt - 1), hemisphere sign, zeros.
+// SigmaH / 10, colatitude / maximum colatitude, hemisphere sign, zeros.
 GCS_DEV void write_features(float *f, double current, double scale, double sp, double sh,
-                            int i, int nt, int hemi) {
+                            double colat_fraction, int hemi) {
   f[0] = scale > 0.0 ? (float)(current / scale) : 0.f;
   f[1] = (float)(sp / 10.0);
   f[2] = (float)(sh / 10.0);
-  f[3] = (float)((double)i / (double)(nt - 1));
+  f[3] = (float)colat_fraction;
   f[4] = (float)hemi;
   for (int k = 5; k < unet::kInChannels; ++k) f[k] = 0.f;
+}
+
+// The network runs on the polar grid pooled by f x f (f = nt / H): the
+// Krylov iterations remove short-wavelength error quickly, the network
+// supplies the long-wavelength part, and a coarse grid lets a 4-level
+// U-Net's receptive field span the polar cap. Row of a coarse point, as a
+// fraction of the fine grid's colatitude range:
+GCS_DEV double coarse_row_fraction(int rc, int f, int nt) {
+  return ((double)rc * f + 0.5 * (double)(f - 1)) / (double)(nt - 1);
+}
+
+// Bilinear sample of a coarse [Hc][Wc] field at fine point (i, j):
+// rows clamped, columns periodic.
+GCS_DEV double coarse_sample(const float *o, int Hc, int Wc, int f, int i, int j) {
+  double y = ((double)i + 0.5) / f - 0.5, x = ((double)j + 0.5) / f - 0.5;
+  y = fmin(fmax(y, 0.0), (double)(Hc - 1));
+  const int y0 = min((int)floor(y), Hc - 2 < 0 ? 0 : Hc - 2);
+  const double wy = Hc > 1 ? y - y0 : 0.0;
+  const int y1 = min(y0 + 1, Hc - 1);
+  const double xf = floor(x);
+  const double wx = x - xf;
+  int x0 = (int)xf % Wc;
+  if (x0 < 0) x0 += Wc;
+  const int x1 = (x0 + 1) % Wc;
+  const double a = (1.0 - wx) * o[y0 * Wc + x0] + wx * o[y0 * Wc + x1];
+  const double b = (1.0 - wx) * o[y1 * Wc + x0] + wx * o[y1 * Wc + x1];
+  return (1.0 - wy) * a + wy * b;
 }
 
 // RMS of the residual's equivalent current over the interior rows, per
@@ -962,37 +989,48 @@ __global__ void k_warm_scale(MiK c, double *scale) {
   if (threadIdx.x == 0) scale[h] = sqrt(red[0] / (double)count);
 }
 
-__global__ void k_warm_features(MiK c, const double *scale, double hall, float *feat) {
+// Features on the coarse grid: the residual's equivalent current averaged
+// over each f x f block.
+__global__ void k_warm_features(MiK c, const double *scale, double hall, int f, float *feat) {
   const int h = blockIdx.y;
+  const int Hc = c.nt / f, Wc = c.np / f;
   const long q = gtid();
-  if (q >= (long)c.nt * c.np) return;
-  const int i = (int)(q / c.np), j = (int)(q % c.np);
+  if (q >= (long)Hc * Wc) return;
+  const int rc = (int)(q / Wc), cc = (int)(q % Wc);
   const double dt = c.max_colat / (double)(c.nt - 1);
-  const double r = (i > 0 && i < c.nt - 1) ? c.vec(h, V_R)[red_at(c.np, i, j)] : 0.0;
-  write_features(feat + ((long)h * c.nt * c.np + q) * unet::kInChannels,
-                 equivalent_current(r, i, c.nt, dt, c.r2, hemi_sign(h)), scale[h], c.pedersen,
-                 hall, i, c.nt, hemi_sign(h));
+  double sum = 0.0;
+  for (int a = 0; a < f; ++a)
+    for (int b = 0; b < f; ++b) {
+      const int i = rc * f + a, j = cc * f + b;
+      const double r = (i > 0 && i < c.nt - 1) ? c.vec(h, V_R)[red_at(c.np, i, j)] : 0.0;
+      sum += equivalent_current(r, i, c.nt, dt, c.r2, hemi_sign(h));
+    }
+  write_features(feat + ((long)h * Hc * Wc + q) * unet::kInChannels, sum / (double)(f * f),
+                 scale[h], c.pedersen, hall, coarse_row_fraction(rc, f, c.nt), hemi_sign(h));
 }
 
-// x += the network's potential correction (out * out_scale * R^2 * scale /
-// SigmaP on the interior rows; the pole unknown takes the mean of row 0).
-__global__ void k_warm_correct(MiK c, const double *scale, const float *out, double out_scale) {
+// x += the network's potential correction, interpolated from the coarse
+// grid (out * out_scale * R^2 * scale / SigmaP; the pole unknown takes the
+// mean over row 0).
+__global__ void k_warm_correct(MiK c, const double *scale, const float *out, double out_scale,
+                               int f) {
   const int h = blockIdx.y;
   const long q = gtid();
   if (q >= c.n) return;
   const double s = scale[h];
   if (!(s > 0.0) || !isfinite(s)) return;
   const double k = out_scale * c.r2 * s / c.pedersen;
-  const float *o = out + (long)h * c.nt * c.np;
+  const int Hc = c.nt / f, Wc = c.np / f;
+  const float *o = out + (long)h * Hc * Wc;
   double *x = c.vec(h, V_X);
   if (q == 0) {
     double mean = 0.0;
-    for (int j = 0; j < c.np; ++j) mean += (double)o[j];
+    for (int j = 0; j < c.np; ++j) mean += coarse_sample(o, Hc, Wc, f, 0, j);
     x[0] += k * (mean / (double)c.np);
     return;
   }
   const int i = 1 + (int)((q - 1) / c.np), j = (int)((q - 1) % c.np);
-  x[q] += k * (double)o[(long)i * c.np + j];
+  x[q] += k * coarse_sample(o, Hc, Wc, f, i, j);
 }
 
 // The guess in V_X (previous potential) becomes guess + N(residual): one
@@ -1006,14 +1044,15 @@ int warm_start(Problem &P, const MiK &c, bool tensor) {
   k_mi_apply<<<g2, kT, 0, s>>>(c, tensor ? 1 : 0, V_X, V_AP, m.active);
   k_mi_vec<<<g2, kT, 0, s>>>(c, OP_RESIDUAL, 0.0, m.active);
   k_warm_scale<<<dim3(1, 2), kT, 0, s>>>(c, m.warm_scale);
-  k_warm_features<<<dim3(blocks_of((long)m.nt * m.np), 2), kT, 0, s>>>(c, m.warm_scale, m.hall,
-                                                                       n.input);
+  const int f = m.nt / n.H;
+  k_warm_features<<<dim3(blocks_of((long)n.H * n.W), 2), kT, 0, s>>>(c, m.warm_scale, m.hall, f,
+                                                                    n.input);
   hipStream_t keep = n.stream;
   n.stream = s;
   const int rc = unet::forward_device(n, 2);
   n.stream = keep;
   if (rc != 0) return -1;
-  k_warm_correct<<<g2, kT, 0, s>>>(c, m.warm_scale, n.out, n.out_scale);
+  k_warm_correct<<<g2, kT, 0, s>>>(c, m.warm_scale, n.out, n.out_scale, f);
   GCS_HIP(hipGetLastError());
   return 0;
 }
@@ -1415,17 +1454,31 @@ __global__ void k_gen_scale(int nt, int np, const double *fac, double *scale) {
   if (threadIdx.x == 0) scale[blockIdx.x] = sqrt(red[0] / (double)count);
 }
 
-__global__ void k_gen_features(const GenSample *smp, int nt, int np, double r2,
+// Features and targets on the coarse grid (f x f block means of the fine
+// current and potential).
+__global__ void k_gen_features(const GenSample *smp, int nt, int np, int f, double r2,
                                const double *fac, const double *phi, const double *scale,
                                double out_scale, float *feat, float *target) {
   const int b = blockIdx.y;
+  const int Hc = nt / f, Wc = np / f;
   const long q = gtid();
-  if (q >= (long)nt * np) return;
-  const int i = (int)(q / np);
+  if (q >= (long)Hc * Wc) return;
+  const int rc = (int)(q / Wc), cc = (int)(q % Wc);
   const GenSample &g = smp[b];
-  const long at = (long)b * nt * np + q;
-  write_features(feat + at * unet::kInChannels, fac[at], scale[b], g.sp, g.sh, i, nt, g.hemi);
-  target[at] = (float)(phi[at] * g.sp / (r2 * scale[b] * out_scale));
+  const long base = (long)b * nt * np;
+  double cur = 0.0, pot = 0.0;
+  for (int a = 0; a < f; ++a)
+    for (int e = 0; e < f; ++e) {
+      const long at = base + (long)(rc * f + a) * np + cc * f + e;
+      cur += fac[at];
+      pot += phi[at];
+    }
+  cur /= (double)(f * f);
+  pot /= (double)(f * f);
+  const long out = (long)b * Hc * Wc + q;
+  write_features(feat + out * unet::kInChannels, cur, scale[b], g.sp, g.sh,
+                 coarse_row_fraction(rc, f, nt), g.hemi);
+  target[out] = (float)(pot * g.sp / (r2 * scale[b] * out_scale));
 }
 
 // A batch of random pair parameters (host side, seeded).
@@ -1487,15 +1540,23 @@ static int make_pairs(PairBatch &pb, int nt, int np, double max_colat, double r2
   return 0;
 }
 
+// The pooling factor f of a network for this MI grid (nt = f H, np = f W),
+// or 0 when the grids do not fit.
+static int coarsening(const MiState &m, const unet::Net &n) {
+  if (n.H <= 0 || n.W <= 0 || m.nt % n.H != 0 || m.np % n.W != 0) return 0;
+  const int f = m.nt / n.H;
+  return m.np / n.W == f ? f : 0;
+}
+
 int mi_attach_warm_start(Problem &P, unet::Net *net) {
   if (P.mi == nullptr) {
     set_error("MI is not enabled");
     return -1;
   }
   MiState &m = *P.mi;
-  if (net != nullptr && (net->H != m.nt || net->W != m.np || net->max_batch < 2)) {
-    set_error("warm-start network grid does not match the MI grid (colatitude x azimuth), "
-              "or its batch is below 2");
+  if (net != nullptr && (coarsening(m, *net) <= 0 || net->max_batch < 2)) {
+    set_error("warm-start network grid must be the MI grid (colatitude x azimuth) pooled "
+              "by one factor in both directions, and its batch at least 2");
     return -1;
   }
   if (m.warm_scale == nullptr) GCS_HIP(hipMalloc(&m.warm_scale, 2 * sizeof(double)));
@@ -1531,7 +1592,7 @@ int mi_train_warm_start(Problem &P, unet::Net &n, const gcs_warm_train_config &t
     return -1;
   }
   const MiState &m = *P.mi;
-  if (n.H != m.nt || n.W != m.np || unet::check_batch(n, t.batch) != 0 || t.steps <= 0 ||
+  if (coarsening(m, n) <= 0 || unet::check_batch(n, t.batch) != 0 || t.steps <= 0 ||
       !(t.learning_rate > 0.0) || !(t.pedersen_min > 0.0) ||
       !(t.pedersen_max >= t.pedersen_min) || !(t.hall_min >= 0.0) ||
       !(t.hall_max >= t.hall_min)) {
@@ -1546,7 +1607,8 @@ int mi_train_warm_start(Problem &P, unet::Net &n, const gcs_warm_train_config &t
   float *host_loss = nullptr;
   GCS_HIP(hipHostMalloc(&host_loss, sizeof(float)));
   std::mt19937_64 rng(t.seed);
-  const dim3 g(blocks_of((long)m.nt * m.np), t.batch);
+  const int f = coarsening(m, n);
+  const dim3 g(blocks_of((long)n.H * n.W), t.batch);
   int rc = 0;
   for (long step = 0; step < t.steps && rc == 0; ++step) {
     draw_samples(rng, t.batch, m.max_colat, t.pedersen_min, t.pedersen_max, t.hall_min,
@@ -1557,9 +1619,9 @@ int mi_train_warm_start(Problem &P, unet::Net &n, const gcs_warm_train_config &t
     }
     if (n.adam_t == 0 && step == 0) {
       // output scale: unit RMS of the first batch's targets
-      k_gen_features<<<g, kT, 0, s>>>(pb.dev, m.nt, m.np, r2, pb.fac, pb.phi, pb.scale, 1.0,
-                                      n.input, n.target);
-      std::vector<float> tg((size_t)t.batch * m.nt * m.np);
+      k_gen_features<<<g, kT, 0, s>>>(pb.dev, m.nt, m.np, f, r2, pb.fac, pb.phi, pb.scale,
+                                      1.0, n.input, n.target);
+      std::vector<float> tg((size_t)t.batch * n.H * n.W);
       if (hipMemcpyAsync(tg.data(), n.target, tg.size() * sizeof(float),
                          hipMemcpyDeviceToHost, s) != hipSuccess ||
           hipStreamSynchronize(s) != hipSuccess) {
@@ -1572,7 +1634,7 @@ int mi_train_warm_start(Problem &P, unet::Net &n, const gcs_warm_train_config &t
       n.out_scale = std::sqrt(ss / (double)tg.size());
       if (!(n.out_scale > 0.0) || !std::isfinite(n.out_scale)) n.out_scale = 1.0;
     }
-    k_gen_features<<<g, kT, 0, s>>>(pb.dev, m.nt, m.np, r2, pb.fac, pb.phi, pb.scale,
+    k_gen_features<<<g, kT, 0, s>>>(pb.dev, m.nt, m.np, f, r2, pb.fac, pb.phi, pb.scale,
                                     n.out_scale, n.input, n.target);
     // cosine decay to 5% of the rate after a short linear warmup
     const double warm = std::min(1.0, (double)(step + 1) / 50.0);

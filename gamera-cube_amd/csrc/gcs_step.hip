@@ -898,6 +898,201 @@ __global__ void __launch_bounds__(128) k_face_flux(K c, int d) {
   for (int e = 0; e < 3; ++e) c.p(P_MAXWELL + 3 * d + e)[at] = area * mx.m[e];
 }
 
+// ---- fused face velocity + edge EMF, LDS tiles ------------------------------------
+//
+// For edge directions e = 0 (t1 = j, t2 = k) and e = 1 (t1 = k, t2 = i) a
+// block owns a tile of E1 x E2 edges in the (t1, t2) plane at one position
+// along e, with the k direction 32 wide so that every LDS fill row is a
+// contiguous run. It stages in LDS the cells the tile's t1-face velocities
+// need (vol * m / rho summands and volumes, 4 cells beyond the tile on each
+// side), the t1-faces' areas and B.n along t2 and the t2-faces' along t1,
+// computes the t1-face velocities once per face (k_face_velocity's
+// arithmetic), and then runs k_edge_emf's arithmetic per edge from LDS. Every
+// value is computed by the same expressions in the same order as the two
+// unfused kernels, so the EMF is unchanged bit for bit; what changes is that
+// each cell and face value is read from HBM once per tile instead of once
+// per stencil use, and the face velocities of these two directions are
+// never written to HBM.
+template <int E>
+struct EmfTile {
+  static constexpr int T1 = (E + 1) % 3, T2 = (E + 2) % 3;
+  static constexpr int E1 = T1 == 2 ? 32 : 8;  // edges along t1
+  static constexpr int E2 = T2 == 2 ? 32 : 8;  // edges along t2
+  static constexpr int C1 = E1 + 7, C2 = E2 + 7;
+  // (position along e, along t1, along t2) -> (i, j, k)
+  GCS_DEV static void ijk(int ce, int a1, int a2, int &i, int &j, int &k) {
+    int x[3];
+    x[E] = ce;
+    x[T1] = a1;
+    x[T2] = a2;
+    i = x[0];
+    j = x[1];
+    k = x[2];
+  }
+};
+
+template <int E>
+__global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
+  using T = EmfTile<E>;
+  constexpr int E1 = T::E1, E2 = T::E2, C1 = T::C1, C2 = T::C2;
+  constexpr int t1 = T::T1, t2 = T::T2;
+  __shared__ double s_vol[C1 * C2], s_wv[3][C1 * C2];
+  __shared__ double s_fv[3][E1 * C2], s_a1[E1 * C2], s_b1[E1 * C2];
+  __shared__ double s_a2[C1 * E2], s_b2[C1 * E2];
+  const Layout &L = c.L;
+  const int act[3] = {L.nr, L.n, L.n};
+  // edge box: [g, g + act) along e, [g, g + act + 1) along t1 and t2
+  const int lo1 = L.ghost, hi1 = L.ghost + act[t1] + 1;
+  const int lo2 = L.ghost, hi2 = L.ghost + act[t2] + 1;
+  const int nt1 = (hi1 - lo1 + E1 - 1) / E1, nt2 = (hi2 - lo2 + E2 - 1) / E2;
+  long b = blockIdx.x;
+  const int tile2 = (int)(b % nt2);
+  b /= nt2;
+  const int tile1 = (int)(b % nt1);
+  b /= nt1;
+  const int ce = L.ghost + (int)(b % act[E]);
+  const int f = (int)(b / act[E]);
+  if (f >= 6) return;
+  const int o1 = lo1 + tile1 * E1, o2 = lo2 + tile2 * E2;  // tile origin
+  const int n1 = min(E1, hi1 - o1), n2 = min(E2, hi2 - o2);  // valid edges
+  const int tid = threadIdx.x;
+  auto at_of = [&](int a1, int a2) {
+    int i, j, k;
+    T::ijk(ce, a1, a2, i, j, k);
+    return L.at(f, i, j, k);
+  };
+  // cells: t1 in [o1 - 4, o1 + n1 + 3), t2 in [o2 - 4, o2 + n2 + 3)
+  {
+    const int w1 = n1 + 7, w2 = n2 + 7;
+    for (int q = tid; q < w1 * w2; q += 256) {
+      const int u1 = q / w2, u2 = q % w2;
+      const long at = at_of(o1 - 4 + u1, o2 - 4 + u2);
+      const int l = u1 * C2 + u2;
+      s_vol[l] = c.p(P_VOL)[at];
+      for (int m = 0; m < 3; ++m) s_wv[m][l] = c.p(P_WV + m)[at];
+    }
+  }
+  // t1-faces: t1 in [o1, o1 + n1), t2 in [o2 - 4, o2 + n2 + 3)
+  {
+    const int w2 = n2 + 7;
+    const double *area = c.p(P_FACE + FACE_PLANES * t1 + FO_AREA), *bn = c.p(P_BN + t1);
+    for (int q = tid; q < n1 * w2; q += 256) {
+      const int u1 = q / w2, u2 = q % w2;
+      const long at = at_of(o1 + u1, o2 - 4 + u2);
+      s_a1[u1 * C2 + u2] = area[at];
+      s_b1[u1 * C2 + u2] = bn[at];
+    }
+  }
+  // t2-faces: t1 in [o1 - 4, o1 + n1 + 3), t2 in [o2, o2 + n2)
+  {
+    const int w1 = n1 + 7;
+    const double *area = c.p(P_FACE + FACE_PLANES * t2 + FO_AREA), *bn = c.p(P_BN + t2);
+    for (int q = tid; q < w1 * n2; q += 256) {
+      const int u1 = q / n2, u2 = q % n2;
+      const long at = at_of(o1 - 4 + u1, o2 + u2);
+      s_a2[u1 * E2 + u2] = area[at];
+      s_b2[u1 * E2 + u2] = bn[at];
+    }
+  }
+  __syncthreads();
+  // t1-face velocities (k_face_velocity, direction t1)
+  {
+    const int w2 = n2 + 7;
+    for (int q = tid; q < n1 * w2; q += 256) {
+      const int u1 = q / w2, u2 = q % w2;
+      double vol[8], wv[3][8];
+      for (int n = 0; n < 8; ++n) {
+        const int l = (u1 + n) * C2 + u2;  // cell t1 = o1 + u1 + n - 4
+        vol[n] = s_vol[l];
+        for (int m = 0; m < 3; ++m) wv[m][n] = s_wv[m][l];
+      }
+      const double iv = central8(vol);
+      const bool ok = isfinite(iv) && fabs(iv) > DBL_MIN;
+      for (int m = 0; m < 3; ++m) s_fv[m][u1 * C2 + u2] = ok ? central8(wv[m]) / iv : NAN;
+    }
+  }
+  __syncthreads();
+  // one edge per thread (k fastest across the wave)
+  int e1, e2;
+  if (t2 == 2) {
+    e2 = tid % E2;
+    e1 = tid / E2;
+  } else {
+    e1 = tid % E1;
+    e2 = tid / E1;
+  }
+  if (e1 >= n1 || e2 >= n2) return;
+  const long at = at_of(o1 + e1, o2 + e2);
+  if (!c.b(Q_VALID + E)[at] || !c.b(Q_EBOK + E)[at]) {
+    c.fail(51, at);
+    return;
+  }
+  const StepOptions &o = c.o;
+  const int eb = P_EDGE + EDGE_PLANES * E;
+  double a1[8], v[3];
+  {
+    double wv[3][8];
+    for (int n = 0; n < 8; ++n) {
+      const int l = e1 * C2 + e2 + n;  // t1-face at t2 = o2 + e2 + n - 4
+      a1[n] = s_a1[l];
+      for (int m = 0; m < 3; ++m) wv[m][n] = a1[n] * s_fv[m][l];
+    }
+    const double ea = central8(a1);
+    if (!isfinite(ea) || fabs(ea) <= DBL_MIN) {
+      c.fail(52, at);
+      return;
+    }
+    for (int m = 0; m < 3; ++m) v[m] = central8(wv[m]) / ea;
+  }
+  double et1[3], et2[3];
+  for (int m = 0; m < 3; ++m) {
+    et1[m] = c.p(eb + EO_T1 + m)[at];
+    et2[m] = c.p(eb + EO_T2 + m)[at];
+  }
+  const double v1 = dot3(v, et1), v2 = dot3(v, et2);
+  double fl, fr, sl, sr;
+  {
+    double fld[8];
+    for (int n = 0; n < 8; ++n) fld[n] = s_b1[e1 * C2 + e2 + n];
+    up7_pdm(a1, central8(a1), fld, o.emf_pdm, fl, fr);
+  }
+  {
+    double a2[8], fld[8];
+    for (int n = 0; n < 8; ++n) {
+      const int l = (e1 + n) * E2 + e2;  // t2-face at t1 = o1 + e1 + n - 4
+      a2[n] = s_a2[l];
+      fld[n] = s_b2[l];
+    }
+    const double ia = central8(a2);
+    if (!isfinite(ia) || fabs(ia) <= DBL_MIN) {
+      c.fail(53, at);
+      return;
+    }
+    up7_pdm(a2, ia, fld, o.emf_pdm, sl, sr);
+  }
+  double solve[5], bt[2];
+  for (int m = 0; m < 5; ++m) solve[m] = c.p(eb + EO_SOLVE + m)[at];
+  edge_field(solve, 0.5 * (fl + fr), 0.5 * (sl + sr), bt);
+  bt[0] += c.p(P_B0ET + 2 * E)[at];
+  bt[1] += c.p(P_B0ET + 2 * E + 1)[at];
+  const double jump = (sr - sl) - (fr - fl);
+  const long step[3] = {L.di(), (long)L.SK, 1};
+  double rho = 0.0;
+  for (int a = 0; a < 2; ++a)
+    for (int bb = 0; bb < 2; ++bb) rho += 0.25 * c.p(P_UPRED)[at - a * step[t1] - bb * step[t2]];
+  if (!isfinite(rho) || rho <= DBL_MIN) {
+    c.fail(54, at);
+    return;
+  }
+  double emf;
+  if (!edge_emf(v1, v2, bt[0], bt[1], jump, rho, c.p(eb + EO_LEN)[at], o.emf_diff,
+                o.emf_boris, o.emf_c, o.emf_cfl, c.sc->dt, emf)) {
+    c.fail(55, at);
+    return;
+  }
+  c.p(P_EMF + E)[at] = emf;
+}
+
 // nonorthogonal_step.c:35-71 on the inner wall face.
 __global__ void k_inner_trap(K c) {
   const Layout &L = c.L;
@@ -1640,10 +1835,12 @@ int enqueue_phase(Problem &P, Phase phase) {
   } else if (phase == PHASE_EMF) {
     // cubedsphere_step.c:99-126 up to the seam EMF average
     mark_stage(P, S_FACE_VELOCITY);
-    for (int d = 0; d < 3; ++d) {
+    const int act[3] = {L.nr, L.n, L.n};
+    // FP64: the i-face velocities for the k-edges only; the j- and i-edges
+    // compute their t1-face velocities inside their LDS tiles.
+    for (int d = 0; d < (P.mixed ? 3 : 1); ++d) {
       long n = 6;
       const int ext[3] = {L.NI, L.NJ, L.NK};
-      const int act[3] = {L.nr, L.n, L.n};
       for (int a = 0; a < 3; ++a)
         n *= a == d ? act[a] + 1 : std::min(L.ghost + act[a] + 4, ext[a]) - (L.ghost - 4);
       if (P.mixed)
@@ -1654,10 +1851,21 @@ int enqueue_phase(Problem &P, Phase phase) {
     for (int e = 0; e < 3; ++e) {
       const long n = 6L * (L.nr + (e != 0)) * (L.n + (e != 1)) * (L.n + (e != 2));
       mark_stage(P, S_EDGE_EMF0 + e);
-      if (P.mixed)
+      if (P.mixed) {
         k_edge_emf_fp32<<<nblocks(n, 128), 128, 0, s>>>(c, e);
-      else
+      } else if (e == 2) {
         k_edge_emf<<<nblocks(n, 128), 128, 0, s>>>(c, e);
+      } else {
+        const int t1 = (e + 1) % 3, t2 = (e + 2) % 3;
+        const int e1 = e == 0 ? EmfTile<0>::E1 : EmfTile<1>::E1;
+        const int e2 = e == 0 ? EmfTile<0>::E2 : EmfTile<1>::E2;
+        const long tiles = 6L * act[e] * ((act[t1] + 1 + e1 - 1) / e1) *
+                           ((act[t2] + 1 + e2 - 1) / e2);
+        if (e == 0)
+          k_edge_emf_tiled<0><<<(unsigned)tiles, 256, 0, s>>>(c);
+        else
+          k_edge_emf_tiled<1><<<(unsigned)tiles, 256, 0, s>>>(c);
+      }
     }
     if (bd.earth_emf) {
       mark_stage(P, S_EARTH_EMF);

@@ -11,10 +11,18 @@
  * pattern to its potential. The Krylov solve that follows is unchanged and
  * runs to the same tolerance, so the potential it returns is the solver's.
  *
- * Network input features, per point of the colatitude x azimuth grid:
- * current / its RMS, SigmaP / 10, SigmaH / 10, colatitude index / (nt - 1),
- * hemisphere sign (-1 north, +1 south), and 11 zero channels. Output: the
- * potential / (R^2 RMS / SigmaP), divided by the network's output scale.
+ * The network may run on the polar grid pooled f x f (its grid is
+ * colatitude_points / f x azimuth_points / f): the Krylov iterations remove
+ * short-wavelength error quickly, the network supplies the long-wavelength
+ * part, and on a coarse grid a 4-level U-Net's receptive field spans the
+ * polar cap. Its answer is interpolated back bilinearly (rows clamped,
+ * longitude periodic).
+ *
+ * Network input features, per point of its grid: the f x f mean of the
+ * current / the current's RMS, SigmaP / 10, SigmaH / 10, colatitude /
+ * maximum colatitude, hemisphere sign (-1 north, +1 south), and 11 zero
+ * channels. Output: the potential (f x f mean) / (R^2 RMS / SigmaP),
+ * divided by the network's output scale.
  *
  * Every function returns 0 on success and -1 on failure (gcs_last_error()).
  */
@@ -78,7 +86,8 @@ int gcs_mfma_gemm_check(const uint16_t *a, const uint16_t *b, int m, int n, int 
                         float *c);
 
 /* Attach (or, with NULL, detach) the network to the problem's MI solve. The
- * network grid must be the MI grid and its batch at least 2. */
+ * network grid must be the MI grid pooled by one factor f in both
+ * directions (f = 1: the MI grid itself) and its batch at least 2. */
 int gcs_problem_mi_attach_warm_start(gcs_problem *problem, gcs_unet *net);
 /* Train on pairs for the problem's MI grid; loss_history (steps entries, may
  * be NULL) receives each step's loss. */
