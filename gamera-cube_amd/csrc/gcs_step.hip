@@ -961,6 +961,28 @@ __global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
     T::ijk(ce, a1, a2, i, j, k);
     return L.at(f, i, j, k);
   };
+  // This thread's edge (k fastest across the wave). Its own global data is
+  // read first so that the loads overlap the tile fill; threads beyond a
+  // partial tile read a valid edge's and leave after the tile phases.
+  int e1, e2;
+  if (t2 == 2) {
+    e2 = tid % E2;
+    e1 = tid / E2;
+  } else {
+    e1 = tid % E1;
+    e2 = tid / E1;
+  }
+  const long at = at_of(o1 + min(e1, n1 - 1), o2 + min(e2, n2 - 1));
+  const int ebp = P_EDGE + EDGE_PLANES * E;
+  const bool edge_ok = c.b(Q_VALID + E)[at] && c.b(Q_EBOK + E)[at];
+  double et1[3], et2[3], solve[5];
+  for (int m = 0; m < 3; ++m) {
+    et1[m] = c.p(ebp + EO_T1 + m)[at];
+    et2[m] = c.p(ebp + EO_T2 + m)[at];
+  }
+  for (int m = 0; m < 5; ++m) solve[m] = c.p(ebp + EO_SOLVE + m)[at];
+  const double b0t0 = c.p(P_B0ET + 2 * E)[at], b0t1 = c.p(P_B0ET + 2 * E + 1)[at];
+  const double elen = c.p(ebp + EO_LEN)[at];
   // cells: t1 in [o1 - 4, o1 + n1 + 3), t2 in [o2 - 4, o2 + n2 + 3)
   {
     const int w1 = n1 + 7, w2 = n2 + 7;
@@ -1012,23 +1034,13 @@ __global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
     }
   }
   __syncthreads();
-  // one edge per thread (k fastest across the wave)
-  int e1, e2;
-  if (t2 == 2) {
-    e2 = tid % E2;
-    e1 = tid / E2;
-  } else {
-    e1 = tid % E1;
-    e2 = tid / E1;
-  }
-  if (e1 >= n1 || e2 >= n2) return;
-  const long at = at_of(o1 + e1, o2 + e2);
-  if (!c.b(Q_VALID + E)[at] || !c.b(Q_EBOK + E)[at]) {
+  const StepOptions &o = c.o;
+  const bool mine = e1 < n1 && e2 < n2;
+  if (!mine) return;
+  if (!edge_ok) {
     c.fail(51, at);
     return;
   }
-  const StepOptions &o = c.o;
-  const int eb = P_EDGE + EDGE_PLANES * E;
   double a1[8], v[3];
   {
     double wv[3][8];
@@ -1043,11 +1055,6 @@ __global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
       return;
     }
     for (int m = 0; m < 3; ++m) v[m] = central8(wv[m]) / ea;
-  }
-  double et1[3], et2[3];
-  for (int m = 0; m < 3; ++m) {
-    et1[m] = c.p(eb + EO_T1 + m)[at];
-    et2[m] = c.p(eb + EO_T2 + m)[at];
   }
   const double v1 = dot3(v, et1), v2 = dot3(v, et2);
   double fl, fr, sl, sr;
@@ -1070,11 +1077,10 @@ __global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
     }
     up7_pdm(a2, ia, fld, o.emf_pdm, sl, sr);
   }
-  double solve[5], bt[2];
-  for (int m = 0; m < 5; ++m) solve[m] = c.p(eb + EO_SOLVE + m)[at];
+  double bt[2];
   edge_field(solve, 0.5 * (fl + fr), 0.5 * (sl + sr), bt);
-  bt[0] += c.p(P_B0ET + 2 * E)[at];
-  bt[1] += c.p(P_B0ET + 2 * E + 1)[at];
+  bt[0] += b0t0;
+  bt[1] += b0t1;
   const double jump = (sr - sl) - (fr - fl);
   const long step[3] = {L.di(), (long)L.SK, 1};
   double rho = 0.0;
@@ -1085,7 +1091,7 @@ __global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
     return;
   }
   double emf;
-  if (!edge_emf(v1, v2, bt[0], bt[1], jump, rho, c.p(eb + EO_LEN)[at], o.emf_diff,
+  if (!edge_emf(v1, v2, bt[0], bt[1], jump, rho, elen, o.emf_diff,
                 o.emf_boris, o.emf_c, o.emf_cfl, c.sc->dt, emf)) {
     c.fail(55, at);
     return;
