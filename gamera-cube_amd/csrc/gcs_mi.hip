@@ -20,6 +20,7 @@
 // iterates therefore differ from the reference's by rounding, and the
 // converged potential agrees with it to the solve's tolerance. Both
 // hemispheres run in the same launches (blockIdx.y).
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -80,6 +81,7 @@ struct MiState {
   // potential plus the network's answer for the current residual
   unet::Net *net = nullptr;
   double *warm_scale = nullptr;  // [2] RMS of the residual's current equivalent
+  double *coop_slots = nullptr;  // partial sums of the cooperative solve
 };
 
 namespace {
@@ -915,6 +917,157 @@ __global__ void __launch_bounds__(kSolveThreads) k_mi_bicgstab(MiK c, double rel
   store_result(c, h, res, first);
 }
 
+// The BiCGStab solve of one hemisphere spread over kCoopBlocks workgroups of a
+// cooperative launch: the same iterations as k_mi_bicgstab, with grid-wide
+// barriers where one workgroup's barrier sufficed there. Dot products sum
+// the workgroups' partial sums in workgroup order from two alternating slot
+// banks (a bank is rewritten only after every workgroup passed the barrier
+// that follows its last read). Every workgroup evaluates the same sums in
+// the same order, so control flow is uniform over the grid and every
+// workgroup reaches every barrier.
+constexpr int kCoopBlocks = 16;
+
+__global__ void __launch_bounds__(kSolveThreads) k_mi_bicgstab_coop(MiK c, int h, double rel_tol,
+                                                                    int maxit, double *slots) {
+  namespace cg = cooperative_groups;
+  cg::grid_group grid = cg::this_grid();
+  __shared__ double red[kSolveThreads / 64];
+  const long n = c.n;
+  const long t0 = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long st = (long)gridDim.x * blockDim.x;
+  const bool pole_wave = blockIdx.x == 0 && threadIdx.x < 64;
+  int bank = 0;
+  auto gsum = [&](double v) -> double {
+    const double b = block_sum(v, red);
+    double *sl = slots + bank * kCoopBlocks;
+    if (threadIdx.x == 0) sl[blockIdx.x] = b;
+    bank ^= 1;
+    grid.sync();
+    double s = 0.0;
+    for (int i = 0; i < (int)gridDim.x; ++i) s += sl[i];
+    return s;
+  };
+  auto precondition = [&](const double *in, double *out) {
+    grid.sync();  // `in` complete
+    if (t0 == 0) out[0] = in[0] / c.vec(h, V_DIAG)[0];
+    const int lane = threadIdx.x & 63;
+    for (long ring = t0 >> 6; ring < c.nt - 2; ring += st >> 6)
+      ring_apply(c, h, in, out, (int)ring, lane);
+    grid.sync();  // `out` complete
+  };
+  const double dt = c.max_colat / (double)(c.nt - 1);
+  const double half = 0.5 * dt;
+  const double pe = sin(half) * c.pedersen / (incl2(half) * dt * dt);
+  const double ps = (double)c.np * pe;
+  double *x = c.vec(h, V_X), *r = c.vec(h, V_R), *p = c.vec(h, V_P), *ap = c.vec(h, V_AP),
+         *sh = c.vec(h, V_SH), *pp = c.vec(h, V_PP), *s = c.vec(h, V_S), *psv = c.vec(h, V_PS),
+         *as = c.vec(h, V_AS);
+  const double *rhs = c.vec(h, V_RHS);
+  for (long ring = t0; ring < c.nt - 2; ring += st) ring_factor(c, h, (int)ring);
+  double a = 0.0, b = 0.0;
+  {
+    const double pole = pole_wave ? pole_row(c, h, true, x, pe) : 0.0;
+    for (long q = t0; q < n; q += st) {
+      r[q] = rhs[q] - (q == 0 ? pole : apply_at(c, h, true, x, q, nullptr, pe));
+      sh[q] = r[q];
+      p[q] = 0.0;
+      ap[q] = 0.0;
+      a += orig_term(c, r[q], q, ps, dt);
+      b += q == 0 ? 0.0 : orig_term(c, rhs[q], q, ps, dt);
+    }
+  }
+  double norm = sqrt(fmax(0.0, gsum(a)));
+  const double first = norm;
+  const double target = rel_tol * fmax(sqrt(fmax(0.0, gsum(b))), 1.0);
+  SolveResult res{0, norm <= target, norm};
+  double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+  for (int it = 1; it <= maxit && !res.ok; ++it) {
+    a = 0.0;
+    for (long q = t0; q < n; q += st) a += sh[q] * r[q];
+    const double rho = gsum(a);
+    if (!isfinite(rho) || fabs(rho) <= DBL_MIN || !isfinite(omega) || fabs(omega) <= DBL_MIN)
+      break;
+    const double beta = (rho / rho_prev) * (alpha / omega);
+    for (long q = t0; q < n; q += st) p[q] = r[q] + beta * (p[q] - omega * ap[q]);
+    precondition(p, pp);
+    a = 0.0;
+    {
+      const double pole = pole_wave ? pole_row(c, h, true, pp, pe) : 0.0;
+      for (long q = t0; q < n; q += st) {
+        ap[q] = q == 0 ? pole : apply_at(c, h, true, pp, q, nullptr, pe);
+        a += sh[q] * ap[q];
+      }
+    }
+    const double den = gsum(a);
+    alpha = rho / den;
+    a = 0.0;
+    for (long q = t0; q < n; q += st) {
+      s[q] = r[q] - alpha * ap[q];
+      a += orig_term(c, s[q], q, ps, dt);
+    }
+    const double snorm = sqrt(fmax(0.0, gsum(a)));
+    if (!isfinite(den) || fabs(den) <= DBL_MIN) break;
+    if (snorm <= target) {
+      res.last = snorm;
+      res.iters = it;
+      res.ok = true;
+      for (long q = t0; q < n; q += st) x[q] += alpha * pp[q];
+      break;
+    }
+    precondition(s, psv);
+    a = 0.0;
+    b = 0.0;
+    {
+      const double pole = pole_wave ? pole_row(c, h, true, psv, pe) : 0.0;
+      for (long q = t0; q < n; q += st) {
+        as[q] = q == 0 ? pole : apply_at(c, h, true, psv, q, nullptr, pe);
+        a += as[q] * as[q];
+        b += as[q] * s[q];
+      }
+    }
+    const double wden = gsum(a);
+    const double wnum = gsum(b);
+    omega = wnum / wden;
+    if (!isfinite(wden) || wden <= DBL_MIN || !isfinite(omega) || fabs(omega) <= DBL_MIN)
+      break;
+    a = 0.0;
+    for (long q = t0; q < n; q += st) {
+      x[q] += alpha * pp[q] + omega * psv[q];
+      r[q] = s[q] - omega * as[q];
+      a += orig_term(c, r[q], q, ps, dt);
+    }
+    norm = sqrt(fmax(0.0, gsum(a)));
+    res.iters = it;
+    res.last = norm;
+    if (norm <= target || it % 256 == 0) {
+      // x complete: the sum above ended with a grid barrier
+      a = 0.0;
+      {
+        const double pole = pole_wave ? pole_row(c, h, true, x, pe) : 0.0;
+        for (long q = t0; q < n; q += st) {
+          r[q] = rhs[q] - (q == 0 ? pole : apply_at(c, h, true, x, q, nullptr, pe));
+          sh[q] = r[q];
+          a += orig_term(c, r[q], q, ps, dt);
+        }
+      }
+      norm = sqrt(fmax(0.0, gsum(a)));
+      res.last = norm;
+      if (norm <= target) {
+        res.ok = true;
+        break;
+      }
+      for (long q = t0; q < n; q += st) {
+        p[q] = 0.0;
+        ap[q] = 0.0;
+      }
+      rho_prev = alpha = omega = 1.0;
+      continue;
+    }
+    rho_prev = rho;
+  }
+  if (blockIdx.x == 0) store_result(c, h, res, first);
+}
+
 // ---- learned warm start ----------------------------------------------------------
 
 // Current density equivalent to a residual row: the right-hand side of
@@ -1072,9 +1225,28 @@ int solve(Problem &P, bool tensor) {
   if (m.net != nullptr && warm_start(P, c, tensor) != 0) return -1;
   const double rel = m.cfg.relative_tolerance;
   const int maxit = m.cfg.maximum_iterations;
-  if (tensor)
-    k_mi_bicgstab<<<2, kSolveThreads, 0, s>>>(c, rel, maxit);
-  else
+  if (tensor) {
+    // cooperative form (one launch per hemisphere) when the device can hold
+    // its workgroups at once; the single-workgroup kernel otherwise
+    bool coop = m.coop_slots != nullptr;
+    for (int h = 0; h < 2 && coop; ++h) {
+      int hh = h;
+      double relv = rel;
+      int maxv = maxit;
+      double *slots = m.coop_slots;
+      void *args[] = {&c, &hh, &relv, &maxv, &slots};
+      if (hipLaunchCooperativeKernel((const void *)k_mi_bicgstab_coop, dim3(kCoopBlocks),
+                                     dim3(kSolveThreads), args, 0, s) != hipSuccess) {
+        (void)hipGetLastError();
+        coop = false;
+        if (h == 1) {
+          set_error("cooperative launch failed after the first hemisphere");
+          return -1;
+        }
+      }
+    }
+    if (!coop) k_mi_bicgstab<<<2, kSolveThreads, 0, s>>>(c, rel, maxit);
+  } else
     k_mi_cg<<<2, kSolveThreads, 4 * (size_t)m.nt * sizeof(double), s>>>(c, rel, maxit);
   k_mi_vec<<<dim3(blocks_of((long)m.np * m.nt), 2), kT, 0, s>>>(c, OP_EXPAND, 0.0, m.active);
   GCS_HIP(hipGetLastError());
@@ -1246,6 +1418,18 @@ int mi_create(Problem &P) {
   GCS_HIP(hipMalloc(&m.fac, polar * sizeof(double)));
   GCS_HIP(hipMalloc(&m.potential, polar * sizeof(double)));
   GCS_HIP(hipMalloc(&m.work, 2L * kVectors * m.n * sizeof(double)));
+  {
+    // the cooperative BiCGStab needs all kCoopBlocks workgroups resident
+    int dev = 0, coop = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mi_bicgstab_coop, kSolveThreads,
+                                                     0) == hipSuccess &&
+        coop != 0 && per_cu * cus >= kCoopBlocks)
+      GCS_HIP(hipMalloc(&m.coop_slots, 2 * kCoopBlocks * sizeof(double)));
+    (void)hipGetLastError();
+  }
   GCS_HIP(hipMalloc(&m.scal, 2 * SL_COUNT * sizeof(double)));
   GCS_HIP(hipHostMalloc(&m.host_scal, 2 * SL_COUNT * sizeof(double)));
   GCS_HIP(hipMalloc(&m.active, 2 * sizeof(int)));
@@ -1285,6 +1469,7 @@ void mi_destroy(Problem &P) {
   (void)hipFree(m->active);
   if (m->host_scal) (void)hipHostFree(m->host_scal);
   (void)hipFree(m->warm_scale);
+  (void)hipFree(m->coop_slots);
   delete m;
   P.mi = nullptr;
 }
