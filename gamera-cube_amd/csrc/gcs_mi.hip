@@ -80,7 +80,7 @@ struct MiState {
   // learned warm start (not owned): the initial guess becomes the previous
   // potential plus the network's answer for the current residual
   unet::Net *net = nullptr;
-  double *warm_scale = nullptr;  // [2] RMS of the residual's current equivalent
+  double *warm_scale = nullptr;  // [2] residual-current RMS, [2] step lengths
   double *coop_slots = nullptr;  // partial sums of the cooperative solve
 };
 
@@ -1080,16 +1080,20 @@ GCS_DEV double equivalent_current(double r, int i, int nt, double dt, double r2,
   return -(r / sin(th)) / (r2 * cos_incl(th, hemi));
 }
 
-// Network features of one point (16 floats): current / scale, SigmaP / 10,
-// SigmaH / 10, colatitude / maximum colatitude, hemisphere sign, zeros.
-GCS_DEV void write_features(float *f, double current, double scale, double sp, double sh,
-                            double colat_fraction, int hemi) {
-  f[0] = scale > 0.0 ? (float)(current / scale) : 0.f;
-  f[1] = (float)(sp / 10.0);
-  f[2] = (float)(sh / 10.0);
-  f[3] = (float)colat_fraction;
-  f[4] = (float)hemi;
-  for (int k = 5; k < unet::kInChannels; ++k) f[k] = 0.f;
+// Network features of one point of its grid (kInChannels floats): the f x f
+// fine currents of the point's block / scale in channels 0 .. f*f - 1 (row
+// major; f <= 4), SigmaP / 10, SigmaH / 10, colatitude / maximum
+// colatitude and hemisphere sign in channels 16 .. 19, zeros elsewhere.
+constexpr int kFeatCurrent = 16, kFeatSp = 16, kFeatSh = 17, kFeatColat = 18, kFeatHemi = 19;
+
+GCS_DEV void write_features(float *f, const double *current, int f2, double scale, double sp,
+                            double sh, double colat_fraction, int hemi) {
+  for (int k = 0; k < unet::kInChannels; ++k) f[k] = 0.f;
+  for (int k = 0; k < f2; ++k) f[k] = scale > 0.0 ? (float)(current[k] / scale) : 0.f;
+  f[kFeatSp] = (float)(sp / 10.0);
+  f[kFeatSh] = (float)(sh / 10.0);
+  f[kFeatColat] = (float)colat_fraction;
+  f[kFeatHemi] = (float)hemi;
 }
 
 // The network runs on the polar grid pooled by f x f (f = nt / H): the
@@ -1142,8 +1146,8 @@ __global__ void k_warm_scale(MiK c, double *scale) {
   if (threadIdx.x == 0) scale[h] = sqrt(red[0] / (double)count);
 }
 
-// Features on the coarse grid: the residual's equivalent current averaged
-// over each f x f block.
+// Features on the coarse grid: the residual's equivalent current of each
+// point of the f x f block, one channel each (space to depth).
 __global__ void k_warm_features(MiK c, const double *scale, double hall, int f, float *feat) {
   const int h = blockIdx.y;
   const int Hc = c.nt / f, Wc = c.np / f;
@@ -1151,44 +1155,85 @@ __global__ void k_warm_features(MiK c, const double *scale, double hall, int f, 
   if (q >= (long)Hc * Wc) return;
   const int rc = (int)(q / Wc), cc = (int)(q % Wc);
   const double dt = c.max_colat / (double)(c.nt - 1);
-  double sum = 0.0;
+  double cur[kFeatCurrent];
   for (int a = 0; a < f; ++a)
     for (int b = 0; b < f; ++b) {
       const int i = rc * f + a, j = cc * f + b;
       const double r = (i > 0 && i < c.nt - 1) ? c.vec(h, V_R)[red_at(c.np, i, j)] : 0.0;
-      sum += equivalent_current(r, i, c.nt, dt, c.r2, hemi_sign(h));
+      cur[a * f + b] = equivalent_current(r, i, c.nt, dt, c.r2, hemi_sign(h));
     }
-  write_features(feat + ((long)h * Hc * Wc + q) * unet::kInChannels, sum / (double)(f * f),
-                 scale[h], c.pedersen, hall, coarse_row_fraction(rc, f, c.nt), hemi_sign(h));
+  write_features(feat + ((long)h * Hc * Wc + q) * unet::kInChannels, cur, f * f, scale[h],
+                 c.pedersen, hall, coarse_row_fraction(rc, f, c.nt), hemi_sign(h));
 }
 
-// x += the network's potential correction, interpolated from the coarse
+// The network's potential correction d in V_Z, interpolated from the coarse
 // grid (out * out_scale * R^2 * scale / SigmaP; the pole unknown takes the
-// mean over row 0).
-__global__ void k_warm_correct(MiK c, const double *scale, const float *out, double out_scale,
-                               int f) {
+// mean over row 0); zero where the residual was zero.
+__global__ void k_warm_correction(MiK c, const double *scale, const float *out,
+                                  double out_scale, int f) {
   const int h = blockIdx.y;
   const long q = gtid();
   if (q >= c.n) return;
+  double *d = c.vec(h, V_Z);
   const double s = scale[h];
-  if (!(s > 0.0) || !isfinite(s)) return;
+  if (!(s > 0.0) || !isfinite(s)) {
+    d[q] = 0.0;
+    return;
+  }
   const double k = out_scale * c.r2 * s / c.pedersen;
   const int Hc = c.nt / f, Wc = c.np / f;
   const float *o = out + (long)h * Hc * Wc;
-  double *x = c.vec(h, V_X);
   if (q == 0) {
     double mean = 0.0;
     for (int j = 0; j < c.np; ++j) mean += coarse_sample(o, Hc, Wc, f, 0, j);
-    x[0] += k * (mean / (double)c.np);
+    d[0] = k * (mean / (double)c.np);
     return;
   }
   const int i = 1 + (int)((q - 1) / c.np), j = (int)((q - 1) % c.np);
-  x[q] += k * coarse_sample(o, Hc, Wc, f, i, j);
+  d[q] = k * coarse_sample(o, Hc, Wc, f, i, j);
 }
 
-// The guess in V_X (previous potential) becomes guess + N(residual): one
-// operator product, the network's forward pass on both hemispheres, one
-// update. The solve that follows recomputes the residual of the new guess.
+// Minimal-residual step along the correction: alpha = (r . A d) / (A d . A d)
+// (r the residual of the previous potential, A d in V_AS), so that the new
+// guess's residual r - alpha A d is never larger than r; one block per
+// hemisphere. alpha = 0 when A d vanishes or is not finite.
+__global__ void k_warm_alpha(MiK c, double *alpha) {
+  __shared__ double rn[kT], rd[kT];
+  const int h = blockIdx.y;
+  const double *r = c.vec(h, V_R), *ad = c.vec(h, V_AS);
+  double num = 0.0, den = 0.0;
+  for (long q = threadIdx.x; q < c.n; q += kT) {
+    num += r[q] * ad[q];
+    den += ad[q] * ad[q];
+  }
+  rn[threadIdx.x] = num;
+  rd[threadIdx.x] = den;
+  __syncthreads();
+  for (int w = kT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      rn[threadIdx.x] += rn[threadIdx.x + w];
+      rd[threadIdx.x] += rd[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double a = rn[0] / rd[0];
+    alpha[h] = (rd[0] > 0.0 && isfinite(a)) ? a : 0.0;
+  }
+}
+
+__global__ void k_warm_update(MiK c, const double *alpha) {
+  const int h = blockIdx.y;
+  const long q = gtid();
+  if (q >= c.n) return;
+  c.vec(h, V_X)[q] += alpha[h] * c.vec(h, V_Z)[q];
+}
+
+// The guess in V_X (previous potential) becomes guess + alpha N(residual):
+// the residual, the network's forward pass on both hemispheres, the
+// correction d, and the minimal-residual step alpha along it (so the guess
+// never gets worse in the residual norm). The solve that follows
+// recomputes the residual of the new guess.
 int warm_start(Problem &P, const MiK &c, bool tensor) {
   MiState &m = *P.mi;
   unet::Net &n = *m.net;
@@ -1205,7 +1250,10 @@ int warm_start(Problem &P, const MiK &c, bool tensor) {
   const int rc = unet::forward_device(n, 2);
   n.stream = keep;
   if (rc != 0) return -1;
-  k_warm_correct<<<g2, kT, 0, s>>>(c, m.warm_scale, n.out, n.out_scale, f);
+  k_warm_correction<<<g2, kT, 0, s>>>(c, m.warm_scale, n.out, n.out_scale, f);
+  k_mi_apply<<<g2, kT, 0, s>>>(c, tensor ? 1 : 0, V_Z, V_AS, m.active);
+  k_warm_alpha<<<dim3(1, 2), kT, 0, s>>>(c, m.warm_scale + 2);
+  k_warm_update<<<g2, kT, 0, s>>>(c, m.warm_scale + 2);
   GCS_HIP(hipGetLastError());
   return 0;
 }
@@ -1639,8 +1687,8 @@ __global__ void k_gen_scale(int nt, int np, const double *fac, double *scale) {
   if (threadIdx.x == 0) scale[blockIdx.x] = sqrt(red[0] / (double)count);
 }
 
-// Features and targets on the coarse grid (f x f block means of the fine
-// current and potential).
+// Features and targets on the coarse grid: the fine currents of each f x f
+// block as channels, the block mean of the potential as the target.
 __global__ void k_gen_features(const GenSample *smp, int nt, int np, int f, double r2,
                                const double *fac, const double *phi, const double *scale,
                                double out_scale, float *feat, float *target) {
@@ -1651,17 +1699,16 @@ __global__ void k_gen_features(const GenSample *smp, int nt, int np, int f, doub
   const int rc = (int)(q / Wc), cc = (int)(q % Wc);
   const GenSample &g = smp[b];
   const long base = (long)b * nt * np;
-  double cur = 0.0, pot = 0.0;
+  double cur[kFeatCurrent], pot = 0.0;
   for (int a = 0; a < f; ++a)
     for (int e = 0; e < f; ++e) {
       const long at = base + (long)(rc * f + a) * np + cc * f + e;
-      cur += fac[at];
+      cur[a * f + e] = fac[at];
       pot += phi[at];
     }
-  cur /= (double)(f * f);
   pot /= (double)(f * f);
   const long out = (long)b * Hc * Wc + q;
-  write_features(feat + out * unet::kInChannels, cur, scale[b], g.sp, g.sh,
+  write_features(feat + out * unet::kInChannels, cur, f * f, scale[b], g.sp, g.sh,
                  coarse_row_fraction(rc, f, nt), g.hemi);
   target[out] = (float)(pot * g.sp / (r2 * scale[b] * out_scale));
 }
@@ -1730,7 +1777,7 @@ static int make_pairs(PairBatch &pb, int nt, int np, double max_colat, double r2
 static int coarsening(const MiState &m, const unet::Net &n) {
   if (n.H <= 0 || n.W <= 0 || m.nt % n.H != 0 || m.np % n.W != 0) return 0;
   const int f = m.nt / n.H;
-  return m.np / n.W == f ? f : 0;
+  return m.np / n.W == f && f * f <= kFeatCurrent ? f : 0;
 }
 
 int mi_attach_warm_start(Problem &P, unet::Net *net) {
@@ -1744,7 +1791,7 @@ int mi_attach_warm_start(Problem &P, unet::Net *net) {
               "by one factor in both directions, and its batch at least 2");
     return -1;
   }
-  if (m.warm_scale == nullptr) GCS_HIP(hipMalloc(&m.warm_scale, 2 * sizeof(double)));
+  if (m.warm_scale == nullptr) GCS_HIP(hipMalloc(&m.warm_scale, 4 * sizeof(double)));
   m.net = net;
   return 0;
 }

@@ -15,7 +15,7 @@
 namespace gcs {
 namespace unet {
 
-constexpr int kInChannels = 16;  // input features, zero-padded to 16
+constexpr int kInChannels = 32;  // input features, zero-padded to 32
 constexpr int kLayerCount = 14;  // 3x3 conv layers
 constexpr int kActs = kLayerCount + 1;
 constexpr int kHeadIn = 32;

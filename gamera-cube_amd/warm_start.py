@@ -11,7 +11,7 @@ import numpy as np
 
 from .native import GpuProblem, last_error, library
 
-IN_CHANNELS = 16
+IN_CHANNELS = 32
 
 _F32P = ctypes.POINTER(ctypes.c_float)
 _F64P = ctypes.POINTER(ctypes.c_double)

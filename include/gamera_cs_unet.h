@@ -18,11 +18,12 @@
  * polar cap. Its answer is interpolated back bilinearly (rows clamped,
  * longitude periodic).
  *
- * Network input features, per point of its grid: the f x f mean of the
- * current / the current's RMS, SigmaP / 10, SigmaH / 10, colatitude /
- * maximum colatitude, hemisphere sign (-1 north, +1 south), and 11 zero
- * channels. Output: the potential (f x f mean) / (R^2 RMS / SigmaP),
- * divided by the network's output scale.
+ * Network input features (32 channels), per point of its grid: the f x f
+ * fine-grid currents of the point's block / the current's RMS in channels
+ * 0 .. f*f - 1 (row major, f <= 4), SigmaP / 10, SigmaH / 10, colatitude /
+ * maximum colatitude and hemisphere sign (-1 north, +1 south) in channels
+ * 16 .. 19, zeros elsewhere. Output: the block mean of the potential /
+ * (R^2 RMS / SigmaP), divided by the network's output scale.
  *
  * Every function returns 0 on success and -1 on failure (gcs_last_error()).
  */
@@ -72,7 +73,7 @@ int gcs_unet_get_grads(gcs_unet *net, float *grads, size_t count);
 double gcs_unet_output_scale(const gcs_unet *net);
 int gcs_unet_set_output_scale(gcs_unet *net, double scale);
 
-/* Forward pass: features [batch][H][W][16] -> out [batch][H][W]. */
+/* Forward pass: features [batch][H][W][32] -> out [batch][H][W]. */
 int gcs_unet_forward(gcs_unet *net, const float *features, int batch, float *out);
 /* Forward, mean-square-error loss against target [batch][H][W], backward. */
 int gcs_unet_gradients(gcs_unet *net, const float *features, const float *target,

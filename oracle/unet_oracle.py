@@ -11,8 +11,8 @@ convolutions on small inputs.
 Network (a 4-level U-Net, 32 base channels), NHWC tensors, H = colatitude
 rows, W = longitude columns (periodic), N = samples:
 
-    x    [N,H,W,16]  input features (bf16)
-    e0a  = conv(x, 16->32);        e0 = conv(e0a, 32->32)
+    x    [N,H,W,32]  input features (bf16)
+    e0a  = conv(x, 32->32);        e0 = conv(e0a, 32->32)
     e1a  = conv(pool(e0), 32->64); e1 = conv(e1a, 64->64)
     e2a  = conv(pool(e1), 64->128); e2 = conv(e2a, 128->128)
     e3a  = conv(pool(e2), 128->256); e3 = conv(e3a, 256->256)
@@ -34,7 +34,7 @@ error of `out` against the target over all pixels.
 import numpy as np
 
 LEVELS = (32, 64, 128, 256)
-IN_CHANNELS = 16
+IN_CHANNELS = 32
 
 # (name, cin, cout, input): input is a tensor name or ("cat", coarse, skip)
 # or ("pool", name).
