@@ -1193,18 +1193,20 @@ __global__ void k_warm_correction(MiK c, const double *scale, const float *out,
   d[q] = k * coarse_sample(o, Hc, Wc, f, i, j);
 }
 
-// Minimal-residual step along the correction: alpha = (r . A d) / (A d . A d)
-// (r the residual of the previous potential, A d in V_AS), so that the new
-// guess's residual r - alpha A d is never larger than r; one block per
-// hemisphere. alpha = 0 when A d vanishes or is not finite.
+// Step length along the correction in the operator's energy norm:
+// alpha = (d . r) / (d . A d) (r the residual of the previous potential, A d
+// in V_AS) minimises the A-norm of the guess's error for the symmetric
+// Pedersen operator, and weighs the long-wavelength error the Krylov
+// iterations reduce slowest more than the residual norm does. One block per
+// hemisphere; alpha = 0 when d . A d is not positive or not finite.
 __global__ void k_warm_alpha(MiK c, double *alpha) {
   __shared__ double rn[kT], rd[kT];
   const int h = blockIdx.y;
-  const double *r = c.vec(h, V_R), *ad = c.vec(h, V_AS);
+  const double *r = c.vec(h, V_R), *d = c.vec(h, V_Z), *ad = c.vec(h, V_AS);
   double num = 0.0, den = 0.0;
   for (long q = threadIdx.x; q < c.n; q += kT) {
-    num += r[q] * ad[q];
-    den += ad[q] * ad[q];
+    num += d[q] * r[q];
+    den += d[q] * ad[q];
   }
   rn[threadIdx.x] = num;
   rd[threadIdx.x] = den;
@@ -1231,8 +1233,8 @@ __global__ void k_warm_update(MiK c, const double *alpha) {
 
 // The guess in V_X (previous potential) becomes guess + alpha N(residual):
 // the residual, the network's forward pass on both hemispheres, the
-// correction d, and the minimal-residual step alpha along it (so the guess
-// never gets worse in the residual norm). The solve that follows
+// correction d, and the energy-norm step alpha along it (which shrinks a
+// correction that does not point along the error). The solve that follows
 // recomputes the residual of the new guess.
 int warm_start(Problem &P, const MiK &c, bool tensor) {
   MiState &m = *P.mi;
