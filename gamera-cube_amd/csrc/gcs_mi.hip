@@ -82,6 +82,7 @@ struct MiState {
   unet::Net *net = nullptr;
   double *warm_scale = nullptr;  // [2] residual-current RMS, [2] step lengths
   double *coop_slots = nullptr;  // partial sums of the cooperative solve
+  int *coop_flags = nullptr;     // its per-hemisphere liveness exchange
 };
 
 namespace {
@@ -917,43 +918,60 @@ __global__ void __launch_bounds__(kSolveThreads) k_mi_bicgstab(MiK c, double rel
   store_result(c, h, res, first);
 }
 
-// The BiCGStab solve of one hemisphere spread over kCoopBlocks workgroups of a
-// cooperative launch: the same iterations as k_mi_bicgstab, with grid-wide
-// barriers where one workgroup's barrier sufficed there. Dot products sum
-// the workgroups' partial sums in workgroup order from two alternating slot
+// Cooperative BiCGStab: each hemisphere's solve spread over kCoopBlocks
+// workgroups (the same iterations as k_mi_bicgstab, with grid-wide barriers
+// where one workgroup's barrier sufficed there). Dot products sum the
+// workgroups' partial sums in workgroup order from two alternating slot
 // banks (a bank is rewritten only after every workgroup passed the barrier
-// that follows its last read). Every workgroup evaluates the same sums in
-// the same order, so control flow is uniform over the grid and every
-// workgroup reaches every barrier.
+// that follows its last read), so every workgroup of a hemisphere computes
+// the same sums and takes the same branches.
 constexpr int kCoopBlocks = 16;
 
-__global__ void __launch_bounds__(kSolveThreads) k_mi_bicgstab_coop(MiK c, int h, double rel_tol,
-                                                                    int maxit, double *slots) {
+// Both hemispheres in one cooperative launch: workgroups [0, kCoopBlocks)
+// solve the north, [kCoopBlocks, 2 kCoopBlocks) the south, concurrently. The
+// grid barriers are shared, so every workgroup must pass the same sequence
+// of barriers whatever its hemisphere does: each iteration runs a fixed
+// schedule of ten barrier-carrying steps, and a hemisphere that has
+// converged, broken down or needs no true-residual check this iteration
+// skips the work of a step but not its barrier. No branch contains a
+// barrier, the loop bound and its exit test are grid-uniform (the exit reads
+// both hemispheres' flags after a barrier), so every workgroup leaves after
+// the same number of barriers, at the latest after maximum_iterations.
+// Per hemisphere the arithmetic is k_mi_bicgstab's.
+__global__ void __launch_bounds__(kSolveThreads) k_mi_bicgstab_pair(MiK c, double rel_tol,
+                                                                    int maxit, double *slots,
+                                                                    int *flags) {
   namespace cg = cooperative_groups;
   cg::grid_group grid = cg::this_grid();
   __shared__ double red[kSolveThreads / 64];
+  const int h = blockIdx.x / kCoopBlocks, lb = blockIdx.x % kCoopBlocks;
   const long n = c.n;
-  const long t0 = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const long st = (long)gridDim.x * blockDim.x;
-  const bool pole_wave = blockIdx.x == 0 && threadIdx.x < 64;
+  const long t0 = lb * (long)blockDim.x + threadIdx.x;
+  const long st = (long)kCoopBlocks * blockDim.x;
+  const bool pole_wave = lb == 0 && threadIdx.x < 64;
   int bank = 0;
-  auto gsum = [&](double v) -> double {
-    const double b = block_sum(v, red);
-    double *sl = slots + bank * kCoopBlocks;
-    if (threadIdx.x == 0) sl[blockIdx.x] = b;
+  // sums of up to two values over this hemisphere's workgroups
+  auto gsum2 = [&](double v, double w, double &sv, double &sw) {
+    const double bv = block_sum(v, red);
+    const double bw = block_sum(w, red);
+    double *sl = slots + bank * (4 * kCoopBlocks) + h * (2 * kCoopBlocks);
+    if (threadIdx.x == 0) {
+      sl[lb] = bv;
+      sl[kCoopBlocks + lb] = bw;
+    }
     bank ^= 1;
     grid.sync();
-    double s = 0.0;
-    for (int i = 0; i < (int)gridDim.x; ++i) s += sl[i];
-    return s;
+    sv = 0.0;
+    sw = 0.0;
+    for (int i = 0; i < kCoopBlocks; ++i) {
+      sv += sl[i];
+      sw += sl[kCoopBlocks + i];
+    }
   };
-  auto precondition = [&](const double *in, double *out) {
-    grid.sync();  // `in` complete
-    if (t0 == 0) out[0] = in[0] / c.vec(h, V_DIAG)[0];
-    const int lane = threadIdx.x & 63;
-    for (long ring = t0 >> 6; ring < c.nt - 2; ring += st >> 6)
-      ring_apply(c, h, in, out, (int)ring, lane);
-    grid.sync();  // `out` complete
+  auto gsum = [&](double v) {
+    double sv, sw;
+    gsum2(v, 0.0, sv, sw);
+    return sv;
   };
   const double dt = c.max_colat / (double)(c.nt - 1);
   const double half = 0.5 * dt;
@@ -963,6 +981,16 @@ __global__ void __launch_bounds__(kSolveThreads) k_mi_bicgstab_coop(MiK c, int h
          *sh = c.vec(h, V_SH), *pp = c.vec(h, V_PP), *s = c.vec(h, V_S), *psv = c.vec(h, V_PS),
          *as = c.vec(h, V_AS);
   const double *rhs = c.vec(h, V_RHS);
+  const int lane = threadIdx.x & 63;
+  auto precondition = [&](bool work, const double *in, double *out) {
+    grid.sync();  // `in` complete
+    if (work) {
+      if (t0 == 0) out[0] = in[0] / c.vec(h, V_DIAG)[0];
+      for (long ring = t0 >> 6; ring < c.nt - 2; ring += st >> 6)
+        ring_apply(c, h, in, out, (int)ring, lane);
+    }
+    grid.sync();  // `out` complete
+  };
   for (long ring = t0; ring < c.nt - 2; ring += st) ring_factor(c, h, (int)ring);
   double a = 0.0, b = 0.0;
   {
@@ -976,96 +1004,130 @@ __global__ void __launch_bounds__(kSolveThreads) k_mi_bicgstab_coop(MiK c, int h
       b += q == 0 ? 0.0 : orig_term(c, rhs[q], q, ps, dt);
     }
   }
-  double norm = sqrt(fmax(0.0, gsum(a)));
+  double n0, b0;
+  gsum2(a, b, n0, b0);  // also orders the ring factors before their first use
+  double norm = sqrt(fmax(0.0, n0));
   const double first = norm;
-  const double target = rel_tol * fmax(sqrt(fmax(0.0, gsum(b))), 1.0);
+  const double target = rel_tol * fmax(sqrt(fmax(0.0, b0)), 1.0);
   SolveResult res{0, norm <= target, norm};
+  bool live = !res.ok;  // this hemisphere still iterates
   double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
-  for (int it = 1; it <= maxit && !res.ok; ++it) {
+  bool other_live = true;
+  for (int it = 1; it <= maxit && (live || other_live); ++it) {
+    // 1: rho
     a = 0.0;
-    for (long q = t0; q < n; q += st) a += sh[q] * r[q];
+    if (live)
+      for (long q = t0; q < n; q += st) a += sh[q] * r[q];
     const double rho = gsum(a);
-    if (!isfinite(rho) || fabs(rho) <= DBL_MIN || !isfinite(omega) || fabs(omega) <= DBL_MIN)
-      break;
-    const double beta = (rho / rho_prev) * (alpha / omega);
-    for (long q = t0; q < n; q += st) p[q] = r[q] + beta * (p[q] - omega * ap[q]);
-    precondition(p, pp);
+    if (live && (!isfinite(rho) || fabs(rho) <= DBL_MIN || !isfinite(omega) ||
+                 fabs(omega) <= DBL_MIN))
+      live = false;
+    // 2, 3: p and its preconditioned image
+    if (live) {
+      const double beta = (rho / rho_prev) * (alpha / omega);
+      for (long q = t0; q < n; q += st) p[q] = r[q] + beta * (p[q] - omega * ap[q]);
+    }
+    precondition(live, p, pp);
+    // 4: A pp and den
     a = 0.0;
     {
-      const double pole = pole_wave ? pole_row(c, h, true, pp, pe) : 0.0;
-      for (long q = t0; q < n; q += st) {
-        ap[q] = q == 0 ? pole : apply_at(c, h, true, pp, q, nullptr, pe);
-        a += sh[q] * ap[q];
-      }
+      const double pole = (live && pole_wave) ? pole_row(c, h, true, pp, pe) : 0.0;
+      if (live)
+        for (long q = t0; q < n; q += st) {
+          ap[q] = q == 0 ? pole : apply_at(c, h, true, pp, q, nullptr, pe);
+          a += sh[q] * ap[q];
+        }
     }
     const double den = gsum(a);
-    alpha = rho / den;
+    // 5: s and its norm
     a = 0.0;
-    for (long q = t0; q < n; q += st) {
-      s[q] = r[q] - alpha * ap[q];
-      a += orig_term(c, s[q], q, ps, dt);
+    if (live) {
+      alpha = rho / den;
+      for (long q = t0; q < n; q += st) {
+        s[q] = r[q] - alpha * ap[q];
+        a += orig_term(c, s[q], q, ps, dt);
+      }
     }
     const double snorm = sqrt(fmax(0.0, gsum(a)));
-    if (!isfinite(den) || fabs(den) <= DBL_MIN) break;
-    if (snorm <= target) {
+    if (live && (!isfinite(den) || fabs(den) <= DBL_MIN)) live = false;
+    if (live && snorm <= target) {
       res.last = snorm;
       res.iters = it;
       res.ok = true;
+      live = false;
       for (long q = t0; q < n; q += st) x[q] += alpha * pp[q];
-      break;
     }
-    precondition(s, psv);
+    // 6, 7: preconditioned s
+    precondition(live, s, psv);
+    // 8: A ps and omega
     a = 0.0;
     b = 0.0;
     {
-      const double pole = pole_wave ? pole_row(c, h, true, psv, pe) : 0.0;
-      for (long q = t0; q < n; q += st) {
-        as[q] = q == 0 ? pole : apply_at(c, h, true, psv, q, nullptr, pe);
-        a += as[q] * as[q];
-        b += as[q] * s[q];
-      }
-    }
-    const double wden = gsum(a);
-    const double wnum = gsum(b);
-    omega = wnum / wden;
-    if (!isfinite(wden) || wden <= DBL_MIN || !isfinite(omega) || fabs(omega) <= DBL_MIN)
-      break;
-    a = 0.0;
-    for (long q = t0; q < n; q += st) {
-      x[q] += alpha * pp[q] + omega * psv[q];
-      r[q] = s[q] - omega * as[q];
-      a += orig_term(c, r[q], q, ps, dt);
-    }
-    norm = sqrt(fmax(0.0, gsum(a)));
-    res.iters = it;
-    res.last = norm;
-    if (norm <= target || it % 256 == 0) {
-      // x complete: the sum above ended with a grid barrier
-      a = 0.0;
-      {
-        const double pole = pole_wave ? pole_row(c, h, true, x, pe) : 0.0;
+      const double pole = (live && pole_wave) ? pole_row(c, h, true, psv, pe) : 0.0;
+      if (live)
         for (long q = t0; q < n; q += st) {
-          r[q] = rhs[q] - (q == 0 ? pole : apply_at(c, h, true, x, q, nullptr, pe));
-          sh[q] = r[q];
-          a += orig_term(c, r[q], q, ps, dt);
+          as[q] = q == 0 ? pole : apply_at(c, h, true, psv, q, nullptr, pe);
+          a += as[q] * as[q];
+          b += as[q] * s[q];
         }
+    }
+    double wden, wnum;
+    gsum2(a, b, wden, wnum);
+    if (live) {
+      omega = wnum / wden;
+      if (!isfinite(wden) || wden <= DBL_MIN || !isfinite(omega) || fabs(omega) <= DBL_MIN)
+        live = false;
+    }
+    // 9: x, r and the recursive residual norm
+    a = 0.0;
+    if (live)
+      for (long q = t0; q < n; q += st) {
+        x[q] += alpha * pp[q] + omega * psv[q];
+        r[q] = s[q] - omega * as[q];
+        a += orig_term(c, r[q], q, ps, dt);
       }
-      norm = sqrt(fmax(0.0, gsum(a)));
+    const double rn = gsum(a);
+    bool verify = false;
+    if (live) {
+      norm = sqrt(fmax(0.0, rn));
+      res.iters = it;
+      res.last = norm;
+      verify = norm <= target || it % 256 == 0;
+    }
+    // 10: true residual when checked; the exit flags of both hemispheres
+    a = 0.0;
+    if (verify) {
+      const double pole = pole_wave ? pole_row(c, h, true, x, pe) : 0.0;
+      for (long q = t0; q < n; q += st) {
+        r[q] = rhs[q] - (q == 0 ? pole : apply_at(c, h, true, x, q, nullptr, pe));
+        sh[q] = r[q];
+        a += orig_term(c, r[q], q, ps, dt);
+      }
+    }
+    const double tn = gsum(a);
+    if (verify) {
+      norm = sqrt(fmax(0.0, tn));
       res.last = norm;
       if (norm <= target) {
         res.ok = true;
-        break;
+        live = false;
+      } else {
+        for (long q = t0; q < n; q += st) {
+          p[q] = 0.0;
+          ap[q] = 0.0;
+        }
+        rho_prev = alpha = omega = 1.0;
       }
-      for (long q = t0; q < n; q += st) {
-        p[q] = 0.0;
-        ap[q] = 0.0;
-      }
-      rho_prev = alpha = omega = 1.0;
-      continue;
+    } else if (live) {
+      rho_prev = rho;
     }
-    rho_prev = rho;
+    // exchange liveness: written now, read after the next iteration's first
+    // barrier; one more round trip than needed only delays the exit
+    if (lb == 0 && threadIdx.x == 0) flags[(it & 1) * 2 + h] = live ? 1 : 0;
+    grid.sync();
+    other_live = flags[(it & 1) * 2 + (1 - h)] != 0;
   }
-  if (blockIdx.x == 0) store_result(c, h, res, first);
+  if (lb == 0) store_result(c, h, res, first);
 }
 
 // ---- learned warm start ----------------------------------------------------------
@@ -1276,23 +1338,20 @@ int solve(Problem &P, bool tensor) {
   const double rel = m.cfg.relative_tolerance;
   const int maxit = m.cfg.maximum_iterations;
   if (tensor) {
-    // cooperative form (one launch per hemisphere) when the device can hold
-    // its workgroups at once; the single-workgroup kernel otherwise
+    // cooperative form (both hemispheres, 2 kCoopBlocks workgroups) when the
+    // device can hold its workgroups at once; one workgroup per hemisphere
+    // otherwise
     bool coop = m.coop_slots != nullptr;
-    for (int h = 0; h < 2 && coop; ++h) {
-      int hh = h;
+    if (coop) {
       double relv = rel;
       int maxv = maxit;
       double *slots = m.coop_slots;
-      void *args[] = {&c, &hh, &relv, &maxv, &slots};
-      if (hipLaunchCooperativeKernel((const void *)k_mi_bicgstab_coop, dim3(kCoopBlocks),
+      int *flags = m.coop_flags;
+      void *args[] = {&c, &relv, &maxv, &slots, &flags};
+      if (hipLaunchCooperativeKernel((const void *)k_mi_bicgstab_pair, dim3(2 * kCoopBlocks),
                                      dim3(kSolveThreads), args, 0, s) != hipSuccess) {
         (void)hipGetLastError();
         coop = false;
-        if (h == 1) {
-          set_error("cooperative launch failed after the first hemisphere");
-          return -1;
-        }
       }
     }
     if (!coop) k_mi_bicgstab<<<2, kSolveThreads, 0, s>>>(c, rel, maxit);
@@ -1469,15 +1528,17 @@ int mi_create(Problem &P) {
   GCS_HIP(hipMalloc(&m.potential, polar * sizeof(double)));
   GCS_HIP(hipMalloc(&m.work, 2L * kVectors * m.n * sizeof(double)));
   {
-    // the cooperative BiCGStab needs all kCoopBlocks workgroups resident
+    // the cooperative BiCGStab needs all 2 kCoopBlocks workgroups resident
     int dev = 0, coop = 0, per_cu = 0, cus = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess &&
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mi_bicgstab_coop, kSolveThreads,
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mi_bicgstab_pair, kSolveThreads,
                                                      0) == hipSuccess &&
-        coop != 0 && per_cu * cus >= kCoopBlocks)
-      GCS_HIP(hipMalloc(&m.coop_slots, 2 * kCoopBlocks * sizeof(double)));
+        coop != 0 && per_cu * cus >= 2 * kCoopBlocks) {
+      GCS_HIP(hipMalloc(&m.coop_slots, 8 * kCoopBlocks * sizeof(double)));
+      GCS_HIP(hipMalloc(&m.coop_flags, 4 * sizeof(int)));
+    }
     (void)hipGetLastError();
   }
   GCS_HIP(hipMalloc(&m.scal, 2 * SL_COUNT * sizeof(double)));
@@ -1520,6 +1581,7 @@ void mi_destroy(Problem &P) {
   if (m->host_scal) (void)hipHostFree(m->host_scal);
   (void)hipFree(m->warm_scale);
   (void)hipFree(m->coop_slots);
+  (void)hipFree(m->coop_flags);
   delete m;
   P.mi = nullptr;
 }
