@@ -64,6 +64,7 @@ void free_problem(Problem *P) {
   for (auto &g : P->graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   (void)hipFree(P->planes);
+  (void)hipFree(P->planes32);
   (void)hipFree(P->bytes);
   (void)hipFree(P->scalars);
   (void)hipFree(P->cfl_partial);
@@ -162,6 +163,11 @@ int download_field(Problem &P, int face, int field, std::vector<double> &out) {
   size_t ce[3] = {(size_t)L.NI, (size_t)L.NJ, (size_t)L.NK};
   size_t ve[3] = {(size_t)L.NI + 1, (size_t)L.NJ + 1, (size_t)L.NK + 1};
   const size_t ncell = ce[0] * ce[1] * ce[2];
+  // The mixed form keeps the predicted state and the face fluxes in FP32.
+  const bool widened = field == GCS_FIELD_PREDICTED_CONSERVED ||
+                       field == GCS_FIELD_PREDICTED_CELL_MAGNETIC ||
+                       (field >= GCS_FIELD_FACE_FLUX + 12 && field < GCS_FIELD_HYDRO_RATE);
+  if (widened && P.last_step_mixed && widen_mixed(P) != 0) return -1;
   auto planes = [&](int first, int ncomp, const size_t e[3]) {
     out.assign(e[0] * e[1] * e[2] * ncomp, 0.0);
     for (int q = 0; q < ncomp; ++q)
@@ -432,10 +438,11 @@ int gcs_problem_edge_valid(Problem *P, int face, int dir, unsigned char **data,
   return 0;
 }
 
-// Keep (1) or skip (0) the work that only reproduces the reference's ghost
-// values in storage; the step's active results are the same either way.
+// FP64 (0) or the mixed form (1); the first switch to the mixed form
+// allocates its FP32 planes.
 int gcs_problem_set_precision(Problem *P, int mixed) {
   if (P == nullptr || (mixed != 0 && mixed != 1)) return -1;
+  if (mixed == 1 && gcs::prepare_mixed(*P) != 0) return -1;
   P->mixed = mixed == 1;
   return 0;
 }
@@ -459,6 +466,8 @@ int gcs_problem_set_graph(Problem *P, int on) {
   return 0;
 }
 
+// Keep (1) or skip (0) the work that only reproduces the reference's ghost
+// values in storage; the step's active results are the same either way.
 int gcs_problem_set_fidelity(Problem *P, int on) {
   if (P == nullptr) return -1;
   P->fidelity = on != 0;

@@ -125,6 +125,7 @@ struct Problem {
   StepBoundary bnd;
   Layout L;
   double *planes = nullptr;        // PLANES * L.total doubles
+  float *planes32 = nullptr;       // PLANES32 * L.total floats (mixed form)
   uint8_t *bytes = nullptr;        // BYTE_PLANES * L.total
   StepScalars *scalars = nullptr;  // device
   unsigned long long *cfl_partial = nullptr;  // per-block CFL minima
@@ -163,6 +164,8 @@ struct Problem {
   WindSeries wind_series;
   bool fidelity = true;            // reproduce ghost values of the storage
   bool mixed = false;              // FP32 reconstruction, fluxes and EMF
+  // The predictor caches and face fluxes of the last step are in planes32.
+  bool last_step_mixed = false;
   // Cell B of both levels equals the recovery from their face fluxes (true
   // after a completed step; false at creation and after a host upload).
   bool b_recovered = false;
@@ -196,6 +199,11 @@ int driver_step(Problem &P);
 // Device buffers, stream and events of a problem whose layout L is set;
 // scalars are initialised. free_problem releases everything.
 int allocate_problem(Problem &P);
+// Allocate planes32 and fill its geometry planes (first use of the mixed
+// form); widen_mixed copies the mixed form's caches and face fluxes into
+// their FP64 planes for download.
+int prepare_mixed(Problem &P);
+int widen_mixed(Problem &P);
 void free_problem(Problem *P);
 int init_scalars(Problem &P);
 int read_scalars(Problem &P, StepScalars &s);

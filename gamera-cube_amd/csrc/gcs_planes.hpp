@@ -57,6 +57,28 @@ constexpr int P_SCRATCH_STRESS = P_FLUID;         // +9 background face stress
 constexpr int P_SCRATCH_AREA = P_FLUID + 9;       // +3 uncalibrated face area
 constexpr int P_SCRATCH_LINE = P_FLUID + 12;      // +3 edge line integrals of A
 
+// ---- FP32 planes of the mixed form ----------------------------------------
+// Problem::planes32 + q * layout.total (floats), allocated when the mixed
+// form is first selected. The FP32 kernels read their inputs and write
+// their results here at the precision they compute in, so the predictor's
+// caches, the face geometry they use, the face velocities and the face
+// fluxes cross HBM at 4 bytes instead of 8.
+constexpr int M_VOL = 0;                           // cell volume
+constexpr int M_WPRED = M_VOL + 1;                 // +5 (as P_WPRED)
+constexpr int M_BPRED = M_WPRED + 5;               // +3
+constexpr int M_UPRED = M_BPRED + 3;               // +5
+constexpr int M_WV = M_UPRED + 5;                  // +3
+constexpr int M_BN = M_WV + 3;                     // +3
+// Faces, direction d: M_FACE + 14 d + {area, normal(3), tangent1(3),
+// tangent2(3), B0(3), B0 flux / area}.
+constexpr int M_FACE = M_BN + 3;
+constexpr int MF_AREA = 0, MF_N = 1, MF_T1 = 4, MF_T2 = 7, MF_B0 = 10, MF_B0N = 13,
+              M_FACE_PLANES = 14;
+constexpr int M_FVEL = M_FACE + 3 * M_FACE_PLANES; // +9 face velocity, 3 per dir
+constexpr int M_FLUID = M_FVEL + 9;                // +15 area-scaled, 5 per dir
+constexpr int M_MAXWELL = M_FLUID + 15;            // +9, 3 per dir
+constexpr int PLANES32 = M_MAXWELL + 9;            // 95
+
 // Byte planes.
 constexpr int Q_VALID = 0;   // +3 edge validity (nonorthogonal_grid.c:391-403)
 constexpr int Q_EBOK = 3;    // +3 edge transverse solve well posed

@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 
 #include "gcs_device.hpp"
 #include "gcs_math.hpp"
@@ -41,12 +42,14 @@ namespace {
 struct K {
   Layout L;
   double *pl;
+  float *fl;  // FP32 planes of the mixed form (null until first used)
   uint8_t *by;
   StepScalars *sc;
   int cur, old;  // time-level slots of this step
   StepOptions o;
   MiView mi;
   __device__ double *p(int q) const { return pl + (long)q * L.total; }
+  __device__ float *sp(int q) const { return fl + (long)q * L.total; }
   __device__ const uint8_t *b(int q) const { return by + (long)q * L.total; }
   __device__ void fail(unsigned stage, long index) const {
     if (atomicOr(&sc->failed, 1u) == 0u) {
@@ -241,19 +244,43 @@ __global__ void k_sync_phi(K c, const SeamPair *pairs, int n, int both) {
   }
 }
 
+// Face-flux planes: FP64 (T = double) or the mixed form's FP32 planes.
+template <typename T>
+GCS_DEV T *fluid_plane(const K &c, int d, int q);
+template <>
+GCS_DEV double *fluid_plane<double>(const K &c, int d, int q) {
+  return c.p(P_FLUID + 5 * d + q);
+}
+template <>
+GCS_DEV float *fluid_plane<float>(const K &c, int d, int q) {
+  return c.sp(M_FLUID + 5 * d + q);
+}
+template <typename T>
+GCS_DEV T *maxwell_plane(const K &c, int d, int m);
+template <>
+GCS_DEV double *maxwell_plane<double>(const K &c, int d, int m) {
+  return c.p(P_MAXWELL + 3 * d + m);
+}
+template <>
+GCS_DEV float *maxwell_plane<float>(const K &c, int d, int m) {
+  return c.sp(M_MAXWELL + 3 * d + m);
+}
+
 // Same reconciliation of the fluid (5) and Maxwell (3) face fluxes.
+template <typename T>
 __global__ void k_sync_flux(K c, const SeamPair *pairs, int n) {
   const long t = tid();
   if (t >= n) return;
   const SeamPair sp = pairs[t];
   for (int q = 0; q < 8; ++q) {
-    double *a = q < 5 ? c.p(P_FLUID + 5 * sp.dir_a + q) + sp.a
-                      : c.p(P_MAXWELL + 3 * sp.dir_a + q - 5) + sp.a;
-    double *b = q < 5 ? c.p(P_FLUID + 5 * sp.dir_b + q) + sp.b
-                      : c.p(P_MAXWELL + 3 * sp.dir_b + q - 5) + sp.b;
-    const double common = 0.5 * (*a + sp.sign * *b);
+    T *a = q < 5 ? fluid_plane<T>(c, sp.dir_a, q) + sp.a
+                 : maxwell_plane<T>(c, sp.dir_a, q - 5) + sp.a;
+    T *b = q < 5 ? fluid_plane<T>(c, sp.dir_b, q) + sp.b
+                 : maxwell_plane<T>(c, sp.dir_b, q - 5) + sp.b;
+    const T sign = (T)sp.sign;
+    const T common = T(0.5) * (*a + sign * *b);
     *a = common;
-    *b = sp.sign * common;
+    *b = sign * common;
   }
 }
 
@@ -724,7 +751,10 @@ __global__ void k_recover_list(K c, const int *cells, int n, int phi_plane,
 // ---- predictor -----------------------------------------------------------------
 
 // nonorthogonal_step.c:379-419: primitive AB predictor of every cell,
-// face-flux predictor of every face and B of the predicted fluxes.
+// face-flux predictor of every face and B of the predicted fluxes. With
+// Mixed the caches that only the FP32 kernels read (primitives, vol * v,
+// B.n, predicted U and B) go to the FP32 planes instead of the FP64 ones.
+template <bool Mixed>
 __global__ void k_predict(K c) {
   const Layout &L = c.L;
   const int lo[3] = {0, 0, 0};
@@ -744,7 +774,11 @@ __global__ void k_predict(K c) {
     c.p(P_PHIPRED + d)[at] = pred;
     // Face-normal B of the predicted flux, read by the face fluxes and the
     // edge EMF (both divide this flux by this area).
-    c.p(P_BN + d)[at] = pred / c.p(P_FACE + FACE_PLANES * d + FO_AREA)[at];
+    const double bn = pred / c.p(P_FACE + FACE_PLANES * d + FO_AREA)[at];
+    if (Mixed)
+      c.sp(M_BN + d)[at] = (float)bn;
+    else
+      c.p(P_BN + d)[at] = bn;
   }
   if (!(in[0] && in[1] && in[2])) return;
   double uo[5], uc[5], up[5];
@@ -752,29 +786,46 @@ __global__ void k_predict(K c) {
     uo[q] = c.p(plane_u(c.old, q))[at];
     uc[q] = c.p(plane_u(c.cur, q))[at];
   }
+  const double nan = __longlong_as_double(0x7ff8000000000000ll);
+  auto wpred = [&](int q, double v) {
+    if (Mixed)
+      c.sp(M_WPRED + q)[at] = (float)v;
+    else
+      c.p(P_WPRED + q)[at] = v;
+  };
   if (!predict_cell(uo, uc, r, c.o.upd_gamma, c.o.upd_rho_floor, c.o.upd_p_floor, up)) {
-    for (int q = 0; q < 5; ++q) c.p(P_WPRED + q)[at] = __longlong_as_double(0x7ff8000000000000ll);
+    for (int q = 0; q < 5; ++q) wpred(q, nan);
     c.fail(31, at);
     return;
   }
-  for (int q = 0; q < 5; ++q) c.p(P_UPRED + q)[at] = up[q];
+  for (int q = 0; q < 5; ++q) {
+    if (Mixed)
+      c.sp(M_UPRED + q)[at] = (float)up[q];
+    else
+      c.p(P_UPRED + q)[at] = up[q];
+  }
   // The sweep's primitive conversion of this cell (nonorthogonal_state.c:60-93
   // with the sweep's floors), once per cell instead of once per stencil use;
   // the face fluxes read it back. NaN marks a failed conversion.
   {
     Prim w;
     const bool ok = cons_to_prim(up, c.o.gamma, c.o.rho_floor, c.o.p_floor, w);
-    const double nan = __longlong_as_double(0x7ff8000000000000ll);
-    c.p(P_WPRED + 0)[at] = ok ? w.rho : nan;
-    for (int e = 0; e < 3; ++e) c.p(P_WPRED + 1 + e)[at] = ok ? w.v[e] : nan;
-    c.p(P_WPRED + 4)[at] = ok ? w.p : nan;
+    wpred(0, ok ? w.rho : nan);
+    for (int e = 0; e < 3; ++e) wpred(1 + e, ok ? w.v[e] : nan);
+    wpred(4, ok ? w.p : nan);
   }
   // Volume-weighted velocity with the EMF's density floor, the summand of the
   // face velocity's central8 (nonorthogonal_sweep.c:223-259).
   {
     const double vol = c.p(P_VOL)[at];
     const double rho = fmax(up[0], c.o.emf_rho_floor);
-    for (int e = 0; e < 3; ++e) c.p(P_WV + e)[at] = vol * up[1 + e] / rho;
+    for (int e = 0; e < 3; ++e) {
+      const double wv = vol * up[1 + e] / rho;
+      if (Mixed)
+        c.sp(M_WV + e)[at] = (float)wv;
+      else
+        c.p(P_WV + e)[at] = wv;
+    }
   }
   // B of the predicted fluxes; the upper faces' predictions are recomputed
   // here with the same expression rather than read back.
@@ -795,7 +846,12 @@ __global__ void k_predict(K c) {
     c.fail(32, at);
     return;
   }
-  for (int d = 0; d < 3; ++d) c.p(P_BPRED + d)[at] = b[d];
+  for (int d = 0; d < 3; ++d) {
+    if (Mixed)
+      c.sp(M_BPRED + d)[at] = (float)b[d];
+    else
+      c.p(P_BPRED + d)[at] = b[d];
+  }
 }
 
 // ---- face fluxes -----------------------------------------------------------------
@@ -1100,6 +1156,7 @@ __global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
 }
 
 // nonorthogonal_step.c:35-71 on the inner wall face.
+template <typename T>
 __global__ void k_inner_trap(K c) {
   const Layout &L = c.L;
   const int lo[3] = {L.ghost, L.ghost, L.ghost};
@@ -1107,11 +1164,11 @@ __global__ void k_inner_trap(K c) {
   int f, i, j, k;
   if (!box(tid(), lo, hi, f, i, j, k)) return;
   const long at = L.at(f, i, j, k);
-  double *m = c.p(P_FLUID + 0) + at;
-  double *e = c.p(P_FLUID + 4) + at;
-  if (*m > 0.0) {
-    *m = 0.0;
-    if (*e > 0.0) *e = 0.0;
+  T *m = fluid_plane<T>(c, 0, 0) + at;
+  T *e = fluid_plane<T>(c, 0, 4) + at;
+  if (*m > T(0)) {
+    *m = T(0);
+    if (*e > T(0)) *e = T(0);
   }
 }
 
@@ -1239,9 +1296,10 @@ __global__ void __launch_bounds__(128) k_edge_emf(K c, int e) {
 // ---- mixed precision: FP32 reconstruction, fluxes and EMF ---------------------
 //
 // The same sweeps as k_face_flux, k_face_velocity and k_edge_emf with the
-// arithmetic in float: inputs are read from the FP64 planes and rounded once,
-// results are stored back into the FP64 flux/EMF planes (the area scaling
-// is done in FP64), so the update, CT, CFL and ionosphere stay FP64.
+// arithmetic in float. Their per-step inputs (the predictor's caches, face
+// velocities) and the face geometry they use are read from FP32 planes,
+// already rounded; the face fluxes are stored to FP32 planes and the edge
+// EMF to its FP64 plane, so the update, CT, CFL and ionosphere stay FP64.
 
 __global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
   using R = float;
@@ -1256,20 +1314,19 @@ __global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
   const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
   const StepOptions &o = c.o;
   const R gamma = (R)o.gamma, pdm = (R)o.pdm;
-  // Primitives of the stencil from the predictor's FP64 cache, rounded to R.
+  // Primitives of the stencil from the predictor's FP32 cache.
   R q8[8];
   bool converted = true;
   for (int n = 0; n < 8; ++n) {
-    const double x = c.p(P_WPRED)[at + (n - 4) * st];
-    converted &= isfinite(x);
-    q8[n] = (R)x;
+    q8[n] = c.sp(M_WPRED)[at + (n - 4) * st];
+    converted &= isfinite(q8[n]);
   }
   if (!converted) {
     c.fail(41, at);
     return;
   }
   R vol[8];
-  for (int n = 0; n < 8; ++n) vol[n] = (R)c.p(P_VOL)[at + (n - 4) * st];
+  for (int n = 0; n < 8; ++n) vol[n] = c.sp(M_VOL)[at + (n - 4) * st];
   const R iv = central8_r(vol);
   if (!isfinite(iv) || fabs(iv) <= tiny<R>()) {
     c.fail(42, at);
@@ -1278,15 +1335,15 @@ __global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
   PrimR<R> w[2];
   up7_pdm_r(vol, iv, q8, pdm, w[0].rho, w[1].rho);
   for (int e = 0; e < 3; ++e) {
-    for (int n = 0; n < 8; ++n) q8[n] = (R)c.p(P_WPRED + 1 + e)[at + (n - 4) * st];
+    for (int n = 0; n < 8; ++n) q8[n] = c.sp(M_WPRED + 1 + e)[at + (n - 4) * st];
     up7_pdm_r(vol, iv, q8, pdm, w[0].v[e], w[1].v[e]);
   }
-  for (int n = 0; n < 8; ++n) q8[n] = (R)c.p(P_WPRED + 4)[at + (n - 4) * st];
+  for (int n = 0; n < 8; ++n) q8[n] = c.sp(M_WPRED + 4)[at + (n - 4) * st];
   up7_pdm_r(vol, iv, q8, pdm, w[0].p, w[1].p);
   R bb[2][3];
   for (int e = 0; e < 3; ++e) {
     R q8[8];
-    for (int n = 0; n < 8; ++n) q8[n] = (R)c.p(P_BPRED + e)[at + (n - 4) * st];
+    for (int n = 0; n < 8; ++n) q8[n] = c.sp(M_BPRED + e)[at + (n - 4) * st];
     up7_pdm_r(vol, iv, q8, pdm, bb[0][e], bb[1][e]);
   }
   if (!valid_interface_r(w[0].rho, w[0].v, w[0].p) ||
@@ -1294,22 +1351,21 @@ __global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
     c.fail(43, at);
     return;
   }
-  const int fb = P_FACE + FACE_PLANES * d;
-  const double area = c.p(fb + FO_AREA)[at];
+  const int fb = M_FACE + M_FACE_PLANES * d;
+  const R area = c.sp(fb + MF_AREA)[at];
   FaceFrameR<R> fr;
   for (int e = 0; e < 3; ++e) {
-    fr.n[e] = (R)c.p(fb + FO_N + e)[at];
-    fr.t1[e] = (R)c.p(fb + FO_T1 + e)[at];
-    fr.t2[e] = (R)c.p(fb + FO_T2 + e)[at];
+    fr.n[e] = c.sp(fb + MF_N + e)[at];
+    fr.t1[e] = c.sp(fb + MF_T1 + e)[at];
+    fr.t2[e] = c.sp(fb + MF_T2 + e)[at];
   }
   FluidFluxR<R> fl;
   fluid_flux_r(w, gamma, fr, fl);
-  const R bn = (R)c.p(P_BN + d)[at];
-  const R b0[3] = {(R)c.p(P_B0F + 3 * d)[at], (R)c.p(P_B0F + 3 * d + 1)[at],
-                   (R)c.p(P_B0F + 3 * d + 2)[at]};
+  const R bn = c.sp(M_BN + d)[at];
+  const R b0[3] = {c.sp(fb + MF_B0)[at], c.sp(fb + MF_B0 + 1)[at], c.sp(fb + MF_B0 + 2)[at]};
   const R cl = (R)o.c;
   MaxwellFluxR<R> mx;
-  if (!maxwell_flux_r(w, bb, bn, fr.n, b0, (R)c.p(P_B0N + d)[at], o.boris != 0, cl, fl.un,
+  if (!maxwell_flux_r(w, bb, bn, fr.n, b0, c.sp(fb + MF_B0N)[at], o.boris != 0, cl, fl.un,
                       mx)) {
     c.fail(44, at);
     return;
@@ -1322,12 +1378,12 @@ __global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
     if (sqrt(R(0.5) * s2) >= R(1.5) * cl) {
       const long lc = at - st;
       for (int q = 0; q < 5; ++q) {
-        const double lo_u = c.p(P_UPRED + q)[lc], up_u = c.p(P_UPRED + q)[at];
+        const R lo_u = c.sp(M_UPRED + q)[lc], up_u = c.sp(M_UPRED + q)[at];
         if (!isfinite(lo_u) || !isfinite(up_u)) {
           c.fail(45, at);
           return;
         }
-        fl.f[q] -= cl * (R)(up_u - lo_u);
+        fl.f[q] -= cl * (up_u - lo_u);
       }
       R t2 = R(0);
       for (int side = 0; side < 2; ++side)
@@ -1337,8 +1393,8 @@ __global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
                 (unsigned long long)__double_as_longlong((double)sqrt(R(0.5) * t2)));
     }
   }
-  for (int q = 0; q < 5; ++q) c.p(P_FLUID + 5 * d + q)[at] = area * (double)fl.f[q];
-  for (int e = 0; e < 3; ++e) c.p(P_MAXWELL + 3 * d + e)[at] = area * (double)mx.m[e];
+  for (int q = 0; q < 5; ++q) c.sp(M_FLUID + 5 * d + q)[at] = area * fl.f[q];
+  for (int e = 0; e < 3; ++e) c.sp(M_MAXWELL + 3 * d + e)[at] = area * mx.m[e];
 }
 
 __global__ void k_face_velocity_fp32(K c, int d) {
@@ -1364,13 +1420,13 @@ __global__ void k_face_velocity_fp32(K c, int d) {
   R vol[8], wv[3][8];
   for (int n = 0; n < 8; ++n) {
     const long cell = at + (n - 4) * st;
-    vol[n] = (R)c.p(P_VOL)[cell];
-    for (int e = 0; e < 3; ++e) wv[e][n] = (R)c.p(P_WV + e)[cell];
+    vol[n] = c.sp(M_VOL)[cell];
+    for (int e = 0; e < 3; ++e) wv[e][n] = c.sp(M_WV + e)[cell];
   }
   const R iv = central8_r(vol);
   const bool ok = isfinite(iv) && fabs(iv) > tiny<R>();
   for (int e = 0; e < 3; ++e)
-    c.p(P_FVEL + 3 * d + e)[at] = ok ? (double)(central8_r(wv[e]) / iv) : (double)NAN;
+    c.sp(M_FVEL + 3 * d + e)[at] = ok ? central8_r(wv[e]) / iv : (R)NAN;
 }
 
 __global__ void __launch_bounds__(128) k_edge_emf_fp32(K c, int e) {
@@ -1396,8 +1452,8 @@ __global__ void __launch_bounds__(128) k_edge_emf_fp32(K c, int e) {
     R wv[3][8];
     for (int n = 0; n < 8; ++n) {
       const long fa = at + (n - 4) * step[t2];
-      a1[n] = (R)c.p(P_FACE + FACE_PLANES * t1 + FO_AREA)[fa];
-      for (int m = 0; m < 3; ++m) wv[m][n] = a1[n] * (R)c.p(P_FVEL + 3 * t1 + m)[fa];
+      a1[n] = c.sp(M_FACE + M_FACE_PLANES * t1 + MF_AREA)[fa];
+      for (int m = 0; m < 3; ++m) wv[m][n] = a1[n] * c.sp(M_FVEL + 3 * t1 + m)[fa];
     }
     const R ea = central8_r(a1);
     if (!isfinite(ea) || fabs(ea) <= tiny<R>()) {
@@ -1416,17 +1472,17 @@ __global__ void __launch_bounds__(128) k_edge_emf_fp32(K c, int e) {
   R fl, fr, sl, sr;
   {
     R fld[8];
-    const double *bn = c.p(P_BN + t1);
-    for (int n = 0; n < 8; ++n) fld[n] = (R)bn[at + (n - 4) * step[t2]];
+    const float *bn = c.sp(M_BN + t1);
+    for (int n = 0; n < 8; ++n) fld[n] = bn[at + (n - 4) * step[t2]];
     up7_pdm_r(a1, central8_r(a1), fld, pdm, fl, fr);
   }
   {
     R a2[8], fld[8];
-    const double *bn = c.p(P_BN + t2);
+    const float *bn = c.sp(M_BN + t2);
     for (int n = 0; n < 8; ++n) {
       const long fa = at + (n - 4) * step[t1];
-      a2[n] = (R)c.p(P_FACE + FACE_PLANES * t2 + FO_AREA)[fa];
-      fld[n] = (R)bn[fa];
+      a2[n] = c.sp(M_FACE + M_FACE_PLANES * t2 + MF_AREA)[fa];
+      fld[n] = bn[fa];
     }
     const R ia = central8_r(a2);
     if (!isfinite(ia) || fabs(ia) <= tiny<R>()) {
@@ -1444,7 +1500,7 @@ __global__ void __launch_bounds__(128) k_edge_emf_fp32(K c, int e) {
   R rho = R(0);
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
-      rho += R(0.25) * (R)c.p(P_UPRED)[at - a * step[t1] - b * step[t2]];
+      rho += R(0.25) * c.sp(M_UPRED)[at - a * step[t1] - b * step[t2]];
   if (!isfinite(rho) || rho <= tiny<R>()) {
     c.fail(54, at);
     return;
@@ -1572,8 +1628,11 @@ __global__ void k_earth_emf_inner(K c) {
 // Fused stress divergence (nonorthogonal_sweep.c:550-622), Reynolds update
 // (nonorthogonal_step.c:542-553), CT on the cell's faces (:233-298), B
 // recovery (:565) and Boris correction (:571-606) of one active cell,
-// writing the new level.
+// writing the new level. With Mixed the face fluxes come from the FP32
+// planes (widened exactly); the arithmetic is FP64 either way.
+template <bool Mixed>
 __global__ void __launch_bounds__(128) k_update(K c) {
+  using Flux = typename std::conditional<Mixed, float, double>::type;
   const Layout &L = c.L;
   const int lo[3] = {L.ghost, L.ghost, L.ghost};
   const int hi[3] = {L.ghost + L.nr, L.ghost + L.n, L.ghost + L.n};
@@ -1590,16 +1649,16 @@ __global__ void __launch_bounds__(128) k_update(K c) {
   for (int q = 0; q < 5; ++q) {
     double diff = 0.0;
     for (int d = 0; d < 3; ++d) {
-      const double *fp = c.p(P_FLUID + 5 * d + q);
-      diff += fp[at] - fp[at + step[d]];
+      const Flux *fp = fluid_plane<Flux>(c, d, q);
+      diff += (double)fp[at] - (double)fp[at + step[d]];
     }
     rate[q] = diff * ivol;
   }
   for (int m = 0; m < 3; ++m) {
     double diff = 0.0;
     for (int d = 0; d < 3; ++d) {
-      const double *mp = c.p(P_MAXWELL + 3 * d + m);
-      diff += mp[at] - mp[at + step[d]];
+      const Flux *mp = maxwell_plane<Flux>(c, d, m);
+      diff += (double)mp[at] - (double)mp[at + step[d]];
     }
     mr[m] = diff * ivol;
     mr[m] += c.p(P_FORCE + m)[at];
@@ -1696,6 +1755,7 @@ static K make_k(Problem &P) {
   K c;
   c.L = P.L;
   c.pl = P.planes;
+  c.fl = P.planes32;
   c.by = P.bytes;
   c.sc = P.scalars;
   c.cur = P.level;
@@ -1823,7 +1883,10 @@ int enqueue_phase(Problem &P, Phase phase) {
   } else if (phase == PHASE_FLUX) {
     // cubedsphere_step.c:67-98
     mark_stage(P, S_PREDICT);
-    k_predict<<<nblocks(n_box, T), T, 0, s>>>(c);
+    if (P.mixed)
+      k_predict<true><<<nblocks(n_box, T), T, 0, s>>>(c);
+    else
+      k_predict<false><<<nblocks(n_box, T), T, 0, s>>>(c);
     for (int d = 0; d < 3; ++d) {
       const long n = 6L * (L.nr + (d == 0)) * (L.n + (d == 1)) * (L.n + (d == 2));
       mark_stage(P, S_FACE_FLUX0 + d);
@@ -1833,10 +1896,16 @@ int enqueue_phase(Problem &P, Phase phase) {
         k_face_flux<<<nblocks(n, 128), 128, 0, s>>>(c, d);
     }
     mark_stage(P, S_SYNC_FLUX);
-    k_sync_flux<<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams);
+    if (P.mixed)
+      k_sync_flux<float><<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams);
+    else
+      k_sync_flux<double><<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams);
     if (bd.inner_trap) {
       mark_stage(P, S_INNER_TRAP);
-      k_inner_trap<<<nblocks(6L * L.n * L.n, T), T, 0, s>>>(c);
+      if (P.mixed)
+        k_inner_trap<float><<<nblocks(6L * L.n * L.n, T), T, 0, s>>>(c);
+      else
+        k_inner_trap<double><<<nblocks(6L * L.n * L.n, T), T, 0, s>>>(c);
     }
   } else if (phase == PHASE_EMF) {
     // cubedsphere_step.c:99-126 up to the seam EMF average
@@ -1890,7 +1959,10 @@ int enqueue_phase(Problem &P, Phase phase) {
     k_sync_edges<<<nblocks(P.n_edge_groups, T), T, 0, s>>>(c, P.edge_groups, P.edge_members,
                                                            P.n_edge_groups);
     mark_stage(P, S_UPDATE);
-    k_update<<<nblocks(n_active, 128), 128, 0, s>>>(c);
+    if (P.mixed)
+      k_update<true><<<nblocks(n_active, 128), 128, 0, s>>>(c);
+    else
+      k_update<false><<<nblocks(n_active, 128), 128, 0, s>>>(c);
     // Final seam average of the new face flux, then B of the touched cells.
     K n = c;
     n.cur = c.old;
@@ -1915,6 +1987,30 @@ int enqueue_phase(Problem &P, Phase phase) {
   return 0;
 }
 
+// FP32 copies of the geometry the mixed form's kernels read (geometry and
+// background are fixed after setup, so this runs once per problem).
+__global__ void k_mixed_geometry(K c) {
+  const long t = tid();
+  if (t >= c.L.total) return;
+  c.sp(M_VOL)[t] = (float)c.p(P_VOL)[t];
+  for (int d = 0; d < 3; ++d) {
+    const int fb = P_FACE + FACE_PLANES * d, mb = M_FACE + M_FACE_PLANES * d;
+    c.sp(mb + MF_AREA)[t] = (float)c.p(fb + FO_AREA)[t];
+    for (int e = 0; e < 3; ++e) {
+      c.sp(mb + MF_N + e)[t] = (float)c.p(fb + FO_N + e)[t];
+      c.sp(mb + MF_T1 + e)[t] = (float)c.p(fb + FO_T1 + e)[t];
+      c.sp(mb + MF_T2 + e)[t] = (float)c.p(fb + FO_T2 + e)[t];
+      c.sp(mb + MF_B0 + e)[t] = (float)c.p(P_B0F + 3 * d + e)[t];
+    }
+    c.sp(mb + MF_B0N)[t] = (float)c.p(P_B0N + d)[t];
+  }
+}
+
+__global__ void k_widen(const float *src, double *dst, long n) {
+  const long t = tid();
+  if (t < n) dst[t] = (double)src[t];
+}
+
 // max over active cells of |net total (B1 + B0) face flux| / volume, the
 // reference driver's divergence gate (test_cubedsphere_dipole_background.c:
 // 808-845); non-negative doubles order like their bit patterns.
@@ -1933,6 +2029,36 @@ __global__ void k_div_b(K c, unsigned long long *out) {
   }
   const double v = fabs(net) / c.p(P_VOL)[at];
   atomicMax(out, (unsigned long long)__double_as_longlong(isfinite(v) ? v : DBL_MAX));
+}
+
+int prepare_mixed(Problem &P) {
+  if (P.planes32 != nullptr) return 0;
+  const size_t bytes = (size_t)PLANES32 * (size_t)P.L.total * sizeof(float);
+  if (hipMalloc((void **)&P.planes32, bytes) != hipSuccess) {
+    P.planes32 = nullptr;
+    set_error("mixed form: cannot allocate " + std::to_string(bytes >> 20) +
+              " MiB of FP32 planes");
+    return -1;
+  }
+  GCS_HIP(hipMemsetAsync(P.planes32, 0, bytes, P.stream));
+  k_mixed_geometry<<<nblocks(P.L.total, 256), 256, 0, P.stream>>>(make_k(P));
+  GCS_HIP(hipGetLastError());
+  GCS_HIP(hipStreamSynchronize(P.stream));
+  return 0;
+}
+
+int widen_mixed(Problem &P) {
+  if (P.planes32 == nullptr) return 0;
+  const long n = P.L.total;
+  // Contiguous runs with the same order in both plane tables.
+  const int runs[4][3] = {{M_UPRED, P_UPRED, 5}, {M_BPRED, P_BPRED, 3},
+                          {M_FLUID, P_FLUID, 15}, {M_MAXWELL, P_MAXWELL, 9}};
+  for (const auto &r : runs)
+    k_widen<<<nblocks(n * r[2], 256), 256, 0, P.stream>>>(P.planes32 + (long)r[0] * n,
+                                                         P.plane(r[1]), n * r[2]);
+  GCS_HIP(hipGetLastError());
+  GCS_HIP(hipStreamSynchronize(P.stream));
+  return 0;
 }
 
 int max_div_b(Problem &P, double *out) {
@@ -1954,6 +2080,11 @@ int max_div_b(Problem &P, double *out) {
 }
 
 int enqueue_step(Problem &P, bool explicit_dt) {
+  if (P.mixed && P.planes32 == nullptr) {
+    set_error("mixed form selected without its FP32 planes");
+    return -1;
+  }
+  P.last_step_mixed = P.mixed;
   K c = make_k(P);
   const Layout &L = P.L;
   hipStream_t s = P.stream;
