@@ -61,8 +61,8 @@ constexpr int P_SCRATCH_LINE = P_FLUID + 12;      // +3 edge line integrals of A
 // Problem::planes32 + q * layout.total (floats), allocated when the mixed
 // form is first selected. The FP32 kernels read their inputs and write
 // their results here at the precision they compute in, so the predictor's
-// caches, the face geometry they use, the face velocities and the face
-// fluxes cross HBM at 4 bytes instead of 8.
+// caches, the face and edge geometry they use, the face velocities and the
+// face fluxes cross HBM at 4 bytes instead of 8.
 constexpr int M_VOL = 0;                           // cell volume
 constexpr int M_WPRED = M_VOL + 1;                 // +5 (as P_WPRED)
 constexpr int M_BPRED = M_WPRED + 5;               // +3
@@ -77,7 +77,12 @@ constexpr int MF_AREA = 0, MF_N = 1, MF_T1 = 4, MF_T2 = 7, MF_B0 = 10, MF_B0N = 
 constexpr int M_FVEL = M_FACE + 3 * M_FACE_PLANES; // +9 face velocity, 3 per dir
 constexpr int M_FLUID = M_FVEL + 9;                // +15 area-scaled, 5 per dir
 constexpr int M_MAXWELL = M_FLUID + 15;            // +9, 3 per dir
-constexpr int PLANES32 = M_MAXWELL + 9;            // 95
+// Edges, direction e: M_EDGE + 14 e + {length, tangent1(3), tangent2(3),
+// transverse solve(5), B0 . (t1, t2)}.
+constexpr int M_EDGE = M_MAXWELL + 9;
+constexpr int ME_LEN = 0, ME_T1 = 1, ME_T2 = 4, ME_SOLVE = 7, ME_B0T = 12,
+              M_EDGE_PLANES = 14;
+constexpr int PLANES32 = M_EDGE + 3 * M_EDGE_PLANES;  // 137
 
 // Byte planes.
 constexpr int Q_VALID = 0;   // +3 edge validity (nonorthogonal_grid.c:391-403)

@@ -1446,7 +1446,7 @@ __global__ void __launch_bounds__(128) k_edge_emf_fp32(K c, int e) {
   }
   const long step[3] = {L.di(), (long)L.SK, 1};
   const StepOptions &o = c.o;
-  const int eb = P_EDGE + EDGE_PLANES * e;
+  const int eb = M_EDGE + M_EDGE_PLANES * e;
   R a1[8], v[3];
   {
     R wv[3][8];
@@ -1464,8 +1464,8 @@ __global__ void __launch_bounds__(128) k_edge_emf_fp32(K c, int e) {
   }
   R et1[3], et2[3];
   for (int m = 0; m < 3; ++m) {
-    et1[m] = (R)c.p(eb + EO_T1 + m)[at];
-    et2[m] = (R)c.p(eb + EO_T2 + m)[at];
+    et1[m] = c.sp(eb + ME_T1 + m)[at];
+    et2[m] = c.sp(eb + ME_T2 + m)[at];
   }
   const R v1 = dot3r(v, et1), v2 = dot3r(v, et2);
   const R pdm = (R)o.emf_pdm;
@@ -1492,10 +1492,10 @@ __global__ void __launch_bounds__(128) k_edge_emf_fp32(K c, int e) {
     up7_pdm_r(a2, ia, fld, pdm, sl, sr);
   }
   R solve[5], bt[2];
-  for (int m = 0; m < 5; ++m) solve[m] = (R)c.p(eb + EO_SOLVE + m)[at];
+  for (int m = 0; m < 5; ++m) solve[m] = c.sp(eb + ME_SOLVE + m)[at];
   edge_field_r(solve, R(0.5) * (fl + fr), R(0.5) * (sl + sr), bt);
-  bt[0] += (R)c.p(P_B0ET + 2 * e)[at];
-  bt[1] += (R)c.p(P_B0ET + 2 * e + 1)[at];
+  bt[0] += c.sp(eb + ME_B0T)[at];
+  bt[1] += c.sp(eb + ME_B0T + 1)[at];
   const R jump = (sr - sl) - (fr - fl);
   R rho = R(0);
   for (int a = 0; a < 2; ++a)
@@ -1506,7 +1506,7 @@ __global__ void __launch_bounds__(128) k_edge_emf_fp32(K c, int e) {
     return;
   }
   R emf;
-  if (!edge_emf_r(v1, v2, bt[0], bt[1], jump, rho, (R)c.p(eb + EO_LEN)[at], (R)o.emf_diff,
+  if (!edge_emf_r(v1, v2, bt[0], bt[1], jump, rho, c.sp(eb + ME_LEN)[at], (R)o.emf_diff,
                   o.emf_boris != 0, (R)o.emf_c, (R)o.emf_cfl, (R)c.sc->dt, emf)) {
     c.fail(55, at);
     return;
@@ -1987,8 +1987,9 @@ int enqueue_phase(Problem &P, Phase phase) {
   return 0;
 }
 
-// FP32 copies of the geometry the mixed form's kernels read (geometry and
-// background are fixed after setup, so this runs once per problem).
+// FP32 copies of the face and edge geometry the mixed form's kernels read
+// (geometry and background are fixed after setup, so this runs once per
+// problem).
 __global__ void k_mixed_geometry(K c) {
   const long t = tid();
   if (t >= c.L.total) return;
@@ -2003,6 +2004,14 @@ __global__ void k_mixed_geometry(K c) {
       c.sp(mb + MF_B0 + e)[t] = (float)c.p(P_B0F + 3 * d + e)[t];
     }
     c.sp(mb + MF_B0N)[t] = (float)c.p(P_B0N + d)[t];
+    const int eb = P_EDGE + EDGE_PLANES * d, me = M_EDGE + M_EDGE_PLANES * d;
+    c.sp(me + ME_LEN)[t] = (float)c.p(eb + EO_LEN)[t];
+    for (int e = 0; e < 3; ++e) {
+      c.sp(me + ME_T1 + e)[t] = (float)c.p(eb + EO_T1 + e)[t];
+      c.sp(me + ME_T2 + e)[t] = (float)c.p(eb + EO_T2 + e)[t];
+    }
+    for (int m = 0; m < 5; ++m) c.sp(me + ME_SOLVE + m)[t] = (float)c.p(eb + EO_SOLVE + m)[t];
+    for (int m = 0; m < 2; ++m) c.sp(me + ME_B0T + m)[t] = (float)c.p(P_B0ET + 2 * d + m)[t];
   }
 }
 

@@ -72,9 +72,9 @@ int gcs_problem_set_fidelity(gcs_problem *problem, int on);
  * form: Up7/PDM reconstruction, face fluxes, face velocities and edge EMFs
  * computed in FP32, with the conserved-variable update, CT, CFL reduction
  * and ionosphere solve in FP64. The predictor's primitive caches, the face
- * geometry the FP32 kernels read, the face velocities and the face fluxes
- * are held in FP32 arrays (allocated on the first switch, 95 floats per
- * grid point); edge EMFs stay FP64. Downloads of the predicted state and
+ * and edge geometry the FP32 kernels read, the face velocities and the face
+ * fluxes are held in FP32 arrays (allocated on the first switch, 137 floats
+ * per grid point); edge EMFs stay FP64. Downloads of the predicted state and
  * the face fluxes widen the FP32 values. No reference counterpart. */
 int gcs_problem_set_precision(gcs_problem *problem, int mixed);
 
