@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "gamera_cs_dropin.h"
+#include "gamera_cs_mpi_dropin.h"
 #include "gcs_device.hpp"
 #include "gcs_transfer.hpp"
 
@@ -667,6 +668,70 @@ int gamera_cs_advance_six_with_background_planned(
   return gcs::advance(grid, storage, active, ghost, predictor_ratio, dt, options,
                       background, radial_boundary, current_time, previous_time,
                       boundary_context);
+}
+
+// cubedsphere/cubedsphere_mpi_step.h:36-91: the six-face MPI entry points
+// step the whole cube on this process's GPU; the communicator and the plans
+// are not read (include/gamera_cs_mpi_dropin.h).
+int gamera_cs_mpi_advance_six_with_background_planned(
+    const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+    gamera_no_storage storage[GAMERA_CS_FACE_COUNT], const size_t active[3],
+    size_t ghost, double predictor_ratio, double dt,
+    const gamera_no_advance_options *options,
+    const gamera_no_background_field background[GAMERA_CS_FACE_COUNT],
+    const gamera_cs_halo_plan *halo_plan,
+    gamera_cs_radial_boundary_function radial_boundary, double current_time,
+    double previous_time, void *boundary_context, gcs_mpi_comm communicator) {
+  (void)halo_plan;
+  (void)communicator;
+  return gcs::advance(grid, storage, active, ghost, predictor_ratio, dt, options,
+                      background, radial_boundary, current_time, previous_time,
+                      boundary_context);
+}
+
+int gamera_cs_mpi_advance_six_with_background_planned_cached_edges(
+    const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+    gamera_no_storage storage[GAMERA_CS_FACE_COUNT], const size_t active[3],
+    size_t ghost, double predictor_ratio, double dt,
+    const gamera_no_advance_options *options,
+    const gamera_no_background_field background[GAMERA_CS_FACE_COUNT],
+    const gamera_cs_halo_plan *halo_plan, const void *edge_plan,
+    gamera_cs_radial_boundary_function radial_boundary, double current_time,
+    double previous_time, void *boundary_context, gcs_mpi_comm communicator) {
+  (void)edge_plan;
+  return gamera_cs_mpi_advance_six_with_background_planned(
+      grid, storage, active, ghost, predictor_ratio, dt, options, background, halo_plan,
+      radial_boundary, current_time, previous_time, boundary_context, communicator);
+}
+
+int gamera_cs_mpi_advance_six_with_background_planned_cached_edges_pre_synced(
+    const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+    gamera_no_storage storage[GAMERA_CS_FACE_COUNT], const size_t active[3],
+    size_t ghost, double predictor_ratio, double dt,
+    const gamera_no_advance_options *options,
+    const gamera_no_background_field background[GAMERA_CS_FACE_COUNT],
+    const gamera_cs_halo_plan *halo_plan, const void *edge_plan,
+    gamera_cs_radial_boundary_function radial_boundary, double current_time,
+    double previous_time, void *boundary_context, gcs_mpi_comm communicator) {
+  (void)edge_plan;
+  return gamera_cs_mpi_advance_six_with_background_planned(
+      grid, storage, active, ghost, predictor_ratio, dt, options, background, halo_plan,
+      radial_boundary, current_time, previous_time, boundary_context, communicator);
+}
+
+int gamera_cs_mpi_advance_six_with_background_sparse_pre_synced(
+    const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+    gamera_no_storage storage[GAMERA_CS_FACE_COUNT], const size_t active[3],
+    size_t ghost, double predictor_ratio, double dt,
+    const gamera_no_advance_options *options,
+    const gamera_no_background_field background[GAMERA_CS_FACE_COUNT],
+    const gamera_cs_halo_plan *halo_plan, const void *edge_plan,
+    gamera_cs_radial_boundary_function radial_boundary, double current_time,
+    double previous_time, void *boundary_context, gcs_mpi_comm communicator) {
+  (void)edge_plan;
+  return gamera_cs_mpi_advance_six_with_background_planned(
+      grid, storage, active, ghost, predictor_ratio, dt, options, background, halo_plan,
+      radial_boundary, current_time, previous_time, boundary_context, communicator);
 }
 
 int gamera_cs_advance_six_with_background(

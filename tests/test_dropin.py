@@ -84,13 +84,16 @@ def release(gpu_lib, ref_lib, p):
                                ref_lib.refh_storage(p._handle))
 
 
-def run_pair(ref_lib, gpu, name, steps, wrapped):
+def run_pair(ref_lib, gpu, name, steps, wrapped, step=None):
+    """`step`: address of a function with the planned entry point's
+    prototype (default: libgcs.so's own)."""
     lib = gpu.library()
     cfg = gcs_config.workload(name)
     a = problem.Problem(ref_lib, "refh", cfg)
     b = problem.Problem(ref_lib, "refh", cfg)
-    ref_lib.refh_set_step(b._handle,
-                          address(lib, "gamera_cs_advance_six_with_background_planned"))
+    if step is None:
+        step = address(lib, "gamera_cs_advance_six_with_background_planned")
+    ref_lib.refh_set_step(b._handle, step)
     if wrapped:
         ref_lib.refh_use_wrapped_callbacks(a._handle, 1)
         ref_lib.refh_use_wrapped_callbacks(b._handle, 1)
@@ -115,6 +118,50 @@ def run_pair(ref_lib, gpu, name, steps, wrapped):
 def test_dropin_matches_reference_step(ref_global, gpu, name, steps):
     """Earth callbacks recognised by address and run as device kernels."""
     run_pair(ref_global, gpu, name, steps, wrapped=False)
+
+
+# The planned entry point's prototype (cubedsphere_step.h:58-67).
+PLANNED = ctypes.CFUNCTYPE(
+    ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+    ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+    ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_void_p)
+
+MPI_ENTRY_POINTS = [
+    "gamera_cs_mpi_advance_six_with_background_planned",
+    "gamera_cs_mpi_advance_six_with_background_planned_cached_edges",
+    "gamera_cs_mpi_advance_six_with_background_planned_cached_edges_pre_synced",
+    "gamera_cs_mpi_advance_six_with_background_sparse_pre_synced",
+]
+
+
+def mpi_as_planned(lib, name):
+    """The MPI entry point `name` (include/gamera_cs_mpi_dropin.h) behind
+    the planned prototype, with no edge plan and the one-process shim
+    communicator."""
+    fn = getattr(lib, name)
+    edge_plan = name != MPI_ENTRY_POINTS[0]
+    v = ctypes.c_void_p
+    fn.restype = ctypes.c_int
+    fn.argtypes = ([v, v, v, ctypes.c_size_t, ctypes.c_double, ctypes.c_double, v, v, v]
+                   + ([v] if edge_plan else [])
+                   + [v, ctypes.c_double, ctypes.c_double, v, v])
+
+    def step(grid, storage, active, ghost, ratio, dt, options, background, plan, radial,
+             t, t_prev, context):
+        extra = [None] if edge_plan else []
+        return fn(grid, storage, active, ghost, ratio, dt, options, background, plan,
+                  *extra, radial, t, t_prev, context, None)
+    return PLANNED(step)
+
+
+@pytest.mark.parametrize("entry", MPI_ENTRY_POINTS)
+def test_mpi_entry_points_match_reference_step(ref_global, gpu, entry):
+    """cubedsphere_mpi_step.h:36-91 with a one-process communicator: the
+    serial step's results (the reference gates its MPI step on exact
+    agreement with the serial one)."""
+    adapter = mpi_as_planned(gpu.library(), entry)
+    run_pair(ref_global, gpu, "small", 3, wrapped=False,
+             step=ctypes.cast(adapter, ctypes.c_void_p).value)
 
 
 def test_dropin_runs_unknown_callbacks_on_host(ref_global, gpu):
