@@ -140,11 +140,11 @@ GpuProblem.mi_field = _mi_field
 
 
 def _set_precision(self, mixed):
-    """False: FP64 throughout. True: FP32 reconstruction/fluxes/EMF, FP64
-    update, CT, CFL and ionosphere."""
+    """False: FP64 throughout. True: FP32 reconstruction/fluxes/EMF with
+    FP32 intermediate arrays, FP64 update, CT, CFL and ionosphere."""
     self._lib.gcs_problem_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int]
     if self._lib.gcs_problem_set_precision(self._handle, 1 if mixed else 0) != 0:
-        raise RuntimeError("gcs_problem_set_precision failed")
+        raise RuntimeError(f"gcs_problem_set_precision: {last_error()}")
 
 
 GpuProblem.set_precision = _set_precision

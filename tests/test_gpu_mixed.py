@@ -51,3 +51,33 @@ def test_mixed_step_close_to_fp64(gpu, name, steps):
     div = max_divergence(b, cfg)
     print(f"  max |div B| (mixed) {div:.2e}", flush=True)
     assert div < 1e-9
+
+
+def test_mixed_downloads_widen_fp32_arrays(gpu):
+    """The mixed form keeps the predicted state and the face fluxes in FP32
+    arrays; downloads widen them, so they track the FP64 step's arrays to
+    FP32 accuracy (and are not the stale FP64 arrays)."""
+    cfg = gcs_config.workload("small")
+    a = gpu.GpuProblem(cfg)
+    b = gpu.GpuProblem(cfg)
+    b.set_precision(True)
+    a.advance(10)
+    b.advance(10)
+    fields = ([problem.PREDICTED_CONSERVED, problem.PREDICTED_CELL_MAGNETIC]
+              + [problem.FLUID_FACE_FLUX + d for d in range(3)]
+              + [problem.MAXWELL_FACE_FLUX + d for d in range(3)])
+    for fid in fields:
+        worst = 0.0
+        for f in range(gcs_config.FACE_COUNT):
+            x, y = a.field(f, fid), b.field(f, fid)
+            scale = np.abs(x).max()
+            assert scale > 0.0, fid
+            worst = max(worst, float(np.abs(x - y).max() / scale))
+        print(f"  field {fid}: max |mixed - fp64| / max |fp64| = {worst:.2e}", flush=True)
+        assert worst <= 1e-4, fid
+    # switching back to FP64 downloads the FP64 arrays again
+    b.set_precision(False)
+    b.advance(1)
+    a.advance(1)
+    x, y = a.field(0, problem.PREDICTED_CONSERVED), b.field(0, problem.PREDICTED_CONSERVED)
+    assert float(np.abs(x - y).max() / np.abs(x).max()) <= 1e-4
