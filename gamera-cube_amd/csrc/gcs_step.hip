@@ -856,25 +856,20 @@ __global__ void k_predict(K c) {
 
 // ---- face fluxes -----------------------------------------------------------------
 
-// nonorthogonal_sweep.c:408-548 for faces of direction d bounding the
+// nonorthogonal_sweep.c:408-548 for one face of direction d bounding the
 // active box: 8-cell Up7/PDM of the predicted state, kinetic fluid and
-// Maxwell fluxes, HOGS, emergency diffusion, area scaling.
-__global__ void __launch_bounds__(128) k_face_flux(K c, int d) {
-  const Layout &L = c.L;
-  const int lo[3] = {L.ghost, L.ghost, L.ghost};
-  const int hi[3] = {L.ghost + L.nr + (d == 0), L.ghost + L.n + (d == 1),
-                     L.ghost + L.n + (d == 2)};
-  int f, i, j, k;
-  if (!box(tid(), lo, hi, f, i, j, k)) return;
-  const long at = L.at(f, i, j, k);
-  const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
+// Maxwell fluxes, HOGS, emergency diffusion, area scaling. cell(v, n) is
+// stencil cell n (n - 4 cells along d from the face) of variable v: 0-4 the
+// predicted primitives (P_WPRED), 5 the volume, 6-8 the predicted cell B.
+template <class Cell>
+GCS_DEV void face_flux_face(const K &c, int d, int f, long at, long st, const Cell &cell) {
   const StepOptions &o = c.o;
   // Primitives of the eight stencil cells come from the predictor's cache;
   // one variable at a time keeps a single 8-point column live.
   double q8[8];
   bool converted = true;
   for (int n = 0; n < 8; ++n) {
-    q8[n] = c.p(P_WPRED)[at + (n - 4) * st];
+    q8[n] = cell(0, n);
     converted &= isfinite(q8[n]);
   }
   if (!converted) {
@@ -882,7 +877,7 @@ __global__ void __launch_bounds__(128) k_face_flux(K c, int d) {
     return;
   }
   double vol[8];
-  for (int n = 0; n < 8; ++n) vol[n] = c.p(P_VOL)[at + (n - 4) * st];
+  for (int n = 0; n < 8; ++n) vol[n] = cell(5, n);
   const double iv = central8(vol);
   if (!isfinite(iv) || fabs(iv) <= DBL_MIN) {
     c.fail(42, at);
@@ -891,15 +886,15 @@ __global__ void __launch_bounds__(128) k_face_flux(K c, int d) {
   Prim w[2];
   up7_pdm(vol, iv, q8, o.pdm, w[0].rho, w[1].rho);
   for (int e = 0; e < 3; ++e) {
-    for (int n = 0; n < 8; ++n) q8[n] = c.p(P_WPRED + 1 + e)[at + (n - 4) * st];
+    for (int n = 0; n < 8; ++n) q8[n] = cell(1 + e, n);
     up7_pdm(vol, iv, q8, o.pdm, w[0].v[e], w[1].v[e]);
   }
-  for (int n = 0; n < 8; ++n) q8[n] = c.p(P_WPRED + 4)[at + (n - 4) * st];
+  for (int n = 0; n < 8; ++n) q8[n] = cell(4, n);
   up7_pdm(vol, iv, q8, o.pdm, w[0].p, w[1].p);
   double bb[2][3];
   for (int e = 0; e < 3; ++e) {
     double q8[8];
-    for (int n = 0; n < 8; ++n) q8[n] = c.p(P_BPRED + e)[at + (n - 4) * st];
+    for (int n = 0; n < 8; ++n) q8[n] = cell(6 + e, n);
     up7_pdm(vol, iv, q8, o.pdm, bb[0][e], bb[1][e]);
   }
   if (!valid_interface(w[0].rho, w[0].v, w[0].p) ||
@@ -953,6 +948,114 @@ __global__ void __launch_bounds__(128) k_face_flux(K c, int d) {
   for (int q = 0; q < 5; ++q) c.p(P_FLUID + 5 * d + q)[at] = area * fl.f[q];
   for (int e = 0; e < 3; ++e) c.p(P_MAXWELL + 3 * d + e)[at] = area * mx.m[e];
 }
+
+// The face flux with the stencil read from global memory (one face per
+// thread, 72 stencil loads per face).
+__global__ void __launch_bounds__(128) k_face_flux(K c, int d) {
+  const Layout &L = c.L;
+  const int lo[3] = {L.ghost, L.ghost, L.ghost};
+  const int hi[3] = {L.ghost + L.nr + (d == 0), L.ghost + L.n + (d == 1),
+                     L.ghost + L.n + (d == 2)};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
+  face_flux_face(c, d, f, at, st, [&](int v, int n) {
+    const int plane = v < 5 ? P_WPRED + v : v == 5 ? P_VOL : P_BPRED + v - 6;
+    return c.p(plane)[at + (n - 4) * st];
+  });
+}
+
+// LDS tiles of the face flux of direction D: TD faces along D by TT along a
+// transverse axis (k, or j for D = 2) at one position of the third axis.
+// The block stages the nine stencil variables of its cells (TD + 7 along D)
+// once, so each face reads its 8 x 9 stencil values from LDS instead of
+// issuing 72 global loads; faces are ordered so that a wave's faces are
+// consecutive in k (coalesced geometry reads and flux writes). The
+// arithmetic is face_flux_face's, bit for bit.
+template <int D, int TD_, int TT_>
+struct FluxTile {
+  static constexpr int TD = TD_, TT = TT_, kThreads = TD * TT;
+  static constexpr int CD = TD + 7;               // stencil cells along D
+  static constexpr int TA = D == 2 ? 1 : 2;       // transverse axis
+  static constexpr int XA = D == 0 ? 1 : 0;       // fixed axis
+  static constexpr int kVars = 9;
+  static constexpr int kSlots = CD * TT;
+  // LDS slot of cell cd (along D) at transverse position t: consecutive
+  // lanes of a wave read consecutive slots.
+  GCS_DEV static int slot(int cd, int t) { return D == 2 ? t * CD + cd : cd * TT + t; }
+  static constexpr int kStep = D == 2 ? 1 : TT;   // slot step along D
+  // blocks for a layout with nr radial and n angular active cells
+  static unsigned blocks(int nr, int n) {
+    const int act[3] = {nr, n, n};
+    return (unsigned)(6L * act[XA] * ((act[D] + 1 + TD - 1) / TD) * ((act[TA] + TT - 1) / TT));
+  }
+};
+
+template <int D, int TD_, int TT_>
+__global__ void __launch_bounds__(TD_ * TT_) k_face_flux_tiled(K c) {
+  using T = FluxTile<D, TD_, TT_>;
+  constexpr int TD = T::TD, TT = T::TT, CD = T::CD, TA = T::TA, XA = T::XA;
+  constexpr int kSlots = T::kSlots;
+  __shared__ double s_cell[T::kVars * kSlots];
+  const Layout &L = c.L;
+  const int act[3] = {L.nr, L.n, L.n};
+  const int nd = act[D] + 1, nt = act[TA], nx = act[XA];
+  const int ntd = (nd + TD - 1) / TD, ntt = (nt + TT - 1) / TT;
+  unsigned b = blockIdx.x;
+  const int tt = (int)(b % (unsigned)ntt);
+  b /= (unsigned)ntt;
+  const int td = (int)(b % (unsigned)ntd);
+  b /= (unsigned)ntd;
+  const int x = (int)(b % (unsigned)nx);
+  const int f = (int)(b / (unsigned)nx);
+  if (f >= 6) return;
+  const int d0 = td * TD, t0 = tt * TT;             // tile origin (active-relative)
+  const int nvd = min(TD, nd - d0), nvt = min(TT, nt - t0);
+  auto at_of = [&](int ad, int at2) {
+    int p[3];
+    p[D] = L.ghost + ad;
+    p[TA] = L.ghost + at2;
+    p[XA] = L.ghost + x;
+    return L.at(f, p[0], p[1], p[2]);
+  };
+  // Stencil cells of the tile's valid faces: along D from d0 - 4 to
+  // d0 + nvd + 2 (inside the ghost layers), memory-contiguous index fastest.
+  // Every load of the fill is issued before the first LDS write.
+  constexpr int kFill = (kSlots + T::kThreads - 1) / T::kThreads;
+  double fill[kFill][T::kVars];
+  int slot[kFill];
+#pragma unroll
+  for (int r = 0; r < kFill; ++r) {
+    const int q = (int)threadIdx.x + r * T::kThreads;
+    const int cd = D == 2 ? q % CD : q / TT;
+    const int t = D == 2 ? q / CD : q % TT;
+    slot[r] = -1;
+    if (q >= kSlots || cd >= nvd + 7 || t >= nvt) continue;
+    const long cell = at_of(d0 - 4 + cd, t0 + t);
+    slot[r] = T::slot(cd, t);
+    for (int v = 0; v < 5; ++v) fill[r][v] = c.p(P_WPRED + v)[cell];
+    fill[r][5] = c.p(P_VOL)[cell];
+    for (int v = 0; v < 3; ++v) fill[r][6 + v] = c.p(P_BPRED + v)[cell];
+  }
+#pragma unroll
+  for (int r = 0; r < kFill; ++r)
+    if (slot[r] >= 0)
+      for (int v = 0; v < T::kVars; ++v) s_cell[v * kSlots + slot[r]] = fill[r][v];
+  __syncthreads();
+  const int a = D == 2 ? (int)threadIdx.x % TD : (int)threadIdx.x / TT;
+  const int t = D == 2 ? (int)threadIdx.x / TD : (int)threadIdx.x % TT;
+  if (a >= nvd || t >= nvt) return;
+  const long at = at_of(d0 + a, t0 + t);
+  const long st = D == 0 ? L.di() : D == 1 ? (long)L.SK : 1;
+  const double *base = s_cell + T::slot(a, t);
+  face_flux_face(c, D, f, at, st,
+                 [&](int v, int n) { return base[v * kSlots + n * T::kStep]; });
+}
+
+// Tile shapes of the FP64 face flux per direction.
+using FluxTileI = FluxTile<0, 8, 16>;
+using FluxTileJ = FluxTile<1, 8, 16>;
 
 // ---- fused face velocity + edge EMF, LDS tiles ------------------------------------
 //
@@ -1301,24 +1404,19 @@ __global__ void __launch_bounds__(128) k_edge_emf(K c, int e) {
 // already rounded; the face fluxes are stored to FP32 planes and the edge
 // EMF to its FP64 plane, so the update, CT, CFL and ionosphere stay FP64.
 
-__global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
+// k_face_flux's face in FP32 (same stencil variables as face_flux_face,
+// read from the FP32 planes).
+template <class Cell>
+GCS_DEV void face_flux_face_fp32(const K &c, int d, int f, long at, long st, const Cell &cell) {
   using R = float;
   using namespace mixed;
-  const Layout &L = c.L;
-  const int lo[3] = {L.ghost, L.ghost, L.ghost};
-  const int hi[3] = {L.ghost + L.nr + (d == 0), L.ghost + L.n + (d == 1),
-                     L.ghost + L.n + (d == 2)};
-  int f, i, j, k;
-  if (!box(tid(), lo, hi, f, i, j, k)) return;
-  const long at = L.at(f, i, j, k);
-  const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
   const StepOptions &o = c.o;
   const R gamma = (R)o.gamma, pdm = (R)o.pdm;
   // Primitives of the stencil from the predictor's FP32 cache.
   R q8[8];
   bool converted = true;
   for (int n = 0; n < 8; ++n) {
-    q8[n] = c.sp(M_WPRED)[at + (n - 4) * st];
+    q8[n] = cell(0, n);
     converted &= isfinite(q8[n]);
   }
   if (!converted) {
@@ -1326,7 +1424,7 @@ __global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
     return;
   }
   R vol[8];
-  for (int n = 0; n < 8; ++n) vol[n] = c.sp(M_VOL)[at + (n - 4) * st];
+  for (int n = 0; n < 8; ++n) vol[n] = cell(5, n);
   const R iv = central8_r(vol);
   if (!isfinite(iv) || fabs(iv) <= tiny<R>()) {
     c.fail(42, at);
@@ -1335,15 +1433,15 @@ __global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
   PrimR<R> w[2];
   up7_pdm_r(vol, iv, q8, pdm, w[0].rho, w[1].rho);
   for (int e = 0; e < 3; ++e) {
-    for (int n = 0; n < 8; ++n) q8[n] = c.sp(M_WPRED + 1 + e)[at + (n - 4) * st];
+    for (int n = 0; n < 8; ++n) q8[n] = cell(1 + e, n);
     up7_pdm_r(vol, iv, q8, pdm, w[0].v[e], w[1].v[e]);
   }
-  for (int n = 0; n < 8; ++n) q8[n] = c.sp(M_WPRED + 4)[at + (n - 4) * st];
+  for (int n = 0; n < 8; ++n) q8[n] = cell(4, n);
   up7_pdm_r(vol, iv, q8, pdm, w[0].p, w[1].p);
   R bb[2][3];
   for (int e = 0; e < 3; ++e) {
     R q8[8];
-    for (int n = 0; n < 8; ++n) q8[n] = c.sp(M_BPRED + e)[at + (n - 4) * st];
+    for (int n = 0; n < 8; ++n) q8[n] = cell(6 + e, n);
     up7_pdm_r(vol, iv, q8, pdm, bb[0][e], bb[1][e]);
   }
   if (!valid_interface_r(w[0].rho, w[0].v, w[0].p) ||
@@ -1395,6 +1493,21 @@ __global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
   }
   for (int q = 0; q < 5; ++q) c.sp(M_FLUID + 5 * d + q)[at] = area * fl.f[q];
   for (int e = 0; e < 3; ++e) c.sp(M_MAXWELL + 3 * d + e)[at] = area * mx.m[e];
+}
+
+__global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
+  const Layout &L = c.L;
+  const int lo[3] = {L.ghost, L.ghost, L.ghost};
+  const int hi[3] = {L.ghost + L.nr + (d == 0), L.ghost + L.n + (d == 1),
+                     L.ghost + L.n + (d == 2)};
+  int f, i, j, k;
+  if (!box(tid(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
+  face_flux_face_fp32(c, d, f, at, st, [&](int v, int n) {
+    const int plane = v < 5 ? M_WPRED + v : v == 5 ? M_VOL : M_BPRED + v - 6;
+    return c.sp(plane)[at + (n - 4) * st];
+  });
 }
 
 __global__ void k_face_velocity_fp32(K c, int d) {
@@ -1890,8 +2003,17 @@ int enqueue_phase(Problem &P, Phase phase) {
     for (int d = 0; d < 3; ++d) {
       const long n = 6L * (L.nr + (d == 0)) * (L.n + (d == 1)) * (L.n + (d == 2));
       mark_stage(P, S_FACE_FLUX0 + d);
+      // FP64: LDS tiles along i and j; the k stencil is contiguous and stays
+      // on the global-load kernel. The FP32 kernels read half the bytes and
+      // are faster without tiles.
       if (P.mixed)
         k_face_flux_fp32<<<nblocks(n, 128), 128, 0, s>>>(c, d);
+      else if (d == 0)
+        k_face_flux_tiled<0, FluxTileI::TD, FluxTileI::TT>
+            <<<FluxTileI::blocks(L.nr, L.n), FluxTileI::kThreads, 0, s>>>(c);
+      else if (d == 1)
+        k_face_flux_tiled<1, FluxTileJ::TD, FluxTileJ::TT>
+            <<<FluxTileJ::blocks(L.nr, L.n), FluxTileJ::kThreads, 0, s>>>(c);
       else
         k_face_flux<<<nblocks(n, 128), 128, 0, s>>>(c, d);
     }
