@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <vector>
@@ -1528,9 +1529,14 @@ int mi_create(Problem &P) {
   GCS_HIP(hipMalloc(&m.potential, polar * sizeof(double)));
   GCS_HIP(hipMalloc(&m.work, 2L * kVectors * m.n * sizeof(double)));
   {
-    // the cooperative BiCGStab needs all 2 kCoopBlocks workgroups resident
+    // the cooperative BiCGStab needs all 2 kCoopBlocks workgroups resident;
+    // GCS_MI_COOPERATIVE=0 selects the single-workgroup solve instead (for
+    // tools that cannot follow a cooperative launch, such as the kernel
+    // tracer of rocprofv3 7.2, which faults at exit after one)
+    const char *env = std::getenv("GCS_MI_COOPERATIVE");
+    const bool allowed = env == nullptr || std::strcmp(env, "0") != 0;
     int dev = 0, coop = 0, per_cu = 0, cus = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
+    if (allowed && hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess &&
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mi_bicgstab_pair, kSolveThreads,

@@ -129,12 +129,12 @@ GCS_DEV long tid() { return blockIdx.x * (long)blockDim.x + threadIdx.x; }
 // stencil neighbours of a block in the same L2. (A bijection of [0, nb).)
 // Used by the face-velocity and edge-EMF sweeps, whose 8-point stencils
 // reach across k-rows; plan-driven kernels keep the plain order.
-GCS_DEV long tid_xcd() {
+GCS_DEV unsigned block_xcd() {
   const unsigned nb = gridDim.x, b = blockIdx.x;
   const unsigned q = nb >> 3, r = nb & 7, x = b & 7, slot = b >> 3;
-  const unsigned lb = x * q + min(x, r) + slot;
-  return lb * (long)blockDim.x + threadIdx.x;
+  return x * q + min(x, r) + slot;
 }
+GCS_DEV long tid_xcd() { return block_xcd() * (long)blockDim.x + threadIdx.x; }
 
 constexpr int kCflThreads = 256;
 
@@ -1002,11 +1002,13 @@ __global__ void __launch_bounds__(TD_ * TT_) k_face_flux_tiled(K c) {
   const int act[3] = {L.nr, L.n, L.n};
   const int nd = act[D] + 1, nt = act[TA], nx = act[XA];
   const int ntd = (nd + TD - 1) / TD, ntt = (nt + TT - 1) / TT;
-  unsigned b = blockIdx.x;
-  const int tt = (int)(b % (unsigned)ntt);
-  b /= (unsigned)ntt;
+  // Tiles that share stencil cells (neighbours along D) are consecutive
+  // logical blocks, and consecutive logical blocks run on one XCD (one L2).
+  unsigned b = block_xcd();
   const int td = (int)(b % (unsigned)ntd);
   b /= (unsigned)ntd;
+  const int tt = (int)(b % (unsigned)ntt);
+  b /= (unsigned)ntt;
   const int x = (int)(b % (unsigned)nx);
   const int f = (int)(b / (unsigned)nx);
   if (f >= 6) return;
