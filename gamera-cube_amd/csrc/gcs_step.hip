@@ -323,19 +323,19 @@ GCS_DEV double block_sum(const double *plane, int donor, int sk,
 }
 
 // cubedsphere_halo.c:582-632: tangential ghost cells, both levels.
-__global__ void k_halo_cells(K c, const HaloCell *e, int n) {
+// Ghost cell B is written for the storage only: no step kernel reads the
+// cell B of a ghost cell (the predictor recovers B* from face fluxes, halo
+// donors and the MI shell are active cells), so without `with_b` (the
+// problem API's fidelity 0) only U is filled.
+__global__ void k_halo_cells(K c, const HaloCell *e, int n, int with_b) {
   const long t = tid();
   if (t >= n) return;
   const HaloCell h = e[t];
   const int sk = c.L.SK;
-  double cu[5], ou[5], cb[3], ob[3];
+  double cu[5], ou[5];
   for (int q = 0; q < 5; ++q) {
     cu[q] = block_sum(c.p(plane_u(c.cur, q)), h.donor, sk, h.w);
     ou[q] = block_sum(c.p(plane_u(c.old, q)), h.donor, sk, h.w);
-  }
-  for (int d = 0; d < 3; ++d) {
-    cb[d] = block_sum(c.p(plane_b(c.cur, d)), h.donor, sk, h.w);
-    ob[d] = block_sum(c.p(plane_b(c.old, d)), h.donor, sk, h.w);
   }
   // Nearest-donor fallback when either interpolated level is non-physical.
   const bool ok = halo_valid(cu) & halo_valid(ou);
@@ -345,10 +345,13 @@ __global__ void k_halo_cells(K c, const HaloCell *e, int n) {
     c.p(plane_u(c.cur, q))[h.target] = ok ? cu[q] : nc;
     c.p(plane_u(c.old, q))[h.target] = ok ? ou[q] : no;
   }
+  if (!with_b) return;
   for (int d = 0; d < 3; ++d) {
+    const double cb = block_sum(c.p(plane_b(c.cur, d)), h.donor, sk, h.w);
+    const double ob = block_sum(c.p(plane_b(c.old, d)), h.donor, sk, h.w);
     const double nc = c.p(plane_b(c.cur, d))[src], no = c.p(plane_b(c.old, d))[src];
-    c.p(plane_b(c.cur, d))[h.target] = ok ? cb[d] : nc;
-    c.p(plane_b(c.old, d))[h.target] = ok ? ob[d] : no;
+    c.p(plane_b(c.cur, d))[h.target] = ok ? cb : nc;
+    c.p(plane_b(c.old, d))[h.target] = ok ? ob : no;
   }
 }
 
@@ -1965,7 +1968,8 @@ int enqueue_phase(Problem &P, Phase phase) {
     mark_stage(P, S_SYNC_PHI);
     k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(c, P.seams, P.n_seams, 1);
     mark_stage(P, S_HALO_CELLS);
-    k_halo_cells<<<nblocks(P.n_halo_cells, T), T, 0, s>>>(c, P.halo_cells, P.n_halo_cells);
+    k_halo_cells<<<nblocks(P.n_halo_cells, T), T, 0, s>>>(c, P.halo_cells, P.n_halo_cells,
+                                                         P.fidelity ? 1 : 0);
     if (bd.outflow) {
       mark_stage(P, S_OUTFLOW);
       k_radial_outflow<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c);
@@ -1985,14 +1989,16 @@ int enqueue_phase(Problem &P, Phase phase) {
       // cubedsphere_earth_boundary.c:169 recovers every cell of both levels.
       // After a completed step the six face fluxes of an active cell are
       // those its B was last recovered from (the step ends with the seam
-      // sync and a recovery), so only the non-active cells change.
+      // sync and a recovery), so only the non-active cells change; and
+      // ghost-cell B is never read by the step (see k_halo_cells), so with
+      // fidelity 0 only the first step's recovery of every cell runs.
       for (int lev : {c.cur, c.old}) {
-        if (P.b_recovered)
-          k_recover_ghosts<<<nblocks(n_cells - n_active, T), T, 0, s>>>(c, plane_phi(lev, 0),
-                                                                        plane_b(lev, 0), 1);
-        else
+        if (!P.b_recovered)
           k_recover<<<nblocks(n_cells, T), T, 0, s>>>(c, plane_phi(lev, 0), plane_b(lev, 0), 1,
                                                       0);
+        else if (P.fidelity)
+          k_recover_ghosts<<<nblocks(n_cells - n_active, T), T, 0, s>>>(c, plane_phi(lev, 0),
+                                                                        plane_b(lev, 0), 1);
       }
     }
   } else if (phase == PHASE_FLUX) {
