@@ -63,9 +63,12 @@ void gcs_problem_counters(gcs_problem *problem, int face, double out[2]);
 
 /* 1 (default): also reproduce the ghost values the reference leaves in
  * storage after a step (ghost U and face flux keep their start-of-step
- * values, ghost B is recovered from them). 0: skip that work; every
- * ghost value is rewritten at the start of the next step, so results in
- * the active domain are unchanged. No reference counterpart. */
+ * values, ghost B is recovered from them). 0: skip that work and the
+ * ghost-cell B of the halo fill and of the Earth boundary's recovery (no
+ * step computation reads ghost-cell B); ghost U and face fluxes are
+ * rewritten at the start of each step, ghost B is not maintained, and the
+ * active state, clock and ionosphere potential are bit-identical to
+ * fidelity 1's. No reference counterpart. */
 int gcs_problem_set_fidelity(gcs_problem *problem, int on);
 
 /* 0 (default): FP64 throughout, the drop-in's arithmetic. 1: the mixed
