@@ -287,6 +287,7 @@ namespace gcs {
 static int launch_step_graph(Problem &P) {
   Problem::StepGraph &g = P.graphs[P.level];
   const bool match = g.exec != nullptr && g.mixed == P.mixed && g.fidelity == P.fidelity &&
+                     g.old_halo_current == P.old_halo_current &&
                      g.b_recovered == P.b_recovered &&
                      g.mi_potential == P.mi_view.potential;
   if (!match) {
@@ -295,7 +296,7 @@ static int launch_step_graph(Problem &P) {
       g.exec = nullptr;
     }
     const int level = P.level;
-    const bool recovered = P.b_recovered;
+    const bool recovered = P.b_recovered, halo_current = P.old_halo_current;
     hipGraph_t graph = nullptr;
     GCS_HIP(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue_step(P, false);
@@ -303,6 +304,7 @@ static int launch_step_graph(Problem &P) {
     // Capturing ran no kernel: the state is still that of the step's start.
     P.level = level;
     P.b_recovered = recovered;
+    P.old_halo_current = halo_current;
     if (rc != 0 || end != hipSuccess) {
       if (graph != nullptr) (void)hipGraphDestroy(graph);
       if (rc == 0) set_error(std::string("step capture: ") + hipGetErrorString(end));
@@ -318,11 +320,13 @@ static int launch_step_graph(Problem &P) {
     g.mixed = P.mixed;
     g.fidelity = P.fidelity;
     g.b_recovered = P.b_recovered;
+    g.old_halo_current = P.old_halo_current;
     g.mi_potential = P.mi_view.potential;
   }
   GCS_HIP(hipGraphLaunch(g.exec, P.stream));
   P.level = 1 - P.level;
   P.host_dirty = true;
+  P.old_halo_current = P.b_recovered;
   P.b_recovered = true;
   return 0;
 }

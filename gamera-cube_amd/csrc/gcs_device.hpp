@@ -152,7 +152,7 @@ struct Problem {
   // baked into its kernel arguments (recaptured when any of it changes).
   struct StepGraph {
     hipGraphExec_t exec = nullptr;
-    bool mixed = false, fidelity = false, b_recovered = false;
+    bool mixed = false, fidelity = false, b_recovered = false, old_halo_current = false;
     const double *mi_potential = nullptr;
   };
   bool use_graph = false;
@@ -169,6 +169,11 @@ struct Problem {
   // Cell B of both levels equals the recovery from their face fluxes (true
   // after a completed step; false at creation and after a host upload).
   bool b_recovered = false;
+  // The old level's ghost face fluxes are those of its active B (true after
+  // a step whose ghost phase did not recover every cell): a level's active B
+  // does not change while it is the old level, so fidelity 0 fills the
+  // ghost fluxes of the current level only.
+  bool old_halo_current = false;
   // Stage timing: when `profiling`, enqueue_step records an event before
   // every stage into `stage_events` (one row of kMaxStages per step).
   bool profiling = false;

@@ -356,7 +356,12 @@ __global__ void k_halo_cells(K c, const HaloCell *e, int n, int with_b) {
 }
 
 // cubedsphere_halo.c:651-686: ghost face flux = interpolated donor B . area.
-__global__ void k_halo_faces(K c, const HaloFace *e, int n) {
+// The donors are active cells. A level's active B does not change while it
+// is the old level, so its ghost fluxes computed a step earlier, when it was
+// the current level, are still exact once that step's ghost phase left the
+// active B alone: with `both` = 0 (fidelity 0, Problem::old_halo_current)
+// only the current level is filled.
+__global__ void k_halo_faces(K c, const HaloFace *e, int n, int both) {
   const long t = tid();
   if (t >= n) return;
   const HaloFace h = e[t];
@@ -366,10 +371,10 @@ __global__ void k_halo_faces(K c, const HaloFace *e, int n) {
   for (int d = 0; d < 3; ++d) {
     const double av = c.p(base + d)[h.target];
     cf += block_sum(c.p(plane_b(c.cur, d)), h.donor, sk, h.w) * av;
-    of += block_sum(c.p(plane_b(c.old, d)), h.donor, sk, h.w) * av;
+    if (both) of += block_sum(c.p(plane_b(c.old, d)), h.donor, sk, h.w) * av;
   }
   c.p(plane_phi(c.cur, h.dir))[h.target] = cf;
-  c.p(plane_phi(c.old, h.dir))[h.target] = of;
+  if (both) c.p(plane_phi(c.old, h.dir))[h.target] = of;
 }
 
 // cubedsphere_halo.c:217-245 (plain problems without the Earth boundary).
@@ -1975,7 +1980,8 @@ int enqueue_phase(Problem &P, Phase phase) {
       k_radial_outflow<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c);
     }
     mark_stage(P, S_HALO_FACES);
-    k_halo_faces<<<nblocks(P.n_halo_faces, T), T, 0, s>>>(c, P.halo_faces, P.n_halo_faces);
+    k_halo_faces<<<nblocks(P.n_halo_faces, T), T, 0, s>>>(
+        c, P.halo_faces, P.n_halo_faces, (P.fidelity || !P.old_halo_current) ? 1 : 0);
     if (bd.earth_cells) {
       mark_stage(P, S_EARTH_CELLS);
       k_earth_cells<<<nblocks(6L * 2 * L.ghost * L.NJ * L.NK, T), T, 0, s>>>(c, w);
@@ -2111,6 +2117,7 @@ int enqueue_phase(Problem &P, Phase phase) {
     }
     P.level = 1 - P.level;
     P.host_dirty = true;
+    P.old_halo_current = P.b_recovered;
     P.b_recovered = true;
   }
   GCS_HIP(hipGetLastError());
