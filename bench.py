@@ -45,7 +45,8 @@ STAGE_MODEL = {
     "edge_emf_i": (184, lambda nr, n: 6 * nr * (n + 1) * (n + 1)),
     "edge_emf_j": (184, lambda nr, n: 6 * (nr + 1) * n * (n + 1)),
     "edge_emf_k": (184, lambda nr, n: 6 * (nr + 1) * (n + 1) * n),
-    "update": (616, lambda nr, n: 6 * nr * n * n),
+    # the update also folds the new state's CFL dt (3 CFL lengths more)
+    "update": (640, lambda nr, n: 6 * nr * n * n),
     "predict": (392, lambda nr, n: 6 * (nr + 9) * (n + 9) * (n + 9)),
 }
 

@@ -68,6 +68,7 @@ void free_problem(Problem *P) {
   (void)hipFree(P->bytes);
   (void)hipFree(P->scalars);
   (void)hipFree(P->cfl_partial);
+  (void)hipFree(P->cfl_slots);
   (void)hipFree(P->halo_cells);
   (void)hipFree(P->halo_faces);
   (void)hipFree(P->seams);
@@ -90,6 +91,7 @@ int allocate_problem(Problem &P) {
       hipMalloc(&P.cfl_partial,
                 ((6L * P.L.nr * P.L.n * P.L.n + 255) / 256 + 1) * sizeof(unsigned long long)) !=
           hipSuccess ||
+      hipMalloc(&P.cfl_slots, kCflSlots * sizeof(unsigned long long)) != hipSuccess ||
       hipMemset(P.planes, 0, nbytes) != hipSuccess ||
       hipMemset(P.bytes, 0, nflags) != hipSuccess ||
       hipEventCreate(&P.ev_start) != hipSuccess ||
@@ -288,6 +290,7 @@ static int launch_step_graph(Problem &P) {
   Problem::StepGraph &g = P.graphs[P.level];
   const bool match = g.exec != nullptr && g.mixed == P.mixed && g.fidelity == P.fidelity &&
                      g.old_halo_current == P.old_halo_current &&
+                     g.cfl_folded == P.cfl_folded &&
                      g.b_recovered == P.b_recovered &&
                      g.mi_potential == P.mi_view.potential;
   if (!match) {
@@ -296,7 +299,8 @@ static int launch_step_graph(Problem &P) {
       g.exec = nullptr;
     }
     const int level = P.level;
-    const bool recovered = P.b_recovered, halo_current = P.old_halo_current;
+    const bool recovered = P.b_recovered, halo_current = P.old_halo_current,
+               folded = P.cfl_folded;
     hipGraph_t graph = nullptr;
     GCS_HIP(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue_step(P, false);
@@ -305,6 +309,7 @@ static int launch_step_graph(Problem &P) {
     P.level = level;
     P.b_recovered = recovered;
     P.old_halo_current = halo_current;
+    P.cfl_folded = folded;
     if (rc != 0 || end != hipSuccess) {
       if (graph != nullptr) (void)hipGraphDestroy(graph);
       if (rc == 0) set_error(std::string("step capture: ") + hipGetErrorString(end));
@@ -321,6 +326,7 @@ static int launch_step_graph(Problem &P) {
     g.fidelity = P.fidelity;
     g.b_recovered = P.b_recovered;
     g.old_halo_current = P.old_halo_current;
+    g.cfl_folded = P.cfl_folded;
     g.mi_potential = P.mi_view.potential;
   }
   GCS_HIP(hipGraphLaunch(g.exec, P.stream));
@@ -328,6 +334,7 @@ static int launch_step_graph(Problem &P) {
   P.host_dirty = true;
   P.old_halo_current = P.b_recovered;
   P.b_recovered = true;
+  P.cfl_folded = P.driver_cfl;
   return 0;
 }
 
@@ -365,6 +372,7 @@ Problem *gcs_problem_create(const gcs_problem_config *cfg) {
   }
   gcs::make_wind(*P);
   gcs::driver_options(*P);
+  P->driver_cfl = true;
   if (gcs::setup_problem(*P) != 0 || gcs::build_plans(*P) != 0 ||
       (cfg->mi_enabled && gcs::mi_create(*P) != 0)) {
     gcs::free_problem(P);

@@ -119,6 +119,11 @@ struct StepBoundary {
   bool earth_emf = false;   // gamera_cs_earth_no_mi_emf
 };
 
+// Slots of the CFL minimum the update and the seam recovery fold their
+// cells' dt into (positive doubles order like their bit patterns, and min
+// is exact, so neither the order nor the slot of a contribution matters).
+constexpr int kCflSlots = 1024;
+
 struct Problem {
   gcs_problem_config cfg;
   StepOptions opt;
@@ -129,6 +134,11 @@ struct Problem {
   uint8_t *bytes = nullptr;        // BYTE_PLANES * L.total
   StepScalars *scalars = nullptr;  // device
   unsigned long long *cfl_partial = nullptr;  // per-block CFL minima
+  unsigned long long *cfl_slots = nullptr;    // kCflSlots CFL minima folded by the update
+  // Driver problems fold the new state's CFL dt into cfl_slots during the
+  // update (drop-in mirrors take dt from the caller); cfl_folded: the slots
+  // hold the current state's minimum.
+  bool driver_cfl = false, cfl_folded = false;
   int level = 0;                   // which U/B/PHI level is current
   long steps = 0;
   double time = 0.0, dt = 0.0, previous_dt = 0.0;
@@ -152,7 +162,8 @@ struct Problem {
   // baked into its kernel arguments (recaptured when any of it changes).
   struct StepGraph {
     hipGraphExec_t exec = nullptr;
-    bool mixed = false, fidelity = false, b_recovered = false, old_halo_current = false;
+    bool mixed = false, fidelity = false, b_recovered = false, old_halo_current = false,
+         cfl_folded = false;
     const double *mi_potential = nullptr;
   };
   bool use_graph = false;

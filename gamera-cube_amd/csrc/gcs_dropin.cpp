@@ -256,6 +256,7 @@ int state_to_device(Problem &P, const gamera_no_storage *st) {
   const Layout &L = P.L;
   P.b_recovered = false;
   P.old_halo_current = false;
+  P.cfl_folded = false;
   const size_t ce[3] = {(size_t)L.NI, (size_t)L.NJ, (size_t)L.NK};
   const int cur = P.level, old = 1 - P.level;
   for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f) {

@@ -140,8 +140,50 @@ constexpr int kCflThreads = 256;
 
 // ---- dt rule ---------------------------------------------------------------
 
-// Local CFL dt of every active cell, min-reduced (nonorthogonal_step.c:
-// 449-525). Positive doubles order like their bit patterns.
+// Local CFL dt of one active cell from its conserved state u and residual
+// cell B b (nonorthogonal_step.c:449-525); DBL_MAX, with the failure
+// recorded, when the state is invalid.
+GCS_DEV double cfl_dt(const K &c, long at, const double u[5], const double b[3]) {
+  Prim w;
+  if (!cons_to_prim(u, c.o.ts_gamma, c.o.ts_rho_floor, c.o.ts_p_floor, w)) {
+    c.fail(1, at);
+    return DBL_MAX;
+  }
+  const double cs = sqrt(c.o.ts_gamma * w.p / w.rho);
+  double m[3];
+  for (int d = 0; d < 3; ++d) m[d] = b[d] + c.p(P_B0C + d)[at];
+  double va = sqrt(dot3(m, m)) / sqrt(w.rho);
+  if (c.o.ts_boris) va = c.o.ts_c * va / hypot(c.o.ts_c, va);
+  const double speed = sqrt(dot3(w.v, w.v)) + hypot(cs, va);
+  double len = c.p(P_CFL)[at];
+  len = fmin(len, c.p(P_CFL + 1)[at]);
+  len = fmin(len, c.p(P_CFL + 2)[at]);
+  const double cand = c.o.ts_cfl * len / speed;
+  if (!isfinite(cand) || cand <= 0.0) {
+    c.fail(2, at);
+    return DBL_MAX;
+  }
+  return cand;
+}
+
+// Minimum of v over the wavefront, folded by lane 0 into one of the slots.
+// Every lane of the wave must call it.
+GCS_DEV void cfl_fold(double v, unsigned long long *slots) {
+  unsigned long long bits = __double_as_longlong(v);
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long other = __shfl_down(bits, off, 64);
+    bits = other < bits ? other : bits;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMin(&slots[(tid() >> 6) % kCflSlots], bits);
+}
+
+__global__ void k_cfl_reset(unsigned long long *slots) {
+  slots[threadIdx.x] = __double_as_longlong(DBL_MAX);
+}
+
+// The CFL dt of every active cell from the current level, min-reduced to
+// one partial per block: the first step after creation or an upload (later
+// steps take the minimum the previous step's update folded into the slots).
 __global__ void __launch_bounds__(kCflThreads) k_cfl(K c, unsigned long long *partial) {
   const Layout &L = c.L;
   const int lo[3] = {L.ghost, L.ghost, L.ghost};
@@ -150,28 +192,12 @@ __global__ void __launch_bounds__(kCflThreads) k_cfl(K c, unsigned long long *pa
   double best = DBL_MAX;
   if (box(tid(), lo, hi, f, i, j, k)) {
     const long at = L.at(f, i, j, k);
-    double u[5];
+    double u[5], b[3];
     for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
-    Prim w;
-    if (!cons_to_prim(u, c.o.ts_gamma, c.o.ts_rho_floor, c.o.ts_p_floor, w)) {
-      c.fail(1, at);
-    } else {
-      const double cs = sqrt(c.o.ts_gamma * w.p / w.rho);
-      double m[3];
-      for (int d = 0; d < 3; ++d) m[d] = c.p(plane_b(c.cur, d))[at] + c.p(P_B0C + d)[at];
-      double va = sqrt(dot3(m, m)) / sqrt(w.rho);
-      if (c.o.ts_boris) va = c.o.ts_c * va / hypot(c.o.ts_c, va);
-      const double speed = sqrt(dot3(w.v, w.v)) + hypot(cs, va);
-      double len = c.p(P_CFL)[at];
-      len = fmin(len, c.p(P_CFL + 1)[at]);
-      len = fmin(len, c.p(P_CFL + 2)[at]);
-      const double cand = c.o.ts_cfl * len / speed;
-      if (!isfinite(cand) || cand <= 0.0) c.fail(2, at);
-      else best = cand;
-    }
+    for (int d = 0; d < 3; ++d) b[d] = c.p(plane_b(c.cur, d))[at];
+    best = cfl_dt(c, at, u, b);
   }
-  // wavefront min, block min through LDS, one partial per block (min is
-  // exact, so the order of the reduction does not matter)
+  // wavefront min, block min through LDS, one partial per block
   unsigned long long bits = __double_as_longlong(best);
   for (int off = 32; off > 0; off >>= 1) {
     const unsigned long long other = __shfl_down(bits, off, 64);
@@ -743,17 +769,27 @@ __global__ void k_recover_ghosts(K c, int phi_plane, int b_plane, int inner_copy
 }
 
 // Recover a listed set of cells (active cells with a seam face).
+// With `cfl`, the cells are the current level's seam cells after the final
+// seam sync, and their CFL dt is folded into the CFL slots.
 __global__ void k_recover_list(K c, const int *cells, int n, int phi_plane,
-                               int b_plane) {
+                               int b_plane, unsigned long long *cfl) {
   const long t = tid();
-  if (t >= n) return;
-  const long at = cells[t];
-  double b[3];
-  if (!recover_at(c, at, phi_plane, b)) {
-    c.fail(23, at);
-    return;
+  double dt_cell = DBL_MAX;
+  if (t < n) {
+    const long at = cells[t];
+    double b[3];
+    if (!recover_at(c, at, phi_plane, b)) {
+      c.fail(23, at);
+    } else {
+      for (int d = 0; d < 3; ++d) c.p(b_plane + d)[at] = b[d];
+      if (cfl != nullptr) {
+        double u[5];
+        for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
+        dt_cell = cfl_dt(c, at, u, b);
+      }
+    }
   }
-  for (int d = 0; d < 3; ++d) c.p(b_plane + d)[at] = b[d];
+  if (cfl != nullptr) cfl_fold(dt_cell, cfl);
 }
 
 // ---- predictor -----------------------------------------------------------------
@@ -1753,15 +1789,15 @@ __global__ void k_earth_emf_inner(K c) {
 // recovery (:565) and Boris correction (:571-606) of one active cell,
 // writing the new level. With Mixed the face fluxes come from the FP32
 // planes (widened exactly); the arithmetic is FP64 either way.
+// With want_cfl, returns the new state's CFL dt of the cell (cfl_dt) for the
+// next step's dt rule; DBL_MAX otherwise, for a seam cell (its B changes
+// again in the final seam sync; k_recover_list folds its dt) and for a
+// failed cell.
 template <bool Mixed>
-__global__ void __launch_bounds__(128) k_update(K c) {
+GCS_DEV double update_cell(const K &c, int i, int j, int k, long at, const int hi[3],
+                           bool want_cfl) {
   using Flux = typename std::conditional<Mixed, float, double>::type;
   const Layout &L = c.L;
-  const int lo[3] = {L.ghost, L.ghost, L.ghost};
-  const int hi[3] = {L.ghost + L.nr, L.ghost + L.n, L.ghost + L.n};
-  int f, i, j, k;
-  if (!box(tid(), lo, hi, f, i, j, k)) return;
-  const long at = L.at(f, i, j, k);
   const long si = L.di(), sj = L.SK;
   const long step[3] = {si, sj, 1};
   const StepOptions &o = c.o;
@@ -1816,7 +1852,7 @@ __global__ void __launch_bounds__(128) k_update(K c) {
   const int piv = c.b(Q_PIV)[at];
   if (piv == 0xff || !recover_cell(flux, fc, cen, vol, rec, piv, bn)) {
     c.fail(71, at);
-    return;
+    return DBL_MAX;
   }
   if (o.upd_boris) {
     double btn[3], bto[3];
@@ -1828,7 +1864,7 @@ __global__ void __launch_bounds__(128) k_update(K c) {
     if (!boris_update(u, u0, btn, bto, rate, mr, o.upd_c, dt, o.upd_gamma,
                       o.upd_rho_floor, o.upd_p_floor)) {
       c.fail(72, at);
-      return;
+      return DBL_MAX;
     }
   } else {
     maxwell_update(u, mr, dt, o.upd_gamma, o.upd_rho_floor, o.upd_p_floor);
@@ -1841,6 +1877,23 @@ __global__ void __launch_bounds__(128) k_update(K c) {
   if (i == hi[0] - 1) c.p(plane_phi(c.old, 0))[at + si] = flux[1];
   if (j == hi[1] - 1) c.p(plane_phi(c.old, 1))[at + sj] = flux[3];
   if (k == hi[2] - 1) c.p(plane_phi(c.old, 2))[at + 1] = flux[5];
+  const int g = L.ghost;
+  if (!want_cfl || j == g || j == hi[1] - 1 || k == g || k == hi[2] - 1) return DBL_MAX;
+  return cfl_dt(c, at, u, bn);
+}
+
+// One active cell per thread; with `cfl` (a driver step), every cell's dt
+// of the new state is folded into the CFL slots.
+template <bool Mixed>
+__global__ void __launch_bounds__(128) k_update(K c, unsigned long long *cfl) {
+  const Layout &L = c.L;
+  const int lo[3] = {L.ghost, L.ghost, L.ghost};
+  const int hi[3] = {L.ghost + L.nr, L.ghost + L.n, L.ghost + L.n};
+  int f, i, j, k;
+  double dt_cell = DBL_MAX;
+  if (box(tid(), lo, hi, f, i, j, k))
+    dt_cell = update_cell<Mixed>(c, i, j, k, L.at(f, i, j, k), hi, cfl != nullptr);
+  if (cfl != nullptr) cfl_fold(dt_cell, cfl);
 }
 
 // Ghost values of the new level as the reference leaves them in storage:
@@ -2095,10 +2148,14 @@ int enqueue_phase(Problem &P, Phase phase) {
     k_sync_edges<<<nblocks(P.n_edge_groups, T), T, 0, s>>>(c, P.edge_groups, P.edge_members,
                                                            P.n_edge_groups);
     mark_stage(P, S_UPDATE);
+    // A driver step folds the new state's CFL dt into the slots that the
+    // next step's dt rule reads (its seam cells after the final sync).
+    unsigned long long *cfl = P.driver_cfl ? P.cfl_slots : nullptr;
+    if (cfl != nullptr) k_cfl_reset<<<1, kCflSlots, 0, s>>>(cfl);
     if (P.mixed)
-      k_update<true><<<nblocks(n_active, 128), 128, 0, s>>>(c);
+      k_update<true><<<nblocks(n_active, 128), 128, 0, s>>>(c, cfl);
     else
-      k_update<false><<<nblocks(n_active, 128), 128, 0, s>>>(c);
+      k_update<false><<<nblocks(n_active, 128), 128, 0, s>>>(c, cfl);
     // Final seam average of the new face flux, then B of the touched cells.
     K n = c;
     n.cur = c.old;
@@ -2108,7 +2165,7 @@ int enqueue_phase(Problem &P, Phase phase) {
     mark_stage(P, S_SEAM_RECOVER);
     k_recover_list<<<nblocks(P.n_seam_cells, T), T, 0, s>>>(n, P.seam_cells, P.n_seam_cells,
                                                             plane_phi(n.cur, 0),
-                                                            plane_b(n.cur, 0));
+                                                            plane_b(n.cur, 0), cfl);
     if (P.fidelity) {
       mark_stage(P, S_FIDELITY);
       k_keep_ghosts<<<nblocks(n_box, T), T, 0, s>>>(c);
@@ -2119,6 +2176,7 @@ int enqueue_phase(Problem &P, Phase phase) {
     P.host_dirty = true;
     P.old_halo_current = P.b_recovered;
     P.b_recovered = true;
+    P.cfl_folded = cfl != nullptr;
   }
   GCS_HIP(hipGetLastError());
   return 0;
@@ -2234,14 +2292,19 @@ int enqueue_step(Problem &P, bool explicit_dt) {
   K c = make_k(P);
   const Layout &L = P.L;
   hipStream_t s = P.stream;
-  const int T = 256;
   const long n_active = 6L * L.nr * L.n * L.n;
   if (!explicit_dt) {
-    mark_stage(P, S_CFL);
-    const int ncfl = (int)nblocks(n_active, kCflThreads);
-    k_cfl<<<ncfl, kCflThreads, 0, s>>>(c, P.cfl_partial);
-    mark_stage(P, S_DT);
-    k_dt<<<1, kCflThreads, 0, s>>>(c, P.cfg.dt_fraction, P.cfl_partial, ncfl);
+    if (P.cfl_folded) {
+      // the previous step's update folded this state's CFL dt
+      mark_stage(P, S_DT);
+      k_dt<<<1, kCflThreads, 0, s>>>(c, P.cfg.dt_fraction, P.cfl_slots, kCflSlots);
+    } else {
+      mark_stage(P, S_CFL);
+      const int ncfl = (int)nblocks(n_active, kCflThreads);
+      k_cfl<<<ncfl, kCflThreads, 0, s>>>(c, P.cfl_partial);
+      mark_stage(P, S_DT);
+      k_dt<<<1, kCflThreads, 0, s>>>(c, P.cfg.dt_fraction, P.cfl_partial, ncfl);
+    }
   }
   for (Phase ph : {PHASE_GHOSTS, PHASE_FLUX, PHASE_EMF, PHASE_UPDATE})
     if (enqueue_phase(P, ph) != 0) return -1;
