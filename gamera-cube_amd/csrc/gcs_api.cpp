@@ -472,6 +472,20 @@ int gcs_problem_max_div_b(Problem *P, double *out) {
   return gcs::max_div_b(*P, out);
 }
 
+int gcs_problem_pressure_control(Problem *P, double dt, double gamma, double density_floor,
+                                 double pressure_floor, double light_speed,
+                                 double low_density_threshold,
+                                 double low_density_sound_speed, unsigned long long counts[2],
+                                 double maxima[2]) {
+  if (P == nullptr || counts == nullptr || maxima == nullptr) return -1;
+  const double args[7] = {dt, gamma, density_floor, pressure_floor, light_speed,
+                          low_density_threshold, low_density_sound_speed};
+  if (gcs::pressure_control(*P, args, counts, maxima) != 0) return -1;
+  gcs::StepScalars s;
+  if (gcs::read_scalars(*P, s) != 0) return -1;
+  return s.failed ? gcs::check_failure(*P) : 0;
+}
+
 int gcs_problem_set_graph(Problem *P, int on) {
   if (P == nullptr) return -1;
   P->use_graph = on != 0;

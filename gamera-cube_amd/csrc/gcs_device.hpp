@@ -220,6 +220,11 @@ int allocate_problem(Problem &P);
 // their FP64 planes for download.
 int prepare_mixed(Problem &P);
 int widen_mixed(Problem &P);
+// gamera_no_apply_pressure_control on the current state; args = dt, gamma,
+// density floor, pressure floor, light speed, low-density threshold,
+// low-density sound speed.
+int pressure_control(Problem &P, const double args[7], unsigned long long counts[2],
+                     double maxima[2]);
 void free_problem(Problem *P);
 int init_scalars(Problem &P);
 int read_scalars(Problem &P, StepScalars &s);

@@ -2265,6 +2265,116 @@ int widen_mixed(Problem &P) {
   return 0;
 }
 
+// gamera_no_apply_pressure_control (NO/nonorthogonal_step.c:74-171) on the
+// active cells of the current level: a cell at or below the low-density
+// threshold takes the pressure of the given sound speed; a cell whose sound
+// speed exceeds 1.5 c relaxes toward that sound speed's pressure over a
+// dipole bounce time. acc = {low-density cells, high-sound-speed cells,
+// maximum sound speed bits, maximum pressure bits}, reduced per wave (the
+// counts are sums and the maxima exact, so the order does not matter).
+struct PressureControlArgs {
+  double dt, gamma, density_floor, pressure_floor, light_speed, low_density_threshold,
+      low_density_sound_speed;
+};
+
+__global__ void k_pressure_control(K c, PressureControlArgs a, unsigned long long *acc) {
+  const Layout &L = c.L;
+  const int lo[3] = {L.ghost, L.ghost, L.ghost};
+  const int hi[3] = {L.ghost + L.nr, L.ghost + L.n, L.ghost + L.n};
+  int f, i, j, k;
+  bool low = false, hot = false;
+  double sound = 0.0, pressure = 0.0;
+  if (box(tid(), lo, hi, f, i, j, k)) {
+    const long at = L.at(f, i, j, k);
+    double u[5];
+    for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
+    Prim w;
+    if (!cons_to_prim(u, a.gamma, a.density_floor, a.pressure_floor, w)) {
+      c.fail(81, at);
+    } else {
+      bool changed = false, ok = true;
+      if (w.rho <= a.low_density_threshold) {
+        w.p = fmax(a.pressure_floor,
+                   w.rho * a.low_density_sound_speed * a.low_density_sound_speed / a.gamma);
+        low = changed = true;
+      }
+      const double cs = sqrt(a.gamma * w.p / w.rho);
+      sound = cs;
+      pressure = w.p;
+      if (cs > 1.5 * a.light_speed) {
+        const double x = c.p(P_CEN)[at], y = c.p(P_CEN + 1)[at], z = c.p(P_CEN + 2)[at];
+        const double radius_square = x * x + y * y + z * z;
+        const double cylindrical_square = x * x + y * y;
+        if (!(radius_square > 0.0) || !(cylindrical_square > DBL_MIN)) {
+          c.fail(82, at);
+          ok = false;
+        } else {
+          const double radius = sqrt(radius_square);
+          const double dipole_l = radius * radius_square / cylindrical_square;
+          const double target =
+              w.rho * (1.5 * a.light_speed) * (1.5 * a.light_speed) / a.gamma;
+          const double bounce_time = dipole_l / cs;
+          w.p -= (a.dt / bounce_time) * (w.p - target);
+          hot = changed = true;
+        }
+      }
+      if (ok && changed) {
+        if (!prim_to_cons(w, a.gamma, a.density_floor, a.pressure_floor, u))
+          c.fail(83, at);
+        else
+          for (int q = 0; q < 5; ++q) c.p(plane_u(c.cur, q))[at] = u[q];
+      }
+    }
+  }
+  const unsigned long long n_low = __popcll(__ballot(low)), n_hot = __popcll(__ballot(hot));
+  unsigned long long s_bits = __double_as_longlong(sound), p_bits = __double_as_longlong(pressure);
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long so = __shfl_down(s_bits, off, 64), po = __shfl_down(p_bits, off, 64);
+    s_bits = so > s_bits ? so : s_bits;
+    p_bits = po > p_bits ? po : p_bits;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (n_low) atomicAdd(&acc[0], n_low);
+    if (n_hot) atomicAdd(&acc[1], n_hot);
+    atomicMax(&acc[2], s_bits);
+    atomicMax(&acc[3], p_bits);
+  }
+}
+
+__global__ void k_zero4(unsigned long long *acc) { acc[threadIdx.x] = 0ull; }
+
+int pressure_control(Problem &P, const double args[7], unsigned long long counts[2],
+                     double maxima[2]) {
+  const PressureControlArgs a{args[0], args[1], args[2], args[3], args[4], args[5], args[6]};
+  if (!(a.dt > 0.0) || !(a.gamma > 1.0) || !(a.density_floor > 0.0) ||
+      !(a.pressure_floor > 0.0) || !(a.light_speed > 0.0) ||
+      !(a.low_density_threshold > 0.0) || !(a.low_density_sound_speed >= 0.0)) {
+    set_error("pressure control: invalid arguments");
+    return -1;
+  }
+  unsigned long long *acc = nullptr, host[4] = {0, 0, 0, 0};
+  GCS_HIP(hipMalloc(&acc, sizeof(host)));
+  k_zero4<<<1, 4, 0, P.stream>>>(acc);
+  k_pressure_control<<<nblocks(6L * P.L.nr * P.L.n * P.L.n, 256), 256, 0, P.stream>>>(
+      make_k(P), a, acc);
+  const hipError_t e = hipGetLastError();
+  const hipError_t m = hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, P.stream);
+  const hipError_t w = hipStreamSynchronize(P.stream);
+  (void)hipFree(acc);
+  if (e != hipSuccess || m != hipSuccess || w != hipSuccess) {
+    set_error("pressure control: device error");
+    return -1;
+  }
+  // The state changed: the next step's dt is no longer the folded one.
+  P.cfl_folded = false;
+  P.host_dirty = true;
+  counts[0] = host[0];
+  counts[1] = host[1];
+  std::memcpy(&maxima[0], &host[2], sizeof(double));
+  std::memcpy(&maxima[1], &host[3], sizeof(double));
+  return 0;
+}
+
 int max_div_b(Problem &P, double *out) {
   unsigned long long *dev = nullptr, host = 0;
   GCS_HIP(hipMalloc(&dev, sizeof(host)));

@@ -150,6 +150,26 @@ def _set_precision(self, mixed):
 GpuProblem.set_precision = _set_precision
 
 
+def _pressure_control(self, dt, gamma, density_floor, pressure_floor, light_speed,
+                      low_density_threshold, low_density_sound_speed):
+    """gamera_no_apply_pressure_control on the current state (see
+    include/gamera_cs_step.h); returns ((low-density cells, high-sound-speed
+    cells), (maximum sound speed, maximum pressure))."""
+    fn = self._lib.gcs_problem_pressure_control
+    fn.restype = ctypes.c_int
+    fn.argtypes = ([ctypes.c_void_p] + [ctypes.c_double] * 7
+                   + [ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_double)])
+    counts = (ctypes.c_ulonglong * 2)()
+    maxima = (ctypes.c_double * 2)()
+    if fn(self._handle, dt, gamma, density_floor, pressure_floor, light_speed,
+          low_density_threshold, low_density_sound_speed, counts, maxima) != 0:
+        raise RuntimeError(f"gcs_problem_pressure_control: {last_error()}")
+    return (counts[0], counts[1]), (maxima[0], maxima[1])
+
+
+GpuProblem.pressure_control = _pressure_control
+
+
 def _set_graph(self, on):
     """True: each driver step is one hipGraph launch (same results)."""
     self._lib.gcs_problem_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]

@@ -81,6 +81,23 @@ int gcs_problem_set_fidelity(gcs_problem *problem, int on);
  * the face fluxes widen the FP32 values. No reference counterpart. */
 int gcs_problem_set_precision(gcs_problem *problem, int mixed);
 
+/* gamera_no_apply_pressure_control (nonorthogonal_step.h:26-33,
+ * NO/nonorthogonal_step.c:74-171) on the current state's active cells, as
+ * the MPI Earth driver applies it after a step
+ * (cubedsphere/tests/test_cubedsphere_mpi_earth_tile.c:459-479): cells at
+ * or below low_density_threshold take the pressure of
+ * low_density_sound_speed; cells whose sound speed exceeds 1.5 light_speed
+ * relax toward it over a dipole bounce time. counts = (low-density cells,
+ * high-sound-speed cells), maxima = (maximum sound speed, maximum pressure)
+ * as the reference reports them (over all faces). Returns -1 on invalid
+ * arguments (nothing changed) or on a cell the reference would reject
+ * (MHD_FAILURE message). */
+int gcs_problem_pressure_control(gcs_problem *problem, double dt, double gamma,
+                                 double density_floor, double pressure_floor,
+                                 double light_speed, double low_density_threshold,
+                                 double low_density_sound_speed,
+                                 unsigned long long counts[2], double maxima[2]);
+
 /* 1: run each driver step (outside profiled timing) as one hipGraph launch,
  * captured per time-level parity on first use and recaptured when the
  * precision, fidelity, ghost-recovery state or ionosphere view changes.

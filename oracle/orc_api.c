@@ -7,6 +7,7 @@
  * reference driver's (test_cubedsphere_dipole_background.c:589-699). */
 #include "orc.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -386,4 +387,91 @@ int orc_mi_fac_export(const orc_problem *p, double *out, size_t count) {
     memcpy(out + (size_t)h * n, v, n * sizeof(double));
   }
   return 0;
+}
+
+/* NO/nonorthogonal_step.c:74-171 (gamera_no_apply_pressure_control) on one
+ * face's active cells, the counters and maxima accumulated into the
+ * caller's. */
+static int pressure_control_face(gamera_no_storage *storage,
+                                 const gamera_no_grid *grid,
+                                 const size_t lower[3], const size_t upper[3],
+                                 double dt, double gamma, double density_floor,
+                                 double pressure_floor, double light_speed,
+                                 double low_density_threshold,
+                                 double low_density_sound_speed,
+                                 unsigned long long counts[2], double maxima[2]) {
+  int failed = 0;
+  for (size_t i = lower[0]; i < upper[0]; ++i)
+    for (size_t j = lower[1]; j < upper[1]; ++j)
+      for (size_t k = lower[2]; k < upper[2]; ++k) {
+        const size_t cell = orc_idx(storage->cell_extent, i, j, k);
+        double *conserved = &storage->conserved[cell * ORC_NQ];
+        gamera_no_primitive primitive;
+        if (orc_cons_to_prim(conserved, gamma, density_floor, pressure_floor,
+                             &primitive) != 0) {
+          failed = 1;
+          continue;
+        }
+        int changed = 0;
+        if (primitive.density <= low_density_threshold) {
+          primitive.pressure =
+              fmax(pressure_floor, primitive.density * low_density_sound_speed *
+                                       low_density_sound_speed / gamma);
+          ++counts[0];
+          changed = 1;
+        }
+        const double sound_speed = sqrt(gamma * primitive.pressure / primitive.density);
+        maxima[0] = fmax(maxima[0], sound_speed);
+        maxima[1] = fmax(maxima[1], primitive.pressure);
+        if (sound_speed > 1.5 * light_speed) {
+          const gamera_no_vec3 point = grid->cell[cell].centroid;
+          const double radius_square = point.value[0] * point.value[0] +
+                                       point.value[1] * point.value[1] +
+                                       point.value[2] * point.value[2];
+          const double cylindrical_square =
+              point.value[0] * point.value[0] + point.value[1] * point.value[1];
+          if (!(radius_square > 0.0) || !(cylindrical_square > DBL_MIN)) {
+            failed = 1;
+            continue;
+          }
+          const double radius = sqrt(radius_square);
+          const double dipole_l = radius * radius_square / cylindrical_square;
+          const double target_pressure =
+              primitive.density * (1.5 * light_speed) * (1.5 * light_speed) / gamma;
+          const double bounce_time = dipole_l / sound_speed;
+          primitive.pressure -= (dt / bounce_time) * (primitive.pressure - target_pressure);
+          ++counts[1];
+          changed = 1;
+        }
+        if (changed && orc_prim_to_cons(&primitive, gamma, density_floor, pressure_floor,
+                                        conserved) != 0)
+          failed = 1;
+      }
+  return failed ? -1 : 0;
+}
+
+/* The post-step pressure control of the MPI Earth driver
+ * (cubedsphere/tests/test_cubedsphere_mpi_earth_tile.c:459-479) on every
+ * face's active cells; counts = (low-density cells, high-sound-speed cells),
+ * maxima = (sound speed, pressure) as the reference reports them. */
+int orc_pressure_control(orc_problem *p, double dt, double gamma, double density_floor,
+                         double pressure_floor, double light_speed,
+                         double low_density_threshold, double low_density_sound_speed,
+                         unsigned long long counts[2], double maxima[2]) {
+  if (p == NULL || counts == NULL || maxima == NULL || !(dt > 0.0) || !(gamma > 1.0) ||
+      !(density_floor > 0.0) || !(pressure_floor > 0.0) || !(light_speed > 0.0) ||
+      !(low_density_threshold > 0.0) || !(low_density_sound_speed >= 0.0))
+    return -1;
+  counts[0] = counts[1] = 0;
+  maxima[0] = maxima[1] = 0.0;
+  const size_t lower[3] = {p->ghost, p->ghost, p->ghost};
+  const size_t upper[3] = {p->ghost + p->active[0], p->ghost + p->active[1],
+                           p->ghost + p->active[2]};
+  int failed = 0;
+  for (int f = 0; f < ORC_FACES; ++f)
+    failed |= pressure_control_face(&p->storage[f], &p->grid[f], lower, upper, dt, gamma,
+                                    density_floor, pressure_floor, light_speed,
+                                    low_density_threshold, low_density_sound_speed, counts,
+                                    maxima) != 0;
+  return failed ? -1 : 0;
 }

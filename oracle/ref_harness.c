@@ -497,3 +497,34 @@ int refh_mi_potential(const refh_problem *p, double *out, size_t count) {
   return gamera_cs_mi_dipole_export_potential(p->mi, out, count, &epoch,
                                               &next);
 }
+
+/* The reference's gamera_no_apply_pressure_control
+ * (NO/nonorthogonal_step.c:74-171) on every face's active cells, as the MPI
+ * Earth driver applies it after a step
+ * (cubedsphere/tests/test_cubedsphere_mpi_earth_tile.c:459-479), with the
+ * counts and maxima accumulated over the faces. */
+int refh_pressure_control(refh_problem *p, double dt, double gamma, double density_floor,
+                          double pressure_floor, double light_speed,
+                          double low_density_threshold, double low_density_sound_speed,
+                          unsigned long long counts[2], double maxima[2]) {
+  if (p == NULL || counts == NULL || maxima == NULL) return -1;
+  counts[0] = counts[1] = 0;
+  maxima[0] = maxima[1] = 0.0;
+  const size_t lower[3] = {p->ghost, p->ghost, p->ghost};
+  const size_t upper[3] = {p->ghost + p->active[0], p->ghost + p->active[1],
+                           p->ghost + p->active[2]};
+  int failed = 0;
+  for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f) {
+    size_t low = 0, high = 0;
+    double sound = 0.0, pressure = 0.0;
+    failed |= gamera_no_apply_pressure_control(
+                  &p->storage[f], &p->grid[f], lower, upper, dt, gamma, density_floor,
+                  pressure_floor, light_speed, low_density_threshold,
+                  low_density_sound_speed, &low, &high, &sound, &pressure) != 0;
+    counts[0] += low;
+    counts[1] += high;
+    maxima[0] = fmax(maxima[0], sound);
+    maxima[1] = fmax(maxima[1], pressure);
+  }
+  return failed ? -1 : 0;
+}
