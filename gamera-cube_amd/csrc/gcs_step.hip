@@ -19,7 +19,14 @@
 //    recovered again: every other cell's six fluxes are unchanged;
 //  * halo stencils carry only their 2 x 2 nonzero-weight donors, summed in
 //    the reference's p-major / q-minor order (the other 60 terms are exact
-//    zeros for finite donors, cubedsphere_halo_fastpath.h:1-24).
+//    zeros for finite donors, cubedsphere_halo_fastpath.h:1-24);
+//  * the i- and j-face fluxes and the j- and i-edge EMFs stage their
+//    stencils in LDS tiles (same arithmetic per face or edge);
+//  * the next step's CFL dt is computed in the update and the seam
+//    recovery from the values they store, and min-folded (exact);
+//  * with fidelity 0 the work that only reproduces ghost values the step
+//    never reads (ghost-cell B, the old level's unchanged ghost face
+//    fluxes, post-update ghost copies) is skipped.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
