@@ -145,6 +145,30 @@ GCS_DEV void up7_pdm(const double vol[8], double iv, const double q[8],
   right = pdm(q[5], q[4], q[3], hr, a);
 }
 
+// a / b given y = 1 / b (IEEE-rounded): q = a y, then one correction with
+// the exact remainder a - b q. With y the correctly rounded reciprocal this
+// is the correctly rounded quotient (Markstein's theorem, P. Markstein, IBM
+// J. Res. Dev. 34(1), 1990), bit for bit a / b for normal operands, in
+// three operations instead of a full division sequence.
+GCS_DEV double div_by_reciprocal(double a, double b, double y) {
+  const double q = a * y;
+  const double r = fma(-q, b, a);
+  return fma(r, y, q);
+}
+
+// up7_pdm for the nine variables of one face, which share the interface
+// volume: its reciprocal riv = 1 / iv is computed once by the caller.
+GCS_DEV void up7_pdm_shared(const double vol[8], double iv, double riv, const double q[8],
+                            double a, double &left, double &right) {
+  double w[8];
+#pragma unroll
+  for (int n = 0; n < 8; ++n) w[n] = vol[n] * q[n];
+  const double hl = div_by_reciprocal(upwind7(w[0], w[1], w[2], w[3], w[4], w[5], w[6]), iv, riv);
+  const double hr = div_by_reciprocal(upwind7(w[7], w[6], w[5], w[4], w[3], w[2], w[1]), iv, riv);
+  left = pdm(q[2], q[3], q[4], hl, a);
+  right = pdm(q[5], q[4], q[3], hr, a);
+}
+
 GCS_DEV bool valid_interface(double rho, const double v[3], double p) {
   return isfinite(rho) && isfinite(p) && rho > DBL_MIN && p > DBL_MIN &&
          isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]);

@@ -934,19 +934,20 @@ GCS_DEV void face_flux_face(const K &c, int d, int f, long at, long st, const Ce
     c.fail(42, at);
     return;
   }
+  const double riv = 1.0 / iv;
   Prim w[2];
-  up7_pdm(vol, iv, q8, o.pdm, w[0].rho, w[1].rho);
+  up7_pdm_shared(vol, iv, riv, q8, o.pdm, w[0].rho, w[1].rho);
   for (int e = 0; e < 3; ++e) {
     for (int n = 0; n < 8; ++n) q8[n] = cell(1 + e, n);
-    up7_pdm(vol, iv, q8, o.pdm, w[0].v[e], w[1].v[e]);
+    up7_pdm_shared(vol, iv, riv, q8, o.pdm, w[0].v[e], w[1].v[e]);
   }
   for (int n = 0; n < 8; ++n) q8[n] = cell(4, n);
-  up7_pdm(vol, iv, q8, o.pdm, w[0].p, w[1].p);
+  up7_pdm_shared(vol, iv, riv, q8, o.pdm, w[0].p, w[1].p);
   double bb[2][3];
   for (int e = 0; e < 3; ++e) {
     double q8[8];
     for (int n = 0; n < 8; ++n) q8[n] = cell(6 + e, n);
-    up7_pdm(vol, iv, q8, o.pdm, bb[0][e], bb[1][e]);
+    up7_pdm_shared(vol, iv, riv, q8, o.pdm, bb[0][e], bb[1][e]);
   }
   if (!valid_interface(w[0].rho, w[0].v, w[0].p) ||
       !valid_interface(w[1].rho, w[1].v, w[1].p)) {
