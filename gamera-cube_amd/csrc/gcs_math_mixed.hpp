@@ -107,6 +107,25 @@ GCS_DEV void up7_pdm_r(const R vol[8], R iv, const R q[8], R a, R &left, R &righ
   right = pdm_r(q[5], q[4], q[3], hr, a);
 }
 
+// up7_pdm_r for the nine variables of one face, which share the interface
+// volume: a / iv as a * riv (riv = 1 / iv, once per face) plus one fused
+// correction, the correctly rounded quotient (gcs::div_by_reciprocal).
+template <typename R>
+GCS_DEV void up7_pdm_shared_r(const R vol[8], R iv, R riv, const R q[8], R a, R &left,
+                              R &right) {
+  R w[8];
+#pragma unroll
+  for (int n = 0; n < 8; ++n) w[n] = vol[n] * q[n];
+  auto div = [&](R x) {
+    const R y = x * riv;
+    return fma(fma(-y, iv, x), riv, y);
+  };
+  const R hl = div(upwind7_r(w[0], w[1], w[2], w[3], w[4], w[5], w[6]));
+  const R hr = div(upwind7_r(w[7], w[6], w[5], w[4], w[3], w[2], w[1]));
+  left = pdm_r(q[2], q[3], q[4], hl, a);
+  right = pdm_r(q[5], q[4], q[3], hr, a);
+}
+
 template <typename R>
 GCS_DEV bool valid_interface_r(R rho, const R v[3], R p) {
   return isfinite(rho) && isfinite(p) && rho > tiny<R>() && p > tiny<R>() &&

@@ -1484,19 +1484,20 @@ GCS_DEV void face_flux_face_fp32(const K &c, int d, int f, long at, long st, con
     c.fail(42, at);
     return;
   }
+  const R riv = R(1) / iv;
   PrimR<R> w[2];
-  up7_pdm_r(vol, iv, q8, pdm, w[0].rho, w[1].rho);
+  up7_pdm_shared_r(vol, iv, riv, q8, pdm, w[0].rho, w[1].rho);
   for (int e = 0; e < 3; ++e) {
     for (int n = 0; n < 8; ++n) q8[n] = cell(1 + e, n);
-    up7_pdm_r(vol, iv, q8, pdm, w[0].v[e], w[1].v[e]);
+    up7_pdm_shared_r(vol, iv, riv, q8, pdm, w[0].v[e], w[1].v[e]);
   }
   for (int n = 0; n < 8; ++n) q8[n] = cell(4, n);
-  up7_pdm_r(vol, iv, q8, pdm, w[0].p, w[1].p);
+  up7_pdm_shared_r(vol, iv, riv, q8, pdm, w[0].p, w[1].p);
   R bb[2][3];
   for (int e = 0; e < 3; ++e) {
     R q8[8];
     for (int n = 0; n < 8; ++n) q8[n] = cell(6 + e, n);
-    up7_pdm_r(vol, iv, q8, pdm, bb[0][e], bb[1][e]);
+    up7_pdm_shared_r(vol, iv, riv, q8, pdm, bb[0][e], bb[1][e]);
   }
   if (!valid_interface_r(w[0].rho, w[0].v, w[0].p) ||
       !valid_interface_r(w[1].rho, w[1].v, w[1].p)) {
