@@ -223,6 +223,8 @@ def workload(name):
         return earth_problem(8, 12)
     if name == "small":
         return earth_problem(16, 12, perturbation=0.01, seed=7)
+    if name == "ragged":       # odd extents: every tile and block range partial
+        return earth_problem(11, 13, perturbation=0.01, seed=3)
     if name == "small_mi":     # MI on a 32 x 12 polar grid (Pedersen only)
         return earth_problem(16, 12, perturbation=0.01, seed=7, mi=True,
                              mi_azimuth=32, mi_colatitude=12)

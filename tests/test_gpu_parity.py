@@ -31,7 +31,7 @@ def worst_error(a, b, fields):
     return worst
 
 
-@pytest.mark.parametrize("name", ["tiny", "small"])
+@pytest.mark.parametrize("name", ["tiny", "small", "ragged"])
 def test_setup_is_bitwise_the_reference_setup(gpu, ref, name):
     cfg = gcs_config.workload(name)
     print(f"setup parity {name}", flush=True)
@@ -45,7 +45,7 @@ def test_setup_is_bitwise_the_reference_setup(gpu, ref, name):
             assert np.array_equal(r.edge_valid(f, d), g.edge_valid(f, d))
 
 
-@pytest.mark.parametrize("name,steps", [("tiny", 10), ("small", 10)])
+@pytest.mark.parametrize("name,steps", [("tiny", 10), ("small", 10), ("ragged", 10)])
 def test_steps_match_oracle(gpu, orc, name, steps):
     cfg = gcs_config.workload(name)
     print(f"step parity {name} x{steps}", flush=True)

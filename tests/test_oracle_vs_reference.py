@@ -14,7 +14,7 @@ CHECKED = (problem.STATE_FIELDS
            + [problem.MAXWELL_FACE_FLUX + d for d in range(3)])
 
 
-@pytest.mark.parametrize("name,steps", [("tiny", 4), ("small", 12)])
+@pytest.mark.parametrize("name,steps", [("tiny", 4), ("small", 12), ("ragged", 6)])
 def test_oracle_steps_bitwise_equal_reference(ref, orc, name, steps):
     cfg = gcs_config.workload(name)
     r = problem.Problem(ref, "refh", cfg)
