@@ -1132,6 +1132,9 @@ struct EmfTile {
   static constexpr int E1 = T1 == 2 ? 32 : 8;  // edges along t1
   static constexpr int E2 = T2 == 2 ? 32 : 8;  // edges along t2
   static constexpr int C1 = E1 + 7, C2 = E2 + 7;
+  // Row stride of the t2-face arrays: when the edge loop walks t1 (= k)
+  // fastest, an odd stride keeps a wave's reads on distinct LDS banks.
+  static constexpr int S2 = T1 == 2 ? E2 + 1 : E2;
   // (position along e, along t1, along t2) -> (i, j, k)
   GCS_DEV static void ijk(int ce, int a1, int a2, int &i, int &j, int &k) {
     int x[3];
@@ -1147,11 +1150,11 @@ struct EmfTile {
 template <int E>
 __global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
   using T = EmfTile<E>;
-  constexpr int E1 = T::E1, E2 = T::E2, C1 = T::C1, C2 = T::C2;
+  constexpr int E1 = T::E1, E2 = T::E2, C1 = T::C1, C2 = T::C2, S2 = T::S2;
   constexpr int t1 = T::T1, t2 = T::T2;
   __shared__ double s_vol[C1 * C2], s_wv[3][C1 * C2];
   __shared__ double s_fv[3][E1 * C2], s_a1[E1 * C2], s_b1[E1 * C2];
-  __shared__ double s_a2[C1 * E2], s_b2[C1 * E2];
+  __shared__ double s_a2[C1 * S2], s_b2[C1 * S2];
   const Layout &L = c.L;
   const int act[3] = {L.nr, L.n, L.n};
   // edge box: [g, g + act) along e, [g, g + act + 1) along t1 and t2
@@ -1196,11 +1199,14 @@ __global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
   for (int m = 0; m < 5; ++m) solve[m] = c.p(ebp + EO_SOLVE + m)[at];
   const double b0t0 = c.p(P_B0ET + 2 * E)[at], b0t1 = c.p(P_B0ET + 2 * E + 1)[at];
   const double elen = c.p(ebp + EO_LEN)[at];
+  // The fills walk k fastest across the threads (along t2 for e = 0, along
+  // t1 for e = 1) so that each wave's loads are contiguous runs; the LDS
+  // layouts do not depend on the walk.
   // cells: t1 in [o1 - 4, o1 + n1 + 3), t2 in [o2 - 4, o2 + n2 + 3)
   {
     const int w1 = n1 + 7, w2 = n2 + 7;
     for (int q = tid; q < w1 * w2; q += 256) {
-      const int u1 = q / w2, u2 = q % w2;
+      const int u1 = t1 == 2 ? q % w1 : q / w2, u2 = t1 == 2 ? q / w1 : q % w2;
       const long at = at_of(o1 - 4 + u1, o2 - 4 + u2);
       const int l = u1 * C2 + u2;
       s_vol[l] = c.p(P_VOL)[at];
@@ -1212,7 +1218,7 @@ __global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
     const int w2 = n2 + 7;
     const double *area = c.p(P_FACE + FACE_PLANES * t1 + FO_AREA), *bn = c.p(P_BN + t1);
     for (int q = tid; q < n1 * w2; q += 256) {
-      const int u1 = q / w2, u2 = q % w2;
+      const int u1 = t1 == 2 ? q % n1 : q / w2, u2 = t1 == 2 ? q / n1 : q % w2;
       const long at = at_of(o1 + u1, o2 - 4 + u2);
       s_a1[u1 * C2 + u2] = area[at];
       s_b1[u1 * C2 + u2] = bn[at];
@@ -1223,10 +1229,10 @@ __global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
     const int w1 = n1 + 7;
     const double *area = c.p(P_FACE + FACE_PLANES * t2 + FO_AREA), *bn = c.p(P_BN + t2);
     for (int q = tid; q < w1 * n2; q += 256) {
-      const int u1 = q / n2, u2 = q % n2;
+      const int u1 = t1 == 2 ? q % w1 : q / n2, u2 = t1 == 2 ? q / w1 : q % n2;
       const long at = at_of(o1 - 4 + u1, o2 + u2);
-      s_a2[u1 * E2 + u2] = area[at];
-      s_b2[u1 * E2 + u2] = bn[at];
+      s_a2[u1 * S2 + u2] = area[at];
+      s_b2[u1 * S2 + u2] = bn[at];
     }
   }
   __syncthreads();
@@ -1279,7 +1285,7 @@ __global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
   {
     double a2[8], fld[8];
     for (int n = 0; n < 8; ++n) {
-      const int l = (e1 + n) * E2 + e2;  // t2-face at t1 = o1 + e1 + n - 4
+      const int l = (e1 + n) * S2 + e2;  // t2-face at t1 = o1 + e1 + n - 4
       a2[n] = s_a2[l];
       fld[n] = s_b2[l];
     }
