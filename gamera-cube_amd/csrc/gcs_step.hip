@@ -1119,7 +1119,7 @@ using FluxTileJ = FluxTile<1, 8, 16>;
 // contiguous run. It stages in LDS the cells the tile's t1-face velocities
 // need (vol * m / rho summands and volumes, 4 cells beyond the tile on each
 // side), the t1-faces' areas and B.n along t2 and the t2-faces' along t1,
-// computes the t1-face velocities once per face (k_face_velocity's
+// computes the t1-face velocities once per face (k_face_velocity_i's
 // arithmetic), and then runs k_edge_emf's arithmetic per edge from LDS. Every
 // value is computed by the same expressions in the same order as the two
 // unfused kernels, so the EMF is unchanged bit for bit; what changes is that
@@ -1236,7 +1236,7 @@ __global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
     }
   }
   __syncthreads();
-  // t1-face velocities (k_face_velocity, direction t1)
+  // t1-face velocities (k_face_velocity_i's central8, direction t1)
   {
     const int w2 = n2 + 7;
     for (int q = tid; q < n1 * w2; q += 256) {
@@ -1337,37 +1337,45 @@ __global__ void k_inner_trap(K c) {
 
 // ---- edge EMF --------------------------------------------------------------------
 
-// nonorthogonal_sweep.c:223-259: central8 cell velocity at faces of
-// direction d, over the box the EMF sweep reads. A failure leaves NaN,
-// which the EMF's finiteness test turns into the reference's error.
-__global__ void k_face_velocity(K c, int d) {
+// nonorthogonal_sweep.c:223-259: central8 cell velocity at the i-faces
+// (direction 0) the k-edge EMF sweep reads (j and k from g - 4 to g + n + 4).
+// A failure leaves NaN, which the EMF's finiteness test turns into the
+// reference's error. Each thread walks a run of RM consecutive i-faces of
+// one (j, k) column and keeps the RM + 7 cells their stencils share in
+// registers, so a cell is loaded once per run instead of once per stencil
+// use; each face is the same central8 expression.
+constexpr int kFaceVelocityRun = 8;
+
+template <int RM>
+__global__ void __launch_bounds__(128) k_face_velocity_i(K c) {
   const Layout &L = c.L;
-  int lo[3], hi[3];
-  const int ext[3] = {L.NI, L.NJ, L.NK};
-  const int act[3] = {L.nr, L.n, L.n};
-  for (int a = 0; a < 3; ++a) {
-    if (a == d) {
-      lo[a] = L.ghost;
-      hi[a] = L.ghost + act[a] + 1;
-    } else {
-      lo[a] = L.ghost - 4;
-      hi[a] = min(L.ghost + act[a] + 4, ext[a]);
+  const int nf = L.nr + 1;  // i-faces of the active box along a column
+  const int lo[3] = {0, L.ghost - 4, L.ghost - 4};
+  const int hi[3] = {(nf + RM - 1) / RM, min(L.ghost + L.n + 4, L.NJ),
+                     min(L.ghost + L.n + 4, L.NK)};
+  int f, r, j, k;
+  if (!box(tid_xcd(), lo, hi, f, r, j, k)) return;
+  const int i0 = L.ghost + r * RM;
+  const int nv = min(RM, L.ghost + nf - i0);  // faces of this run
+  const long si = L.di(), at0 = L.at(f, i0, j, k);
+  // cells i0 - 4 .. i0 + nv + 2
+  double vol[RM + 7], wv[3][RM + 7];
+#pragma unroll
+  for (int n = 0; n < RM + 7; ++n) {
+    const bool in = n < nv + 7;
+    const long cell = at0 + (n - 4) * si;
+    vol[n] = in ? c.p(P_VOL)[cell] : 0.0;
+    for (int e = 0; e < 3; ++e) wv[e][n] = in ? c.p(P_WV + e)[cell] : 0.0;
+  }
+#pragma unroll
+  for (int m = 0; m < RM; ++m) {
+    if (m < nv) {
+      const double iv = central8(vol + m);
+      const bool ok = isfinite(iv) && fabs(iv) > DBL_MIN;
+      for (int e = 0; e < 3; ++e)
+        c.p(P_FVEL + e)[at0 + m * si] = ok ? central8(wv[e] + m) / iv : NAN;
     }
   }
-  int f, i, j, k;
-  if (!box(tid_xcd(), lo, hi, f, i, j, k)) return;
-  const long at = L.at(f, i, j, k);
-  const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
-  double vol[8], wv[3][8];
-  for (int n = 0; n < 8; ++n) {
-    const long cell = at + (n - 4) * st;
-    vol[n] = c.p(P_VOL)[cell];
-    for (int e = 0; e < 3; ++e) wv[e][n] = c.p(P_WV + e)[cell];
-  }
-  const double iv = central8(vol);
-  const bool ok = isfinite(iv) && fabs(iv) > DBL_MIN;
-  for (int e = 0; e < 3; ++e)
-    c.p(P_FVEL + 3 * d + e)[at] = ok ? central8(wv[e]) / iv : NAN;
 }
 
 // nonorthogonal_sweep.c:624-752 for edges of direction e bounding the
@@ -1458,7 +1466,7 @@ __global__ void __launch_bounds__(128) k_edge_emf(K c, int e) {
 
 // ---- mixed precision: FP32 reconstruction, fluxes and EMF ---------------------
 //
-// The same sweeps as k_face_flux, k_face_velocity and k_edge_emf with the
+// The same sweeps as k_face_flux, k_face_velocity_i and k_edge_emf with the
 // arithmetic in float. Their per-step inputs (the predictor's caches, face
 // velocities) and the face geometry they use are read from FP32 planes,
 // already rounded; the face fluxes are stored to FP32 planes and the edge
@@ -2125,7 +2133,9 @@ int enqueue_phase(Problem &P, Phase phase) {
       if (P.mixed)
         k_face_velocity_fp32<<<nblocks(n, T), T, 0, s>>>(c, d);
       else
-        k_face_velocity<<<nblocks(n, T), T, 0, s>>>(c, d);
+        // 128 threads: the kernel's launch bound
+        k_face_velocity_i<kFaceVelocityRun><<<nblocks(6L * ((L.nr + kFaceVelocityRun) / kFaceVelocityRun) *
+                                                           (n / (6L * (L.nr + 1))), 128), 128, 0, s>>>(c);
     }
     for (int e = 0; e < 3; ++e) {
       const long n = 6L * (L.nr + (e != 0)) * (L.n + (e != 1)) * (L.n + (e != 2));
