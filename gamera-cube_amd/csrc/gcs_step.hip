@@ -1611,6 +1611,42 @@ __global__ void k_face_velocity_fp32(K c, int d) {
     c.sp(M_FVEL + 3 * d + e)[at] = ok ? central8_r(wv[e]) / iv : (R)NAN;
 }
 
+// The FP32 i-face velocities as the same register march as
+// k_face_velocity_i (runs of RM faces of one (j, k) column), with this
+// kernel's FP32 expressions.
+template <int RM>
+__global__ void __launch_bounds__(128) k_face_velocity_i_fp32(K c) {
+  using R = float;
+  using namespace mixed;
+  const Layout &L = c.L;
+  const int nf = L.nr + 1;
+  const int lo[3] = {0, L.ghost - 4, L.ghost - 4};
+  const int hi[3] = {(nf + RM - 1) / RM, min(L.ghost + L.n + 4, L.NJ),
+                     min(L.ghost + L.n + 4, L.NK)};
+  int f, r, j, k;
+  if (!box(tid_xcd(), lo, hi, f, r, j, k)) return;
+  const int i0 = L.ghost + r * RM;
+  const int nv = min(RM, L.ghost + nf - i0);
+  const long si = L.di(), at0 = L.at(f, i0, j, k);
+  R vol[RM + 7], wv[3][RM + 7];
+#pragma unroll
+  for (int n = 0; n < RM + 7; ++n) {
+    const bool in = n < nv + 7;
+    const long cell = at0 + (n - 4) * si;
+    vol[n] = in ? c.sp(M_VOL)[cell] : R(0);
+    for (int e = 0; e < 3; ++e) wv[e][n] = in ? c.sp(M_WV + e)[cell] : R(0);
+  }
+#pragma unroll
+  for (int m = 0; m < RM; ++m) {
+    if (m < nv) {
+      const R iv = central8_r(vol + m);
+      const bool ok = isfinite(iv) && fabs(iv) > tiny<R>();
+      for (int e = 0; e < 3; ++e)
+        c.sp(M_FVEL + e)[at0 + m * si] = ok ? central8_r(wv[e] + m) / iv : (R)NAN;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(128) k_edge_emf_fp32(K c, int e) {
   using R = float;
   using namespace mixed;
@@ -2130,7 +2166,10 @@ int enqueue_phase(Problem &P, Phase phase) {
       const int ext[3] = {L.NI, L.NJ, L.NK};
       for (int a = 0; a < 3; ++a)
         n *= a == d ? act[a] + 1 : std::min(L.ghost + act[a] + 4, ext[a]) - (L.ghost - 4);
-      if (P.mixed)
+      if (P.mixed && d == 0)  // 128 threads: the kernel's launch bound
+        k_face_velocity_i_fp32<kFaceVelocityRun><<<nblocks(6L * ((L.nr + kFaceVelocityRun) / kFaceVelocityRun) *
+                                                                (n / (6L * (L.nr + 1))), 128), 128, 0, s>>>(c);
+      else if (P.mixed)
         k_face_velocity_fp32<<<nblocks(n, T), T, 0, s>>>(c, d);
       else
         // 128 threads: the kernel's launch bound
