@@ -36,7 +36,9 @@ STEP_BYTES_PER_CELL = 1216
 # cell's U*, B*, volume (9 doubles), the face frame, B0, B0.n, Phi* (15),
 # fluid + Maxwell flux out (8). Edge EMF: edge frame, solve, B0.t, length
 # (14), two transverse faces' area/velocity/Phi* (7), rho* (1), EMF out (1).
-# Update: 66 doubles in, U/B/Phi out (11). Predict: 38 in, 11 out. The second
+# Update: 66 doubles in, U/B/Phi out (11). Predict: Phi, Phi_old, U, U_old,
+# face centroids, centroid, recovery constants, volume (38 in); Phi*, U*, B*
+# and the primitive, velocity and B.n caches (22 out). The second
 # entry is the unit count per launch as a function of (nr, n).
 STAGE_MODEL = {
     "face_flux_i": (256, lambda nr, n: 6 * (nr + 1) * n * n),
@@ -47,45 +49,128 @@ STAGE_MODEL = {
     "edge_emf_k": (184, lambda nr, n: 6 * (nr + 1) * (n + 1) * n),
     # the update also folds the new state's CFL dt (3 CFL lengths more)
     "update": (640, lambda nr, n: 6 * nr * n * n),
-    "predict": (392, lambda nr, n: 6 * (nr + 9) * (n + 9) * (n + 9)),
+    "predict": (480, lambda nr, n: 6 * (nr + 9) * (n + 9) * (n + 9)),
 }
 
 
-def cpu_baseline(gpu_problem, cfg, steps=2):
-    """The oracle's CPU step on the same workload, from the GPU's arrays."""
-    path = os.path.join(ROOT, "oracle", "liboracle.so")
+# The reference's production switches (scripts/run_earth.py:18-20): result-
+# preserving caches and the parallel planned halo fill, read with getenv by
+# the reference library while it steps.
+REFERENCE_FLAGS = ("GAMERA_CS_HALO_COMPACT", "GAMERA_CS_HALO_INDEX_CACHE",
+                   "GAMERA_CS_HALO_BLOCK_ACCESS", "GAMERA_CS_HALO_PARALLEL",
+                   "GAMERA_NO_EMF_FACE_CACHE", "GAMERA_NO_PRIMITIVE_CACHE")
+
+
+def cpu_baseline(gpu_problem, cfg, steps=2, kind="reference"):
+    """The CPU step on the same workload, started from the GPU's arrays:
+    kind "reference" is the reference library compiled from its sources
+    (oracle/_ref, with its production switches on), "port" the oracle (the
+    C restatement). Both run the driver loop without the MI coupling, whose
+    solve comes once per ~360 steps at config 1 and would dominate a
+    two-step sample; the inner wall is the GPU problem's, copied in."""
+    name = {"reference": os.path.join("_ref", "libgamera_ref.so"),
+            "port": "liboracle.so"}[kind]
+    path = os.path.join(ROOT, "oracle", name)
     if not os.path.exists(path):
         return None
     prob = importlib.import_module("gamera-cube_amd.problem")
     lib = ctypes.CDLL(path)
-    lib.orc_set_clock.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
-                                  ctypes.c_double, ctypes.c_long]
-    # the step's rate: the oracle runs without the MI coupling (whose solve
-    # comes once per ~360 steps at this size and would dominate a two-step
-    # CPU sample); the GPU problem's inner wall is its own, copied below
+    prefix = "refh" if kind == "reference" else "orc"
+    set_clock = getattr(lib, f"{prefix}_set_clock")
+    set_clock.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                          ctypes.c_double, ctypes.c_long]
     ocfg = type(cfg).from_buffer_copy(cfg)
     ocfg.mi_enabled = 0
-    o = prob.Problem(lib, "orc", ocfg)
+    if kind == "reference":
+        for flag in REFERENCE_FLAGS:
+            os.environ[flag] = "1"
+        o = prob.Problem(lib, prefix, ocfg, create="create_shell")
+    else:
+        o = prob.Problem(lib, prefix, ocfg)
     o.copy_fields(gpu_problem, prob.GEOMETRY_FIELDS + prob.BACKGROUND_FIELDS
                   + prob.STATE_FIELDS)
     for f in range(6):
         for d in range(3):
             o.edge_valid(f, d)[:] = gpu_problem.edge_valid(f, d)
     c = gpu_problem.clock()
-    lib.orc_set_clock(o._handle, c["time"], c["dt"], c["previous_dt"], c["steps"])
-    o.advance(1)  # first step builds the oracle's plans: not timed
+    set_clock(o._handle, c["time"], c["dt"], c["previous_dt"], c["steps"])
+    o.advance(1)  # first step builds the halo plans: not timed
     t0 = time.perf_counter()
     o.advance(steps)
     seconds = time.perf_counter() - t0
     o.close()
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     n, nr = cfg.angular_cells, cfg.radial_cells
+    what = ("the reference library compiled from its sources (oracle/_ref), OpenMP, "
+            "its production cache and halo switches on" if kind == "reference" else
+            "the oracle, a C restatement of the reference step, OpenMP")
     return {"value": cfg.active_cells * steps / seconds, "unit": "cell-updates/s",
-            "cores": cores, "kind": "port",
-            "sample": f"{steps} driver steps (after one untimed) of the oracle, a C "
-                      f"restatement of the reference step with OpenMP, on the same "
+            "cores": cores, "kind": kind,
+            "sample": f"{steps} driver steps (after one untimed) of {what}, on the same "
                       f"6x{n}^2x{nr} problem started from the GPU's arrays (no MI "
                       f"update inside the sample), {seconds:.1f} s"}
+
+
+def cpu_only(workload):
+    """The compiled reference (oracle/_ref, its own setup, production
+    switches on) and the oracle (copied from it) on the CPU at `workload`:
+    cell-updates/s with all OpenMP threads and with one."""
+    prob = importlib.import_module("gamera-cube_amd.problem")
+    cfgm = importlib.import_module("gamera-cube_amd.config")
+    cfg = cfgm.workload(workload)
+    for flag in REFERENCE_FLAGS:
+        os.environ[flag] = "1"
+    gomp = ctypes.CDLL("libgomp.so.1")
+    threads = gomp.omp_get_max_threads()
+    ref = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libgamera_ref.so"))
+    orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+    t0 = time.perf_counter()
+    r = prob.Problem(ref, "refh", cfg)
+    setup_s = time.perf_counter() - t0
+    o = prob.Problem(orc, "orc", cfg)
+    o.copy_fields(r, prob.GEOMETRY_FIELDS + prob.BACKGROUND_FIELDS + prob.STATE_FIELDS)
+    for f in range(6):
+        for d in range(3):
+            o.edge_valid(f, d)[:] = r.edge_valid(f, d)
+    r.advance(1)
+    o.advance(1)
+    out = {"workload": workload, "active_cells": cfg.active_cells,
+           "reference_setup_s": setup_s, "threads_all": threads}
+    for label, n, steps in (("all", threads, 20), ("one", 1, 4)):
+        gomp.omp_set_num_threads(n)
+        for kind, p in (("reference", r), ("port", o)):
+            t = time.perf_counter()
+            p.advance(steps)
+            sec = time.perf_counter() - t
+            out[f"{kind}_{label}_threads"] = {
+                "cell_updates_per_s": cfg.active_cells * steps / sec, "steps": steps,
+                "seconds": sec, "threads": n}
+            print(f"bench: {kind} {n} threads {steps} steps {sec:.1f} s", file=sys.stderr,
+                  flush=True)
+    gomp.omp_set_num_threads(threads)
+    print(json.dumps({"cpu_only": out}))
+    return 0
+
+
+def copy_peak(gib=2.0, iters=10):
+    """HBM bandwidth of a plain device-to-device copy (read + write bytes per
+    second) on this GPU, beside the 8 TB/s specification."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(iters):
+        b.copy_(a)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop)
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * 8 * n * iters / (ms * 1e-3) / 1e9
 
 
 def active_state(p, cfg, prob):
@@ -269,6 +354,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="config1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-kind", choices=("reference", "port"), default="reference",
+                    help="CPU baseline: the compiled reference (oracle/_ref) or the "
+                         "oracle's restatement")
+    ap.add_argument("--cpu-only", metavar="WORKLOAD",
+                    help="only time the compiled reference and the oracle on the CPU, "
+                         "on WORKLOAD set up by the reference itself, with all threads "
+                         "and with one")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="fp64: the drop-in's arithmetic; mixed: FP32 reconstruction, "
                          "fluxes and EMF with FP64 update, CT, CFL and ionosphere")
@@ -287,6 +379,8 @@ def main():
     ap.add_argument("--large-run", type=int, default=0, metavar="STEPS",
                     help="only run config 4 (FP64 and mixed, Bz switch) for STEPS steps")
     args = ap.parse_args()
+    if args.cpu_only:
+        return cpu_only(args.cpu_only)
     if args.copy_calibration:
         import torch
         a = torch.ones(args.copy_calibration, dtype=torch.float64, device="cuda")
@@ -368,6 +462,8 @@ def main():
                        (ms / args.steps + rep["solve_ms"] / steps_per_update) * 1e-3)}
     else:
         mi_info = "off: ionosphere potential held at zero (no-MI wall)"
+    peak_copy = copy_peak()
+    print(f"bench: copy kernel {peak_copy:.0f} GB/s", file=sys.stderr, flush=True)
     modeled = {k: v for k, v in stages.items() if k in STAGE_MODEL and v > 0}
     top = max(modeled, key=modeled.get)
     bytes_per_unit, units = STAGE_MODEL[top]
@@ -393,11 +489,18 @@ def main():
                    "precision": args.precision,
                    "mixed_form": mixed_info,
                    "simulated_s_per_step": sim_per_step,
-                   "hbm_fraction_step_model": STEP_BYTES_PER_CELL * value / (HBM_PEAK_GBS * 1e9)},
+                   "hbm_fraction_step_model": STEP_BYTES_PER_CELL * value / (HBM_PEAK_GBS * 1e9),
+                   "hbm_fraction_step_model_of_copy_peak":
+                       STEP_BYTES_PER_CELL * value / (peak_copy * 1e9)},
         "roofline": {"bound": "hbm", "kernel": top,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, top),
+                     "traffic_source": f"profiles/step_{args.workload}_hbm_traffic.json: "
+                                       "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of "
+                                       "this workload, calibrated on a copy kernel",
+                     "measured_copy_peak": peak_copy,
+                     "frac_of_measured_copy_peak": achieved / peak_copy,
                      "launch_ms": launch_ms, "bytes_per_unit": bytes_per_unit,
                      "units_per_launch": units(nr, n)},
         "stage_ms_per_step": {k: round(v / args.steps, 4) for k, v in stages.items() if v > 0},
@@ -408,9 +511,9 @@ def main():
     if args.warm_start and cfg.mi_enabled:
         result["config"]["warm_start"] = warm_start_report(g, cfg, args.warm_start)
     if not args.no_cpu_baseline:
-        print("bench: CPU baseline (oracle, same workload) ...",
+        print("bench: CPU baseline (compiled reference, same workload) ...",
               file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(g, cfg)
+        result["cpu_baseline"] = cpu_baseline(g, cfg, kind=args.cpu_kind)
     g.close()
     print(json.dumps(result))
     return 0

@@ -225,6 +225,9 @@ def workload(name):
         return earth_problem(16, 12, perturbation=0.01, seed=7)
     if name == "ragged":       # odd extents: every tile and block range partial
         return earth_problem(11, 13, perturbation=0.01, seed=3)
+    if name == "fast_wind":    # 1600 km/s wind = 16 code speed >= 1.5 x the Boris
+        # light speed 10: emergency interface diffusion fires on most faces
+        return earth_problem(16, 12, perturbation=0.01, seed=7, speed_km_s=1600.0)
     if name == "small_mi":     # MI on a 32 x 12 polar grid (Pedersen only)
         return earth_problem(16, 12, perturbation=0.01, seed=7, mi=True,
                              mi_azimuth=32, mi_colatitude=12)

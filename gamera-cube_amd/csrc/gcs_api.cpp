@@ -69,6 +69,7 @@ void free_problem(Problem *P) {
   (void)hipFree(P->scalars);
   (void)hipFree(P->cfl_partial);
   (void)hipFree(P->cfl_slots);
+  (void)hipFree(P->acc4);
   (void)hipFree(P->halo_cells);
   (void)hipFree(P->halo_faces);
   (void)hipFree(P->seams);
@@ -92,6 +93,7 @@ int allocate_problem(Problem &P) {
                 ((6L * P.L.nr * P.L.n * P.L.n + 255) / 256 + 1) * sizeof(unsigned long long)) !=
           hipSuccess ||
       hipMalloc(&P.cfl_slots, kCflSlots * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&P.acc4, 4 * sizeof(unsigned long long)) != hipSuccess ||
       hipMemset(P.planes, 0, nbytes) != hipSuccess ||
       hipMemset(P.bytes, 0, nflags) != hipSuccess ||
       hipEventCreate(&P.ev_start) != hipSuccess ||
@@ -289,6 +291,7 @@ namespace gcs {
 static int launch_step_graph(Problem &P) {
   Problem::StepGraph &g = P.graphs[P.level];
   const bool match = g.exec != nullptr && g.mixed == P.mixed && g.fidelity == P.fidelity &&
+                     g.plain == P.plain &&
                      g.old_halo_current == P.old_halo_current &&
                      g.cfl_folded == P.cfl_folded &&
                      g.b_recovered == P.b_recovered &&
@@ -323,6 +326,7 @@ static int launch_step_graph(Problem &P) {
       return -1;
     }
     g.mixed = P.mixed;
+    g.plain = P.plain;
     g.fidelity = P.fidelity;
     g.b_recovered = P.b_recovered;
     g.old_halo_current = P.old_halo_current;
@@ -484,6 +488,12 @@ int gcs_problem_pressure_control(Problem *P, double dt, double gamma, double den
   gcs::StepScalars s;
   if (gcs::read_scalars(*P, s) != 0) return -1;
   return s.failed ? gcs::check_failure(*P) : 0;
+}
+
+int gcs_problem_set_plain_kernels(Problem *P, int on) {
+  if (P == nullptr) return -1;
+  P->plain = on != 0;
+  return 0;
 }
 
 int gcs_problem_set_graph(Problem *P, int on) {

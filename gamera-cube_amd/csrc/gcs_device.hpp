@@ -135,6 +135,7 @@ struct Problem {
   StepScalars *scalars = nullptr;  // device
   unsigned long long *cfl_partial = nullptr;  // per-block CFL minima
   unsigned long long *cfl_slots = nullptr;    // kCflSlots CFL minima folded by the update
+  unsigned long long *acc4 = nullptr;         // 4 accumulator words (pressure control, div B)
   // Driver problems fold the new state's CFL dt into cfl_slots during the
   // update (drop-in mirrors take dt from the caller); cfl_folded: the slots
   // hold the current state's minimum.
@@ -163,7 +164,7 @@ struct Problem {
   struct StepGraph {
     hipGraphExec_t exec = nullptr;
     bool mixed = false, fidelity = false, b_recovered = false, old_halo_current = false,
-         cfl_folded = false;
+         cfl_folded = false, plain = false;
     const double *mi_potential = nullptr;
   };
   bool use_graph = false;
@@ -175,6 +176,9 @@ struct Problem {
   WindSeries wind_series;
   bool fidelity = true;            // reproduce ghost values of the storage
   bool mixed = false;              // FP32 reconstruction, fluxes and EMF
+  // One face or edge per thread in every face-flux, face-velocity and EMF
+  // sweep instead of the LDS tiles and register marches (same results).
+  bool plain = false;
   // The predictor caches and face fluxes of the last step are in planes32.
   bool last_step_mixed = false;
   // Cell B of both levels equals the recovery from their face fluxes (true

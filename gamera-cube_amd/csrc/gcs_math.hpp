@@ -148,12 +148,18 @@ GCS_DEV void up7_pdm(const double vol[8], double iv, const double q[8],
 // a / b given y = 1 / b (IEEE-rounded): q = a y, then one correction with
 // the exact remainder a - b q. With y the correctly rounded reciprocal this
 // is the correctly rounded quotient (Markstein's theorem, P. Markstein, IBM
-// J. Res. Dev. 34(1), 1990), bit for bit a / b for normal operands, in
-// three operations instead of a full division sequence.
+// J. Res. Dev. 34(1), 1990), bit for bit a / b, in three operations instead
+// of a full division sequence. The theorem needs the remainder to be exact,
+// i.e. a and the quotient well inside the normal range: zeros (whose sign
+// the sequence does not keep), infinities, NaNs, subnormal dividends and
+// quotients near underflow or overflow take the plain division instead.
 GCS_DEV double div_by_reciprocal(double a, double b, double y) {
   const double q = a * y;
   const double r = fma(-q, b, a);
-  return fma(r, y, q);
+  const double x = fma(r, y, q);
+  const double ax = fabs(a), xx = fabs(x);
+  if (__builtin_expect(ax >= 0x1p-960 && xx >= 0x1p-960 && xx <= 0x1p1000, 1)) return x;
+  return a / b;
 }
 
 // up7_pdm for the nine variables of one face, which share the interface

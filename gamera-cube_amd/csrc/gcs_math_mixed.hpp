@@ -109,7 +109,8 @@ GCS_DEV void up7_pdm_r(const R vol[8], R iv, const R q[8], R a, R &left, R &righ
 
 // up7_pdm_r for the nine variables of one face, which share the interface
 // volume: a / iv as a * riv (riv = 1 / iv, once per face) plus one fused
-// correction, the correctly rounded quotient (gcs::div_by_reciprocal).
+// correction, the correctly rounded quotient (gcs::div_by_reciprocal), with
+// the plain division outside the range where the correction is exact.
 template <typename R>
 GCS_DEV void up7_pdm_shared_r(const R vol[8], R iv, R riv, const R q[8], R a, R &left,
                               R &right) {
@@ -118,7 +119,11 @@ GCS_DEV void up7_pdm_shared_r(const R vol[8], R iv, R riv, const R q[8], R a, R 
   for (int n = 0; n < 8; ++n) w[n] = vol[n] * q[n];
   auto div = [&](R x) {
     const R y = x * riv;
-    return fma(fma(-y, iv, x), riv, y);
+    const R z = fma(fma(-y, iv, x), riv, y);
+    const R lo = (R)(sizeof(R) == 4 ? 0x1p-100 : 0x1p-960);
+    const R hi = (R)(sizeof(R) == 4 ? 0x1p120 : 0x1p1000);
+    if (__builtin_expect(fabs(x) >= lo && fabs(z) >= lo && fabs(z) <= hi, 1)) return z;
+    return x / iv;
   };
   const R hl = div(upwind7_r(w[0], w[1], w[2], w[3], w[4], w[5], w[6]));
   const R hr = div(upwind7_r(w[7], w[6], w[5], w[4], w[3], w[2], w[1]));

@@ -1378,6 +1378,39 @@ __global__ void __launch_bounds__(128) k_face_velocity_i(K c) {
   }
 }
 
+// nonorthogonal_sweep.c:223-259 one face per thread for the faces of
+// direction d that the EMF sweeps read (d from g to g + n, the other axes
+// from g - 4 to g + n + 4): the plain counterpart of k_face_velocity_i and
+// of the face velocities inside k_edge_emf_tiled, same expression.
+__global__ void __launch_bounds__(128) k_face_velocity(K c, int d) {
+  const Layout &L = c.L;
+  int lo[3], hi[3];
+  const int ext[3] = {L.NI, L.NJ, L.NK};
+  const int act[3] = {L.nr, L.n, L.n};
+  for (int a = 0; a < 3; ++a) {
+    if (a == d) {
+      lo[a] = L.ghost;
+      hi[a] = L.ghost + act[a] + 1;
+    } else {
+      lo[a] = L.ghost - 4;
+      hi[a] = min(L.ghost + act[a] + 4, ext[a]);
+    }
+  }
+  int f, i, j, k;
+  if (!box(tid_xcd(), lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  const long st = d == 0 ? L.di() : d == 1 ? (long)L.SK : 1;
+  double vol[8], wv[3][8];
+  for (int n = 0; n < 8; ++n) {
+    const long cell = at + (n - 4) * st;
+    vol[n] = c.p(P_VOL)[cell];
+    for (int e = 0; e < 3; ++e) wv[e][n] = c.p(P_WV + e)[cell];
+  }
+  const double iv = central8(vol);
+  const bool ok = isfinite(iv) && fabs(iv) > DBL_MIN;
+  for (int e = 0; e < 3; ++e) c.p(P_FVEL + 3 * d + e)[at] = ok ? central8(wv[e]) / iv : NAN;
+}
+
 // nonorthogonal_sweep.c:624-752 for edges of direction e bounding the
 // active box.
 __global__ void __launch_bounds__(128) k_edge_emf(K c, int e) {
@@ -1515,8 +1548,29 @@ GCS_DEV void face_flux_face_fp32(const K &c, int d, int f, long at, long st, con
   }
   if (!valid_interface_r(w[0].rho, w[0].v, w[0].p) ||
       !valid_interface_r(w[1].rho, w[1].v, w[1].p)) {
-    c.fail(43, at);
-    return;
+    // The limiter's last subtraction cancels in FP32 when an interface
+    // density or pressure is orders of magnitude below a neighbour's (gas
+    // near its floors beside the wind): it can land on zero or below. Such
+    // a face is reconstructed again in FP64 from the same stencil values
+    // (widened exactly) and rounded once, which keeps it positive.
+    double vd[8], qd[8];
+    for (int n = 0; n < 8; ++n) vd[n] = (double)cell(5, n);
+    const double ivd = central8(vd);
+    double side[2][5];
+    for (int v = 0; v < 5; ++v) {
+      for (int n = 0; n < 8; ++n) qd[n] = (double)cell(v, n);
+      up7_pdm(vd, ivd, qd, o.pdm, side[0][v], side[1][v]);
+    }
+    for (int s = 0; s < 2; ++s) {
+      w[s].rho = (R)side[s][0];
+      for (int e = 0; e < 3; ++e) w[s].v[e] = (R)side[s][1 + e];
+      w[s].p = (R)side[s][4];
+    }
+    if (!valid_interface_r(w[0].rho, w[0].v, w[0].p) ||
+        !valid_interface_r(w[1].rho, w[1].v, w[1].p)) {
+      c.fail(43, at);
+      return;
+    }
   }
   const int fb = M_FACE + M_FACE_PLANES * d;
   const R area = c.sp(fb + MF_AREA)[at];
@@ -2134,6 +2188,8 @@ int enqueue_phase(Problem &P, Phase phase) {
       // are faster without tiles.
       if (P.mixed)
         k_face_flux_fp32<<<nblocks(n, 128), 128, 0, s>>>(c, d);
+      else if (P.plain)
+        k_face_flux<<<nblocks(n, 128), 128, 0, s>>>(c, d);
       else if (d == 0)
         k_face_flux_tiled<0, FluxTileI::TD, FluxTileI::TT>
             <<<FluxTileI::blocks(L.nr, L.n), FluxTileI::kThreads, 0, s>>>(c);
@@ -2161,12 +2217,14 @@ int enqueue_phase(Problem &P, Phase phase) {
     const int act[3] = {L.nr, L.n, L.n};
     // FP64: the i-face velocities for the k-edges only; the j- and i-edges
     // compute their t1-face velocities inside their LDS tiles.
-    for (int d = 0; d < (P.mixed ? 3 : 1); ++d) {
+    for (int d = 0; d < (P.mixed || P.plain ? 3 : 1); ++d) {
       long n = 6;
       const int ext[3] = {L.NI, L.NJ, L.NK};
       for (int a = 0; a < 3; ++a)
         n *= a == d ? act[a] + 1 : std::min(L.ghost + act[a] + 4, ext[a]) - (L.ghost - 4);
-      if (P.mixed && d == 0)  // 128 threads: the kernel's launch bound
+      if (P.plain && !P.mixed)
+        k_face_velocity<<<nblocks(n, 128), 128, 0, s>>>(c, d);
+      else if (P.mixed && d == 0 && !P.plain)  // 128 threads: the kernel's launch bound
         k_face_velocity_i_fp32<kFaceVelocityRun><<<nblocks(6L * ((L.nr + kFaceVelocityRun) / kFaceVelocityRun) *
                                                                 (n / (6L * (L.nr + 1))), 128), 128, 0, s>>>(c);
       else if (P.mixed)
@@ -2181,7 +2239,7 @@ int enqueue_phase(Problem &P, Phase phase) {
       mark_stage(P, S_EDGE_EMF0 + e);
       if (P.mixed) {
         k_edge_emf_fp32<<<nblocks(n, 128), 128, 0, s>>>(c, e);
-      } else if (e == 2) {
+      } else if (e == 2 || P.plain) {
         k_edge_emf<<<nblocks(n, 128), 128, 0, s>>>(c, e);
       } else {
         const int t1 = (e + 1) % 3, t2 = (e + 2) % 3;
@@ -2416,15 +2474,13 @@ int pressure_control(Problem &P, const double args[7], unsigned long long counts
     set_error("pressure control: invalid arguments");
     return -1;
   }
-  unsigned long long *acc = nullptr, host[4] = {0, 0, 0, 0};
-  GCS_HIP(hipMalloc(&acc, sizeof(host)));
+  unsigned long long *acc = P.acc4, host[4] = {0, 0, 0, 0};
   k_zero4<<<1, 4, 0, P.stream>>>(acc);
   k_pressure_control<<<nblocks(6L * P.L.nr * P.L.n * P.L.n, 256), 256, 0, P.stream>>>(
       make_k(P), a, acc);
   const hipError_t e = hipGetLastError();
   const hipError_t m = hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, P.stream);
   const hipError_t w = hipStreamSynchronize(P.stream);
-  (void)hipFree(acc);
   if (e != hipSuccess || m != hipSuccess || w != hipSuccess) {
     set_error("pressure control: device error");
     return -1;
@@ -2440,15 +2496,13 @@ int pressure_control(Problem &P, const double args[7], unsigned long long counts
 }
 
 int max_div_b(Problem &P, double *out) {
-  unsigned long long *dev = nullptr, host = 0;
-  GCS_HIP(hipMalloc(&dev, sizeof(host)));
-  GCS_HIP(hipMemsetAsync(dev, 0, sizeof(host), P.stream));
+  unsigned long long *dev = P.acc4, host = 0;
+  k_zero4<<<1, 4, 0, P.stream>>>(dev);
   K c = make_k(P);
   k_div_b<<<nblocks(6L * P.L.nr * P.L.n * P.L.n, 256), 256, 0, P.stream>>>(c, dev);
   const hipError_t e = hipGetLastError();
   const hipError_t m = hipMemcpyAsync(&host, dev, sizeof(host), hipMemcpyDeviceToHost, P.stream);
   const hipError_t w = hipStreamSynchronize(P.stream);
-  (void)hipFree(dev);
   if (e != hipSuccess || m != hipSuccess || w != hipSuccess) {
     set_error("divergence check failed");
     return -1;

@@ -191,3 +191,14 @@ def _max_div_b(self):
 
 
 GpuProblem.max_div_b = _max_div_b
+
+
+def _set_plain_kernels(self, on):
+    """True: one face or edge per thread in the face-flux, face-velocity and
+    EMF sweeps (no LDS tiles or register marches); results are identical."""
+    self._lib.gcs_problem_set_plain_kernels.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    if self._lib.gcs_problem_set_plain_kernels(self._handle, 1 if on else 0) != 0:
+        raise RuntimeError("gcs_problem_set_plain_kernels failed")
+
+
+GpuProblem.set_plain_kernels = _set_plain_kernels

@@ -48,7 +48,7 @@ BACKGROUND_FIELDS = ([BG_CELL_MAGNETIC, BG_CELL_FORCE]
                      + [BG_EDGE_MAGNETIC + d for d in range(3)])
 
 
-def bind(lib, prefix):
+def bind(lib, prefix, create="create"):
     """Declare argument/return types of one problem API on a CDLL."""
     def f(name, restype, *argtypes):
         fn = getattr(lib, f"{prefix}_{name}")
@@ -57,7 +57,7 @@ def bind(lib, prefix):
         return fn
     vp = ctypes.c_void_p
     return {
-        "create": f("create", vp, ctypes.POINTER(ProblemConfig)),
+        "create": f(create, vp, ctypes.POINTER(ProblemConfig)),
         "destroy": f("destroy", None, vp),
         "advance": f("advance", ctypes.c_int, vp, ctypes.c_long),
         "step_explicit": f("step_explicit", ctypes.c_int, vp, ctypes.c_double,
@@ -76,10 +76,13 @@ def bind(lib, prefix):
 class Problem:
     """One six-face problem owned by a native library."""
 
-    def __init__(self, lib, prefix, config):
+    def __init__(self, lib, prefix, config, create="create"):
+        """`create` names the constructor: `<prefix>_create` sets the
+        problem up; the CPU implementations also export constructors that
+        only allocate (the caller copies the arrays in)."""
         if not isinstance(config, ProblemConfig):
             raise TypeError("config must be a ProblemConfig")
-        self._api = bind(lib, prefix)
+        self._api = bind(lib, prefix, create)
         self._lib = lib
         self.prefix = prefix
         self.config = config

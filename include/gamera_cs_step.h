@@ -105,6 +105,14 @@ int gcs_problem_pressure_control(gcs_problem *problem, double dt, double gamma,
  * either way. No reference counterpart. */
 int gcs_problem_set_graph(gcs_problem *problem, int on);
 
+/* 1: run every face-flux, face-velocity and edge-EMF sweep with one face or
+ * edge per thread (the untiled kernels), in FP64 and in the mixed form.
+ * 0 (default): the LDS-tiled and register-march kernels. Both compute each
+ * face and edge with the same expressions, so results are bit-identical;
+ * the switch lets tests hold the tiled kernels to the plain ones at any
+ * size. No reference counterpart. */
+int gcs_problem_set_plain_kernels(gcs_problem *problem, int on);
+
 /* Sample the field-aligned current of the current state onto the MI polar
  * grids (cubedsphere_mi_dipole.c:223-370) without solving; the potential
  * and the update clock are unchanged. gcs_problem_mi_field(what = 1) then

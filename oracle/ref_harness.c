@@ -280,8 +280,127 @@ fail:
   return NULL;
 }
 
+/* The problem's arrays in the reference's layout, allocated as
+ * gamera_no_grid_create, gamera_no_storage_create and
+ * gamera_no_background_create allocate them (so the reference's destroy
+ * functions release them), but left for the caller to fill: geometry,
+ * background and state are copied in from another problem of the same
+ * config, which skips the reference's setup (minutes of CPU at 6 x 96^2 x
+ * 192) when only its step is to be timed. The driver options, wind and
+ * Earth boundary are those of refh_create; the halo plan and the first dt
+ * are made on the first step, from the copied geometry and state. No MI. */
+refh_problem *refh_create_shell(const refh_config *cfg) {
+  if (cfg == NULL || cfg->ghost < 4 || cfg->wind_count < 1 ||
+      cfg->wind_count > REFH_MAX_WIND || cfg->mi_enabled)
+    return NULL;
+  refh_problem *p = calloc(1, sizeof(*p));
+  if (p == NULL) return NULL;
+  p->cfg = *cfg;
+  p->step = gamera_cs_advance_six_with_background_planned;
+  p->radial = gamera_cs_earth_fill_radial_ghosts;
+  p->ghost = cfg->ghost;
+  p->active[0] = cfg->radial_cells;
+  p->active[1] = p->active[2] = cfg->angular_cells;
+  for (int d = 0; d < 3; ++d) p->extent[d] = p->active[d] + 2 * p->ghost;
+  gamera_solar_wind_init(&p->wind);
+  const gamera_cs_earth_startup *startup = &p->cfg.startup;
+  const double gamma = startup->gamma;
+  for (int face = 0; face < GAMERA_CS_FACE_COUNT; ++face) {
+    gamera_no_grid *g = &p->grid[face];
+    gamera_no_background_data *b = &p->owned[face];
+    for (int a = 0; a < 3; ++a) {
+      g->cell_extent[a] = p->extent[a];
+      g->vertex_extent[a] = p->extent[a] + 1;
+    }
+    for (int d = 0; d < 3; ++d)
+      for (int a = 0; a < 3; ++a) {
+        g->face[d].extent[a] = p->extent[a] + (a == d ? 1U : 0U);
+        g->edge[d].extent[a] = p->extent[a] + (a == d ? 0U : 1U);
+      }
+    const size_t cells = gamera_no_element_count3(g->cell_extent);
+    g->vertex = calloc(gamera_no_element_count3(g->vertex_extent), sizeof(*g->vertex));
+    g->cell = calloc(cells, sizeof(*g->cell));
+    b->cell_magnetic = calloc(cells, sizeof(*b->cell_magnetic));
+    b->cell_force = calloc(cells, sizeof(*b->cell_force));
+    int ok = g->vertex != NULL && g->cell != NULL && b->cell_magnetic != NULL &&
+             b->cell_force != NULL;
+    for (int d = 0; d < 3; ++d) {
+      const size_t faces = gamera_no_element_count3(g->face[d].extent);
+      const size_t edges = gamera_no_element_count3(g->edge[d].extent);
+      g->face[d].value = calloc(faces, sizeof(*g->face[d].value));
+      g->edge[d].value = calloc(edges, sizeof(*g->edge[d].value));
+      g->edge[d].valid = calloc(edges, 1);
+      b->face_magnetic[d] = calloc(faces, sizeof(*b->face_magnetic[d]));
+      b->face_flux[d] = calloc(faces, sizeof(*b->face_flux[d]));
+      b->edge_magnetic[d] = calloc(edges, sizeof(*b->edge_magnetic[d]));
+      ok = ok && g->face[d].value != NULL && g->edge[d].value != NULL &&
+           g->edge[d].valid != NULL && b->face_magnetic[d] != NULL &&
+           b->face_flux[d] != NULL && b->edge_magnetic[d] != NULL;
+      b->field.face_magnetic[d] = b->face_magnetic[d];
+      b->field.face_flux[d] = b->face_flux[d];
+      b->field.edge_magnetic[d] = b->edge_magnetic[d];
+    }
+    b->field.cell_magnetic = b->cell_magnetic;
+    b->field.cell_force = b->cell_force;
+    if (!ok || gamera_no_storage_create(p->extent, &p->storage[face]) != 0) goto fail;
+    p->background[face] = b->field;
+  }
+  p->timestep = (gamera_no_timestep_options){
+      gamma, 1e-6, 1e-8, cfg->cfl, 10.0, true, cfg->use_boris != 0, true};
+  p->options = (gamera_no_advance_options){
+      .stress = {gamma, 1e-6, 1e-8, 1.0, 0.25, 0.25, 10.0, true, true,
+                 cfg->use_boris != 0, true},
+      .emf = {1e-6, 1.0, 0.5, 10.0, cfg->cfl, 0.0, cfg->use_boris != 0, true},
+      .update = {gamma, 1e-6, 1e-8, 10.0, true, cfg->use_boris != 0, true}};
+  gamera_solar_wind_state states[REFH_MAX_WIND];
+  for (size_t n = 0; n < cfg->wind_count; ++n) {
+    states[n] = (gamera_solar_wind_state){
+        .density = startup->density,
+        .pressure = startup->density * startup->speed * startup->speed /
+                    (startup->gamma * startup->mach * startup->mach),
+        .velocity = {startup->speed, 0.0, 0.0},
+        .magnetic = {startup->imf.value[0], startup->imf.value[1],
+                     startup->imf.value[2]}};
+    if (cfg->wind_count > 1) states[n].magnetic[GAMERA_WIND_Z] = cfg->wind_bz[n];
+  }
+  p->wind.linear_interpolation = cfg->wind_count > 1 ? cfg->wind_linear : 1;
+  if (gamera_solar_wind_set(&p->wind, cfg->wind_count, cfg->wind_time, states) != 0)
+    goto fail;
+  p->wind.reference[0] = cfg->wind_reference_x;
+  p->boundary = (gamera_cs_earth_boundary){
+      .startup = startup, .wind = &p->wind, .background = p->background,
+      .density_floor = 1e-6, .pressure_floor = 1e-8, .cfl = cfg->cfl,
+      .strict_inner_gas_wall = cfg->strict_inner_gas_wall};
+  p->options.fluid_flux_sync = gamera_cs_earth_inner_fluid_flux;
+  p->options.edge_emf_sync = gamera_cs_earth_no_mi_emf;
+  p->options.edge_emf_context = &p->boundary;
+  return p;
+fail:
+  refh_destroy(p);
+  return NULL;
+}
+
+/* Clock of the driver loop (time, dt of the last step, previous dt, steps),
+ * e.g. to continue a run another implementation started. */
+void refh_set_clock(refh_problem *p, double time, double dt, double previous_dt,
+                    long steps) {
+  if (p == NULL) return;
+  p->time = time;
+  p->dt = dt;
+  p->previous_dt = previous_dt;
+  p->steps = steps;
+}
+
 /* One driver iteration: dt rule, MI cadence, one six-face step. */
 static int advance_one(refh_problem *p) {
+  if (p->halo_plan == NULL &&
+      gamera_cs_halo_plan_create(p->grid, p->active, p->ghost, &p->halo_plan) != 0)
+    return -1;
+  if (p->steps == 0 && !(p->dt > 0.0)) {
+    const double limit = cfl_limit(p);
+    if (!(limit > 0.0)) return -1;
+    p->dt = p->cfg.dt_fraction * limit;
+  }
   if (p->steps > 0) {
     const double limit = cfl_limit(p);
     if (!(limit > 0.0)) return -1;

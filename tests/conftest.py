@@ -57,6 +57,76 @@ def scaled_error(a, b):
     return float(np.max(np.abs(a - b) / np.maximum(1.0, np.abs(a))))
 
 
+def relative_error(a, b, ncomp=1):
+    """max over the components c of max |a_c - b_c| / max |a_c|: the error
+    relative to each variable's own largest magnitude, which stays relative
+    for arrays whose values are all below 1 (residual B, face fluxes, EMFs),
+    where the scaled measure above is absolute."""
+    a = np.asarray(a, dtype=np.float64).reshape(-1, ncomp)
+    b = np.asarray(b, dtype=np.float64).reshape(-1, ncomp)
+    if a.size == 0:
+        return 0.0
+    scale = np.abs(a).max(axis=0)
+    diff = np.abs(a - b).max(axis=0)
+    return float(np.max(np.where(scale > 0.0, diff / np.where(scale > 0.0, scale, 1.0),
+                                 diff)))
+
+
+# components per element of the state fields (problem.STATE_FIELDS order)
+STATE_COMPONENTS = [5, 5, 3, 3, 1, 1, 1, 1, 1, 1]
+
+# The state's physical quantities: (field ids, components per element,
+# component slices), each quantity's scale being its largest magnitude over
+# all six faces (and, for the face flux, all three directions).
+STATE_QUANTITIES = {
+    "rho": ([problem.CONSERVED], 5, slice(0, 1)),
+    "momentum": ([problem.CONSERVED], 5, slice(1, 4)),
+    "energy": ([problem.CONSERVED], 5, slice(4, 5)),
+    "old rho": ([problem.OLD_CONSERVED], 5, slice(0, 1)),
+    "old momentum": ([problem.OLD_CONSERVED], 5, slice(1, 4)),
+    "old energy": ([problem.OLD_CONSERVED], 5, slice(4, 5)),
+    "B": ([problem.CELL_MAGNETIC], 3, slice(0, 3)),
+    "old B": ([problem.OLD_CELL_MAGNETIC], 3, slice(0, 3)),
+    "face flux": ([problem.FACE_FLUX + d for d in range(3)], 1, slice(0, 1)),
+    "old face flux": ([problem.OLD_FACE_FLUX + d for d in range(3)], 1, slice(0, 1)),
+}
+
+
+def state_relative_errors(a, b):
+    """{quantity: max |a - b| / max |a|} over the state of two problems:
+    the error of each conserved variable (density, momentum vector, energy),
+    the residual B vector and the face flux relative to that quantity's own
+    scale. Unlike the scaled measure, it stays relative for quantities whose
+    values are all below 1; unlike a per-component one, it does not divide
+    by a component that is zero up to rounding (a momentum component
+    across the flow at the start of a run)."""
+    out = {}
+    for name, (fids, nc, comp) in STATE_QUANTITIES.items():
+        scale = diff = 0.0
+        for fid in fids:
+            for f in range(gcs_config.FACE_COUNT):
+                x = np.asarray(a.field(f, fid)).reshape(-1, nc)[:, comp]
+                y = np.asarray(b.field(f, fid)).reshape(-1, nc)[:, comp]
+                scale = max(scale, float(np.abs(x).max()))
+                diff = max(diff, float(np.abs(x - y).max()))
+        out[name] = diff / scale if scale > 0.0 else diff
+    return out
+
+
+def counters(p, face):
+    """(emergency-diffusion face count, maximum interface speed) that the
+    steps so far accumulated into one cube face's storage
+    (nonorthogonal_storage.h:30-31), from any of the three problem APIs."""
+    if hasattr(p, "counters"):
+        return tuple(p.counters(face))
+    out = (ctypes.c_double * 2)()
+    fn = getattr(p._lib, f"{p.prefix}_counters")
+    fn.restype = None
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    fn(p._handle, face, out)
+    return (out[0], out[1])
+
+
 def bits_equal(a, b):
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
@@ -72,6 +142,25 @@ def oracle_from(source, orc_lib, cfg):
         for d in range(3):
             o.edge_valid(f, d)[:] = source.edge_valid(f, d)
     return o
+
+
+def reference_from(source, ref_lib, cfg):
+    """The compiled reference's problem holding a copy of `source`'s inputs
+    (allocated by refh_create_shell, so the reference's setup is skipped);
+    its clock continues `source`'s."""
+    r = problem.Problem(ref_lib, "refh", cfg, create="create_shell")
+    r.copy_fields(source, problem.GEOMETRY_FIELDS + problem.BACKGROUND_FIELDS
+                  + problem.STATE_FIELDS)
+    for f in range(gcs_config.FACE_COUNT):
+        for d in range(3):
+            r.edge_valid(f, d)[:] = source.edge_valid(f, d)
+    c = source.clock()
+    fn = ref_lib.refh_set_clock
+    fn.restype = None
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                   ctypes.c_long]
+    fn(r._handle, c["time"], c["dt"], c["previous_dt"], c["steps"])
+    return r
 
 
 def active_view(p, face, field_id, cfg, ncomp):

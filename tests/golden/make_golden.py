@@ -238,6 +238,36 @@ def generate(ref):
             st = fn(face, edge, ctypes.byref(x))
             nb.append([face, edge, st, x.face, x.edge, x.reversed])
     out["neighbor"] = np.array(nb)
+    # emergency interface diffusion (nonorthogonal_flux.c:206-241): interface
+    # speeds drawn around the 1.5 c threshold, light speeds 10 and 2, Boris
+    # off, an exactly-at-threshold pair, a non-physical interface and a
+    # non-finite cell value part-way through the variables.
+    ein, eout = [], []
+    for n in range(N):
+        c = 10.0 if n % 2 else 2.0
+        boris = n % 11 != 5
+        wl, wr = state(rng), state(rng)
+        scale = rng.uniform(0.5, 2.0) * 1.5 * c
+        for w in (wl, wr):
+            v = np.array(w[1:4])
+            w[1:4] = list(v / np.linalg.norm(v) * scale * rng.uniform(0.8, 1.2))
+        if n % 13 == 3:      # sqrt(0.5 (225 + 225)) == 15 == 1.5 c exactly
+            c = 10.0
+            wl[1:4], wr[1:4] = [15.0, 0.0, 0.0], [0.0, -15.0, 0.0]
+        if n % 17 == 7:
+            wr[0] = -1.0     # rejected interface state
+        lo, up = rng.normal(0, 3, 5), rng.normal(0, 3, 5)
+        if n % 19 == 9:
+            up[3] = np.inf   # the first three variables are already updated
+        fl = R.FluidFlux()
+        flux0 = rng.normal(size=5)
+        fl.conserved[:] = list(flux0)
+        applied = ctypes.c_bool(False)
+        st = f["emergency"](ctypes.byref(fl), (R.Primitive * 2)(prim(wl), prim(wr)),
+                            arr(lo), arr(up), boris, c, ctypes.byref(applied))
+        ein.append([c, float(boris), *wl, *wr, *lo, *up, *flux0])
+        eout.append([st, float(applied.value), *fl.conserved])
+    out["emerg_in"], out["emerg_out"] = np.array(ein), np.array(eout)
     return out
 
 

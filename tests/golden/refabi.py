@@ -76,6 +76,10 @@ FUNCTIONS = {
     "hogs": ("gamera_no_apply_hogs", "orc_hogs", I,
              [ctypes.POINTER(FluidFlux), ctypes.POINTER(MaxwellFlux), D, D, B,
               D]),
+    "emergency": ("gamera_no_apply_emergency_interface_diffusion",
+                  "orc_emergency_diffusion", I,
+                  [ctypes.POINTER(FluidFlux), ctypes.POINTER(Primitive), PD, PD,
+                   B, D, ctypes.POINTER(B)]),
     "boris": ("gamera_no_apply_boris", "orc_apply_boris", I,
               [PD, PD, Vec3, Vec3, PD, Vec3, D, D, D, D, D]),
     "edge_emf": ("gamera_no_compute_edge_emf", "orc_edge_emf_formula", I,

@@ -5,15 +5,21 @@ The device solves use parallel reductions (and, with a Hall conductance, a
 ring preconditioner in place of the reference's sequential symmetric
 Gauss-Seidel), so potentials agree to the solve's tolerance rather than
 bitwise; the field-aligned current, computed from the same field, agrees to
-rounding."""
+rounding. Both sides therefore solve to a residual of 1e-13 max(|rhs|, 1)
+here (the workloads' 1e-10 leaves potentials 1e-10 apart), which puts the
+MI-coupled state within the per-step tolerance of the uncoupled step."""
 import ctypes
 
 import numpy as np
 import pytest
 
-from conftest import gcs_config, oracle_from, problem, scaled_error
+from conftest import (gcs_config, oracle_from, problem, scaled_error,
+                      state_relative_errors)
 
 pytestmark = pytest.mark.gpu
+
+STEP_TOL = 1e-13
+MI_TOL = 1e-13
 
 VP = ctypes.c_void_p
 
@@ -30,17 +36,20 @@ def mi_arrays(orc, o, n):
 @pytest.mark.parametrize("name", ["small_mi", "small_mi_hall"])
 def test_mi_run_matches_oracle(gpu, orc, name):
     cfg = gcs_config.workload(name)
+    cfg.mi.relative_tolerance = MI_TOL
     print(f"mi parity {name}", flush=True)
     g = gpu.GpuProblem(cfg)
     o = oracle_from(g, orc, cfg)
     n = 2 * cfg.mi.azimuth_points * cfg.mi.colatitude_points
-    worst_state = 0.0
+    worst_state = worst_rel = 0.0
     for step in range(45):  # MI updates at t = 0 and t = 10 s
         g.advance(1)
         o.advance(1)
         for fid in problem.STATE_FIELDS:
             for f in range(gcs_config.FACE_COUNT):
                 worst_state = max(worst_state, scaled_error(o.field(f, fid), g.field(f, fid)))
+        worst_rel = max(worst_rel, max(state_relative_errors(o, g).values()))
+        assert worst_state <= STEP_TOL * (step + 1), (step, worst_state)
         if step in (0, 44):
             rep = g.mi_report()
             pot_o, fac_o = mi_arrays(orc, o, n)
@@ -51,9 +60,9 @@ def test_mi_run_matches_oracle(gpu, orc, name):
             pot_err = np.abs(pot_g - pot_o).max() / max(np.abs(pot_o).max(), 1e-6)
             print(f"  step {step + 1}: updates {rep['updates']} iterations {rep['iterations']} "
                   f"solve {rep['solve_ms']:.1f} ms; fac rel {fac_err:.2e} potential rel "
-                  f"{pot_err:.2e} (max {np.abs(pot_o).max():.3e} V); state {worst_state:.2e}",
-                  flush=True)
+                  f"{pot_err:.2e} (max {np.abs(pot_o).max():.3e} V); state scaled "
+                  f"{worst_state:.2e} relative {worst_rel:.2e}", flush=True)
             assert fac_err <= 1e-10
-            assert pot_err <= 1e-6
+            assert pot_err <= 1e-10
     assert g.mi_report()["updates"] == 2
-    assert worst_state <= 1e-9
+    assert worst_rel <= STEP_TOL * 45

@@ -5,7 +5,7 @@ stay within a stated tolerance of it and keep div B at FP64 roundoff."""
 import numpy as np
 import pytest
 
-from conftest import gcs_config, max_divergence, problem
+from conftest import gcs_config, max_divergence, oracle_from, problem
 
 pytestmark = pytest.mark.gpu
 
@@ -51,6 +51,35 @@ def test_mixed_step_close_to_fp64(gpu, name, steps):
     div = max_divergence(b, cfg)
     print(f"  max |div B| (mixed) {div:.2e}", flush=True)
     assert div < 1e-9
+
+
+@pytest.mark.timeout(900)
+def test_mixed_form_against_oracle_config0(gpu, orc):
+    """Config 0 (6 x 24^2 x 48) for its 200 steps in the mixed form against
+    the FP64 oracle started from the same arrays: relative L2 / Linf error
+    of rho, p, |v|, |B| and max |div B|, each with its own dt rule."""
+    cfg = gcs_config.workload("config0")
+    print("mixed vs oracle config0 x200", flush=True)
+    b = gpu.GpuProblem(cfg)
+    b.set_precision(True)
+    o = oracle_from(b, orc, cfg)
+    for chunk in range(4):
+        b.advance(50)
+        o.advance(50)
+        print(f"  step {50 * (chunk + 1)}", flush=True)
+    po, pb = primitives(o, cfg), primitives(b, cfg)
+    worst = {}
+    for key in ("rho", "p", "v", "b"):
+        l2 = np.linalg.norm(pb[key] - po[key]) / np.linalg.norm(po[key])
+        linf = np.abs(pb[key] - po[key]).max() / np.abs(po[key]).max()
+        worst[key] = (l2, linf)
+        print(f"  {key}: rel L2 {l2:.2e} rel Linf {linf:.2e}", flush=True)
+        assert l2 <= MIXED_TOL
+        assert linf <= 10 * MIXED_TOL
+    div_b, div_o = max_divergence(b, cfg), max_divergence(o, cfg)
+    print(f"  max |div B| mixed {div_b:.2e}, oracle {div_o:.2e}", flush=True)
+    assert div_b < 1e-9
+    assert all(np.all(pb[k] > 0) for k in ("rho", "p"))
 
 
 def test_mixed_downloads_widen_fp32_arrays(gpu):

@@ -6,8 +6,8 @@ import time
 import numpy as np
 import pytest
 
-from conftest import (bits_equal, gcs_config, max_divergence, oracle_from,
-                      problem, scaled_error)
+from conftest import (bits_equal, counters, gcs_config, max_divergence,
+                      oracle_from, problem, scaled_error)
 
 pytestmark = pytest.mark.gpu
 
@@ -61,6 +61,34 @@ def test_steps_match_oracle(gpu, orc, name, steps):
         assert abs(cg["dt"] - co["dt"]) <= 1e-14 * co["dt"]
         err = worst_error(o, g, checked)
         assert err <= STEP_TOL * (s + 1), (s, err)
+    assert max_divergence(g, cfg) < 1e-10
+
+
+def test_emergency_diffusion_steps_match_oracle(gpu, orc):
+    """A 16 code-speed wind against the Boris light speed 10: interface
+    speeds reach 1.5 c and emergency diffusion (nonorthogonal_flux.c:206-241)
+    fires on most faces. State, predicted state, EMF and the per-face
+    counters (nonorthogonal_sweep.c:541-545) against the oracle every step."""
+    cfg = gcs_config.workload("fast_wind")
+    print("step parity fast_wind x6 (emergency diffusion)", flush=True)
+    g = gpu.GpuProblem(cfg)
+    o = oracle_from(g, orc, cfg)
+    checked = STATE + [problem.EDGE_EMF + d for d in range(3)] + [
+        problem.PREDICTED_CONSERVED, problem.PREDICTED_CELL_MAGNETIC]
+    for s in range(6):
+        g.advance(1)
+        o.advance(1)
+        assert abs(g.clock()["dt"] - o.clock()["dt"]) <= 1e-14 * o.clock()["dt"]
+        err = worst_error(o, g, checked)
+        assert err <= STEP_TOL * (s + 1), (s, err)
+        for f in range(gcs_config.FACE_COUNT):
+            (cg, vg), (co, vo) = counters(g, f), counters(o, f)
+            assert cg == co, (s, f, cg, co)
+            assert abs(vg - vo) <= 1e-13 * vo, (s, f, vg, vo)
+    fired = sum(counters(g, f)[0] for f in range(gcs_config.FACE_COUNT))
+    print(f"fast_wind: emergency diffusion on {fired:.0f} faces, scaled error {err:.2e}",
+          flush=True)
+    assert fired > 0
     assert max_divergence(g, cfg) < 1e-10
 
 

@@ -76,6 +76,29 @@ def test_kinetic_fluxes_and_hogs_match_reference(fo):
         assert bits_equal([st, *fl.conserved, *mx.momentum.value], hw)
 
 
+def test_emergency_diffusion_matches_reference(fo):
+    """orc_emergency_diffusion against nonorthogonal_flux.c:206-241 on
+    interfaces around the 1.5 c threshold (one exactly at it), Boris off,
+    a rejected interface and a non-finite cell value."""
+    applied_cases = 0
+    for row, want in zip(GOLDEN["emerg_in"], GOLDEN["emerg_out"]):
+        c, boris = row[0], bool(row[1])
+        pr = (R.Primitive * 2)(R.Primitive(row[2], vec(row[3:6]), row[6]),
+                               R.Primitive(row[7], vec(row[8:11]), row[11]))
+        fl = R.FluidFlux()
+        fl.conserved[:] = list(row[22:27])
+        applied = ctypes.c_bool(True)
+        st = fo["emergency"](ctypes.byref(fl), pr, arr(row[12:17]), arr(row[17:22]),
+                             boris, c, ctypes.byref(applied))
+        assert bits_equal([st, float(applied.value), *fl.conserved], want)
+        applied_cases += int(applied.value)
+        if st == 0 and applied.value:
+            # the applied branch subtracts c (upper - lower), nothing else
+            expect = row[22:27] - c * (row[17:22] - row[12:17])
+            assert bits_equal(list(fl.conserved), expect)
+    assert applied_cases >= 16
+
+
 def test_boris_matches_reference(fo):
     for row, want in zip(GOLDEN["boris_in"], GOLDEN["boris_out"]):
         u = arr(row[0:5])

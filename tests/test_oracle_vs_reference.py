@@ -3,8 +3,8 @@
 import numpy as np
 import pytest
 
-from conftest import (bits_equal, gcs_config, max_divergence, oracle_from,
-                      problem)
+from conftest import (bits_equal, counters, gcs_config, max_divergence,
+                      oracle_from, problem, reference_from)
 
 CHECKED = (problem.STATE_FIELDS
            + [problem.PREDICTED_CONSERVED, problem.PREDICTED_CELL_MAGNETIC]
@@ -14,7 +14,8 @@ CHECKED = (problem.STATE_FIELDS
            + [problem.MAXWELL_FACE_FLUX + d for d in range(3)])
 
 
-@pytest.mark.parametrize("name,steps", [("tiny", 4), ("small", 12), ("ragged", 6)])
+@pytest.mark.parametrize("name,steps", [("tiny", 4), ("small", 12), ("ragged", 6),
+                                        ("fast_wind", 5)])
 def test_oracle_steps_bitwise_equal_reference(ref, orc, name, steps):
     cfg = gcs_config.workload(name)
     r = problem.Problem(ref, "refh", cfg)
@@ -26,6 +27,15 @@ def test_oracle_steps_bitwise_equal_reference(ref, orc, name, steps):
         for fid in CHECKED:
             for f in range(gcs_config.FACE_COUNT):
                 assert bits_equal(r.field(f, fid), o.field(f, fid)), (fid, f)
+        # emergency-diffusion counters (nonorthogonal_sweep.c:541-545)
+        for f in range(gcs_config.FACE_COUNT):
+            assert bits_equal(counters(r, f), counters(o, f)), f
+    fired = sum(counters(o, f)[0] for f in range(gcs_config.FACE_COUNT))
+    if name == "fast_wind":
+        print(f"fast_wind: emergency diffusion on {fired:.0f} faces in {steps} steps")
+        assert fired > 0
+    else:
+        assert fired == 0
     # the reference's own gate: positive state, div B at roundoff
     assert max_divergence(o, cfg) < 1e-9
     for f in range(gcs_config.FACE_COUNT):
@@ -43,3 +53,38 @@ def test_oracle_explicit_step_matches_reference(ref, orc):
         for fid in problem.STATE_FIELDS:
             for f in range(gcs_config.FACE_COUNT):
                 assert bits_equal(r.field(f, fid), o.field(f, fid)), (fid, f)
+
+
+def test_reference_shell_steps_like_reference(ref):
+    """refh_create_shell + copied arrays (how the CPU baseline times the
+    reference at sizes whose reference setup takes minutes) steps bit for
+    bit like the reference problem it was copied from, from mid-run."""
+    cfg = gcs_config.workload("small")
+    r = problem.Problem(ref, "refh", cfg)
+    r.advance(2)
+    s = reference_from(r, ref, cfg)
+    for _ in range(3):
+        r.advance(1)
+        s.advance(1)
+        assert r.clock() == s.clock()
+        for fid in problem.STATE_FIELDS:
+            for f in range(gcs_config.FACE_COUNT):
+                assert bits_equal(r.field(f, fid), s.field(f, fid)), (fid, f)
+
+
+def test_oracle_config0_200_steps_bitwise_equal_reference(ref, orc):
+    """Config 0 (6 x 24^2 x 48) for its 200 steps: the oracle is the
+    compiled reference bit for bit over the whole run, through the halo
+    positivity fallback the run first takes at step 141."""
+    cfg = gcs_config.workload("config0")
+    r = problem.Problem(ref, "refh", cfg)
+    o = oracle_from(r, orc, cfg)
+    for chunk in range(4):
+        for _ in range(50):
+            r.advance(1)
+            o.advance(1)
+            assert r.clock() == o.clock()
+        for fid in CHECKED:
+            for f in range(gcs_config.FACE_COUNT):
+                assert bits_equal(r.field(f, fid), o.field(f, fid)), (chunk, fid, f)
+    assert max_divergence(o, cfg) < 1e-9
