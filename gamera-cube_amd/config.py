@@ -225,6 +225,10 @@ def workload(name):
         return earth_problem(16, 12, perturbation=0.01, seed=7)
     if name == "ragged":       # odd extents: every tile and block range partial
         return earth_problem(11, 13, perturbation=0.01, seed=3)
+    if name == "ghost5":       # five ghost layers (the reference allows ghost >= 4)
+        c = earth_problem(16, 12, perturbation=0.01, seed=7)
+        c.ghost = 5
+        return c
     if name == "fast_wind":    # 1600 km/s wind = 16 code speed >= 1.5 x the Boris
         # light speed 10: emergency interface diffusion fires on most faces
         return earth_problem(16, 12, perturbation=0.01, seed=7, speed_km_s=1600.0)

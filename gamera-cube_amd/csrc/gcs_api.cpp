@@ -3,7 +3,9 @@
 // include/gamera_cs_step.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -11,6 +13,7 @@
 #include <vector>
 
 #include "gcs_device.hpp"
+#include "gcs_transfer.hpp"
 
 namespace gcs {
 
@@ -22,30 +25,60 @@ const char *last_error() { return g_error.c_str(); }
 namespace {
 
 int check_config(const gcs_problem_config &c) {
-  if (c.ghost != 4) return set_error("ghost must be 4"), -1;
-  if (c.radial_cells < 2 || c.angular_cells < 9)
-    return set_error("grid needs nr >= 2 and n >= 9"), -1;
+  if (c.ghost < 4 || c.ghost > 16) return set_error("ghost must be 4 to 16"), -1;
+  if (c.radial_cells < 2 || c.angular_cells <= 2 * c.ghost)
+    return set_error("grid needs nr >= 2 and n > 2 ghost (cubedsphere_mesh.c:188-193)"), -1;
   if (c.wind_count < 1 || c.wind_count > GCS_MAX_WIND_SAMPLES)
     return set_error("wind_count out of range"), -1;
   if (!(c.startup.gamma > 1.0) || !(c.dt_fraction > 0.0) || !(c.cfl > 0.0))
     return set_error("invalid gamma, dt fraction or CFL"), -1;
-  const long cells = (long)(c.radial_cells + 9) * (c.angular_cells + 9) *
-                     ((c.angular_cells + 9 + 7) / 8 * 8) * 6;
+  const long g2 = 2 * (long)c.ghost + 1;
+  const long cells = ((long)c.radial_cells + g2) * ((long)c.angular_cells + g2) *
+                     (((long)c.angular_cells + g2 + 7) / 8 * 8) * 6;
   if (cells >= (1L << 31)) return set_error("grid too large for 32-bit indices"), -1;
   return 0;
 }
 
+}  // namespace
+
+int upload_wind(Problem &P, int count, const double *time, const double *state) {
+  if (count < 1) return set_error("empty wind series"), -1;
+  if ((size_t)count > P.wind_capacity) {
+    (void)hipFree(P.wind_buffer);
+    P.wind_buffer = nullptr;
+    P.wind_capacity = 0;
+    const size_t cap = std::max<size_t>((size_t)count, 16);
+    if (hipMalloc(&P.wind_buffer, cap * 9 * sizeof(double)) != hipSuccess) {
+      P.wind_buffer = nullptr;
+      return set_error("wind series: device allocation failed"), -1;
+    }
+    P.wind_capacity = cap;
+  }
+  // the step works on its own stream; finish what may still read the buffer
+  GCS_HIP(hipStreamSynchronize(P.stream));
+  GCS_HIP(hipMemcpy(P.wind_buffer, time, (size_t)count * sizeof(double),
+                    hipMemcpyHostToDevice));
+  GCS_HIP(hipMemcpy(P.wind_buffer + P.wind_capacity, state,
+                    (size_t)count * 8 * sizeof(double), hipMemcpyHostToDevice));
+  P.wind_series.count = count;
+  P.wind_series.time = P.wind_buffer;
+  P.wind_series.state = P.wind_buffer + P.wind_capacity;
+  return 0;
+}
+
+namespace {
+
 // Wind series of the config (solar_wind.c:142-171 with the startup state).
-void make_wind(Problem &P) {
+int make_wind(Problem &P) {
   const gcs_problem_config &c = P.cfg;
   const gamera_cs_earth_startup &s = c.startup;
   WindSeries &w = P.wind_series;
   std::memset(&w, 0, sizeof(w));
-  w.count = (int)c.wind_count;
   w.linear = c.wind_count > 1 ? c.wind_linear : 1;
+  std::vector<double> time(c.wind_count), state(8 * c.wind_count);
   for (size_t n = 0; n < c.wind_count; ++n) {
-    w.time[n] = c.wind_time[n];
-    double *st = w.state[n];
+    time[n] = c.wind_time[n];
+    double *st = &state[8 * n];
     st[0] = s.density;
     st[1] = s.density * s.speed * s.speed / (s.gamma * s.mach * s.mach);
     st[2] = s.speed;
@@ -54,6 +87,7 @@ void make_wind(Problem &P) {
     if (c.wind_count > 1) st[7] = c.wind_bz[n];
   }
   w.reference[0] = c.wind_reference_x;
+  return upload_wind(P, (int)c.wind_count, time.data(), state.data());
 }
 
 }  // namespace
@@ -76,6 +110,7 @@ void free_problem(Problem *P) {
   (void)hipFree(P->edge_members);
   (void)hipFree(P->edge_groups);
   (void)hipFree(P->seam_cells);
+  (void)hipFree(P->wind_buffer);
   if (P->ev_start) (void)hipEventDestroy(P->ev_start);
   if (P->ev_stop) (void)hipEventDestroy(P->ev_stop);
   if (P->stream) (void)hipStreamDestroy(P->stream);
@@ -199,13 +234,17 @@ int download_field(Problem &P, int face, int field, std::vector<double> &out) {
       default: return planes(P_MAXWELL + 3 * d, 3, fe);
     }
   }
-  if (field == GCS_FIELD_HYDRO_RATE) {  // fused into the update, not stored
-    out.assign(ncell * 5, 0.0);
-    return 0;
-  }
-  if (field == GCS_FIELD_MAXWELL_RATE) {
-    out.assign(ncell * 3, 0.0);
-    return 0;
+  if (field == GCS_FIELD_HYDRO_RATE || field == GCS_FIELD_MAXWELL_RATE) {
+    // active cells: what the last fidelity step's update stored in the
+    // face-velocity planes; ghost cells (and steps without fidelity): zero
+    const int nc = field == GCS_FIELD_HYDRO_RATE ? 5 : 3;
+    out.assign(ncell * nc, 0.0);
+    if (!P.rates_stored) return 0;
+    const size_t lo[3] = {(size_t)L.ghost, (size_t)L.ghost, (size_t)L.ghost};
+    const size_t hi[3] = {(size_t)(L.ghost + L.nr), (size_t)(L.ghost + L.n),
+                          (size_t)(L.ghost + L.n)};
+    return box_planes_to_host(P, nc == 5 ? P_FVEL : P_FVEL + 5, nc, face, ce, lo, hi,
+                              out.data());
   }
   if (field == GCS_FIELD_VERTEX) return planes(P_VERT, 3, ve);
   if (field == GCS_FIELD_CELL_GEOMETRY) {
@@ -374,7 +413,10 @@ Problem *gcs_problem_create(const gcs_problem_config *cfg) {
     gcs::free_problem(P);
     return nullptr;
   }
-  gcs::make_wind(*P);
+  if (gcs::make_wind(*P) != 0) {
+    gcs::free_problem(P);
+    return nullptr;
+  }
   gcs::driver_options(*P);
   P->driver_cfl = true;
   if (gcs::setup_problem(*P) != 0 || gcs::build_plans(*P) != 0 ||
@@ -488,6 +530,44 @@ int gcs_problem_pressure_control(Problem *P, double dt, double gamma, double den
   gcs::StepScalars s;
   if (gcs::read_scalars(*P, s) != 0) return -1;
   return s.failed ? gcs::check_failure(*P) : 0;
+}
+
+int gcs_problem_final_gates(Problem *P, double out[3]) {
+  if (P == nullptr || out == nullptr) return -1;
+  if (gcs::final_gates(*P, P->cfg.startup.gamma, out) != 0) return -1;
+  gcs::StepScalars s;
+  if (gcs::read_scalars(*P, s) != 0) return -1;
+  return s.failed ? gcs::check_failure(*P) : 0;
+}
+
+int gcs_problem_set_wind(Problem *P, size_t count, const double *time, const double *state,
+                         int linear) {
+  if (P == nullptr || count < 1 || count > (size_t)INT32_MAX || time == nullptr ||
+      state == nullptr) {
+    gcs::set_error("invalid wind series arguments");
+    return -1;
+  }
+  // solar_wind.c:55-68: finite samples, strictly increasing times
+  for (size_t n = 0; n < count; ++n) {
+    if (!std::isfinite(time[n]) || (n > 0 && !(time[n] > time[n - 1]))) {
+      gcs::set_error("wind series times must be finite and increasing");
+      return -1;
+    }
+    for (int q = 0; q < 8; ++q)
+      if (!std::isfinite(state[8 * n + q])) {
+        gcs::set_error("wind series states must be finite");
+        return -1;
+      }
+  }
+  if (gcs::upload_wind(*P, (int)count, time, state) != 0) return -1;
+  P->wind_series.linear = count > 1 ? (linear != 0) : 1;
+  // captured steps hold the series' scalars: capture again
+  for (auto &g : P->graphs)
+    if (g.exec != nullptr) {
+      (void)hipGraphExecDestroy(g.exec);
+      g.exec = nullptr;
+    }
+  return 0;
 }
 
 int gcs_problem_set_plain_kernels(Problem *P, int on) {

@@ -41,10 +41,12 @@ struct StepScalars {
 };
 
 // Solar-wind series for device sampling (solar_wind.h:24-35).
+// A solar-wind time series of any length (gamera_solar_wind_series,
+// solar_wind.h): the samples live in a device buffer of the problem.
 struct WindSeries {
   int count, linear;
-  double time[GCS_MAX_WIND_SAMPLES];
-  double state[GCS_MAX_WIND_SAMPLES][8];  // rho, p, v[3], b[3]
+  const double *time;   // device, count sample times
+  const double *state;  // device, count x 8: rho, p, v[3], b[3]
   double reference[3];
   double time_offset, by, bz;
 };
@@ -94,6 +96,12 @@ struct StepOptions {
   // Earth wall (startup gamma, context floors and cfl)
   double wall_gamma, wall_rho_floor, wall_p_floor, wall_cfl;
   int strict_wall;
+  // 1: MHD step; 0: hydrodynamic only (use_mhd = false: no Maxwell flux,
+  // EMF, CT, B recovery or Boris; B and face flux carried unchanged)
+  int mhd = 1;
+  // 1: the update also stores the stress rates (hydro 5, Maxwell 3) in the
+  // face-velocity planes, for downloads of hydro_rate / maxwell_rate
+  int rates = 0;
 };
 
 
@@ -174,11 +182,16 @@ struct Problem {
   int *seam_cells = nullptr;       // active cells with a seam face
   int n_seam_cells = 0;
   WindSeries wind_series;
+  double *wind_buffer = nullptr;   // device: times, then 8 values per sample
+  size_t wind_capacity = 0;        // samples the buffer holds
   bool fidelity = true;            // reproduce ghost values of the storage
   bool mixed = false;              // FP32 reconstruction, fluxes and EMF
   // One face or edge per thread in every face-flux, face-velocity and EMF
   // sweep instead of the LDS tiles and register marches (same results).
   bool plain = false;
+  // The last step stored the stress rates of the active cells in the
+  // face-velocity planes (fidelity steps).
+  bool rates_stored = false;
   // The predictor caches and face fluxes of the last step are in planes32.
   bool last_step_mixed = false;
   // Cell B of both levels equals the recovery from their face fluxes (true
@@ -271,6 +284,13 @@ int enqueue_step(Problem &P, bool explicit_dt);
 // last phase swaps the time levels.
 enum Phase { PHASE_GHOSTS, PHASE_FLUX, PHASE_EMF, PHASE_UPDATE };
 int max_div_b(Problem &P, double *out);
+// {min density, min pressure, max total div B} of the current state's
+// active cells (the Earth driver's end-of-run gate).
+int final_gates(Problem &P, double gamma, double out[3]);
+// Upload a wind series (count samples: time[n], state[8 n .. 8 n + 7] =
+// rho, p, v, B) into the problem's device buffer, growing it when needed,
+// and point P.wind_series at it; the other fields are set by the caller.
+int upload_wind(Problem &P, int count, const double *time, const double *state);
 int enqueue_phase(Problem &P, Phase phase);
 // Constants derived from uploaded geometry and background: recovery
 // pivots, edge-B solve, B0 normal and edge-tangential components.

@@ -70,6 +70,7 @@ struct Mirror {
   bool has_background = false;
   Problem *P = nullptr;
   bool device_current = false;  // resident mode: device state is newest
+  std::vector<double> wind;     // the wind samples last uploaded
 };
 
 std::mutex g_lock;
@@ -312,6 +313,15 @@ int state_to_host(Problem &P, gamera_no_storage *st) {
                          s.maxwell_face_flux[d][0].value) != 0)
         return -1;
     }
+    // stress rates of the active cells (nonorthogonal_sweep.c:550-622), which
+    // the update stores in the face-velocity planes
+    const size_t lo[3] = {(size_t)L.ghost, (size_t)L.ghost, (size_t)L.ghost};
+    const size_t hi[3] = {(size_t)(L.ghost + L.nr), (size_t)(L.ghost + L.n),
+                          (size_t)(L.ghost + L.n)};
+    if (P.rates_stored &&
+        (box_planes_to_host(P, P_FVEL, 5, f, ce, lo, hi, s.hydro_rate) != 0 ||
+         box_planes_to_host(P, P_FVEL + 5, 3, f, ce, lo, hi, s.maxwell_rate[0].value) != 0))
+      return -1;
   }
   return 0;
 }
@@ -325,16 +335,14 @@ void release(Mirror *m) {
 Mirror *build_mirror(const gamera_no_grid *grid, gamera_no_storage *storage,
                      const size_t active[3], size_t ghost,
                      const gamera_no_background_field *background) {
-  if (ghost != 4) {
-    set_error("the device step is built for ghost = 4");
+  if (ghost < 4 || active[0] < 2 || active[1] <= 2 * ghost) {
+    set_error("grid needs ghost >= 4, at least 2 radial and more than 2 ghost angular "
+              "active cells (cubedsphere_mesh.c:188-193)");
     return nullptr;
   }
-  if (active[0] < 2 || active[1] < 9) {
-    set_error("grid needs at least 2 radial and 9 angular active cells");
-    return nullptr;
-  }
-  const long cells = (long)(active[0] + 9) * (active[1] + 9) *
-                     ((active[1] + 9 + 7) / 8 * 8) * 6;
+  const long g2 = 2 * (long)ghost + 1;
+  const long cells = ((long)active[0] + g2) * ((long)active[1] + g2) *
+                     (((long)active[1] + g2 + 7) / 8 * 8) * 6;
   if (cells >= (1L << 31)) {
     set_error("grid too large for 32-bit indices");
     return nullptr;
@@ -433,6 +441,7 @@ void set_options(Problem &P, const gamera_no_advance_options &o) {
   s.upd_p_floor = o.update.pressure_floor;
   s.upd_c = o.update.light_speed;
   s.upd_boris = o.update.use_boris;
+  s.mhd = o.update.use_mhd ? 1 : 0;
 }
 
 // The Earth context can run on the device when it matches what the mirror
@@ -447,30 +456,38 @@ bool earth_on_device(const Mirror &m, const gamera_cs_earth_boundary *b,
     return false;
   if (need_no_mi && b->mi != nullptr) return false;
   const gamera_solar_wind_series &w = *b->wind;
-  return w.count >= 1 && w.count <= GCS_MAX_WIND_SAMPLES && w.time != nullptr &&
+  return w.count >= 1 && w.count <= (size_t)INT32_MAX && w.time != nullptr &&
          w.state != nullptr;
 }
 
-void set_wind(Problem &P, const gamera_cs_earth_boundary &b) {
-  WindSeries &w = P.wind_series;
-  std::memset(&w, 0, sizeof(w));
+// The Earth context's wind series (any length) on the device; its samples
+// are uploaded again only when they differ from the last upload.
+int set_wind(Problem &P, std::vector<double> &last, const gamera_cs_earth_boundary &b) {
   const gamera_solar_wind_series &s = *b.wind;
-  w.count = (int)s.count;
-  w.linear = s.linear_interpolation;
+  std::vector<double> samples(9 * s.count);
   for (size_t n = 0; n < s.count; ++n) {
-    w.time[n] = s.time[n];
+    samples[n] = s.time[n];
     const gamera_solar_wind_state &x = s.state[n];
-    w.state[n][0] = x.density;
-    w.state[n][1] = x.pressure;
+    double *st = &samples[s.count + 8 * n];
+    st[0] = x.density;
+    st[1] = x.pressure;
     for (int d = 0; d < 3; ++d) {
-      w.state[n][2 + d] = x.velocity[d];
-      w.state[n][5 + d] = x.magnetic[d];
+      st[2 + d] = x.velocity[d];
+      st[5 + d] = x.magnetic[d];
     }
   }
+  if (samples != last) {
+    if (upload_wind(P, (int)s.count, samples.data(), samples.data() + s.count) != 0)
+      return -1;
+    last.swap(samples);
+  }
+  WindSeries &w = P.wind_series;
+  w.linear = s.linear_interpolation;
   for (int d = 0; d < 3; ++d) w.reference[d] = s.reference[d];
   w.time_offset = s.time_offset;
   w.by = s.by_coefficient;
   w.bz = s.bz_coefficient;
+  return 0;
 }
 
 int report_failure(Problem &P, const StepScalars &s) {
@@ -499,10 +516,6 @@ int advance(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
   if (!valid_call(grid, storage, active, ghost, ratio, dt, o, background, radial,
                   current_time, previous_time)) {
     set_error("invalid arguments (cubedsphere_step.c:23-47)");
-    return -1;
-  }
-  if (!o->update.use_mhd) {
-    set_error("hydrodynamic-only steps (use_mhd = false) are not built on the device");
     return -1;
   }
   std::lock_guard<std::mutex> guard(g_lock);
@@ -555,7 +568,7 @@ int advance(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
     s.wall_p_floor = wall->pressure_floor;
     s.wall_cfl = wall->cfl;
     s.strict_wall = wall->strict_inner_gas_wall != 0;
-    set_wind(P, *wall);
+    if (set_wind(P, m->wind, *wall) != 0) return -1;
   }
 
   // Scalars of this call.

@@ -32,6 +32,8 @@
 #include <cstring>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <cfloat>
 #include <cmath>
 #include <type_traits>
@@ -440,11 +442,11 @@ GCS_DEV bool wind_ok(const double s[8]) {
 GCS_DEV bool wind_at(const WindSeries &w, double t, double out[8]) {
   if (!isfinite(t)) return false;
   if (w.count == 1 || t <= w.time[0]) {
-    for (int q = 0; q < 8; ++q) out[q] = w.state[0][q];
+    for (int q = 0; q < 8; ++q) out[q] = w.state[q];
     return true;
   }
   if (t >= w.time[w.count - 1]) {
-    for (int q = 0; q < 8; ++q) out[q] = w.state[w.count - 1][q];
+    for (int q = 0; q < 8; ++q) out[q] = w.state[8L * (w.count - 1) + q];
     return true;
   }
   int lo = 0, hi = w.count - 1;
@@ -453,13 +455,13 @@ GCS_DEV bool wind_at(const WindSeries &w, double t, double out[8]) {
     if (w.time[mid] <= t) lo = mid;
     else hi = mid;
   }
+  const double *a = w.state + 8L * lo, *b = a + 8;
   if (!w.linear) {
-    for (int q = 0; q < 8; ++q) out[q] = w.state[lo][q];
+    for (int q = 0; q < 8; ++q) out[q] = a[q];
     return true;
   }
   const double fr = (t - w.time[lo]) / (w.time[lo + 1] - w.time[lo]);
-  for (int q = 0; q < 8; ++q)
-    out[q] = w.state[lo][q] + fr * (w.state[lo + 1][q] - w.state[lo][q]);
+  for (int q = 0; q < 8; ++q) out[q] = a[q] + fr * (b[q] - a[q]);
   return wind_ok(out);
 }
 
@@ -822,7 +824,9 @@ __global__ void k_predict(K c) {
     const double cf = c.p(plane_phi(c.cur, d))[at];
     const double of = c.p(plane_phi(c.old, d))[at];
     const double pred = cf + r * (cf - of);
-    c.p(P_PHIPRED + d)[at] = pred;
+    // The predicted flux itself is storage output only (the step reads its
+    // B.n below, and the outer wall recomputes it): written with fidelity.
+    if (c.o.rates) c.p(P_PHIPRED + d)[at] = pred;
     // Face-normal B of the predicted flux, read by the face fluxes and the
     // edge EMF (both divide this flux by this area).
     const double bn = pred / c.p(P_FACE + FACE_PLANES * d + FO_AREA)[at];
@@ -849,10 +853,12 @@ __global__ void k_predict(K c) {
     c.fail(31, at);
     return;
   }
+  // U*: the FP64 step reads only its density (the EMF's edge density); the
+  // rest is storage output, written with fidelity.
   for (int q = 0; q < 5; ++q) {
     if (Mixed)
       c.sp(M_UPRED + q)[at] = (float)up[q];
-    else
+    else if (q == 0 || c.o.rates)
       c.p(P_UPRED + q)[at] = up[q];
   }
   // The sweep's primitive conversion of this cell (nonorthogonal_state.c:60-93
@@ -943,11 +949,13 @@ GCS_DEV void face_flux_face(const K &c, int d, int f, long at, long st, const Ce
   }
   for (int n = 0; n < 8; ++n) q8[n] = cell(4, n);
   up7_pdm_shared(vol, iv, riv, q8, o.pdm, w[0].p, w[1].p);
-  double bb[2][3];
-  for (int e = 0; e < 3; ++e) {
-    double q8[8];
-    for (int n = 0; n < 8; ++n) q8[n] = cell(6 + e, n);
-    up7_pdm_shared(vol, iv, riv, q8, o.pdm, bb[0][e], bb[1][e]);
+  double bb[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  if (o.mhd) {
+    for (int e = 0; e < 3; ++e) {
+      double q8[8];
+      for (int n = 0; n < 8; ++n) q8[n] = cell(6 + e, n);
+      up7_pdm_shared(vol, iv, riv, q8, o.pdm, bb[0][e], bb[1][e]);
+    }
   }
   if (!valid_interface(w[0].rho, w[0].v, w[0].p) ||
       !valid_interface(w[1].rho, w[1].v, w[1].p)) {
@@ -964,14 +972,16 @@ GCS_DEV void face_flux_face(const K &c, int d, int f, long at, long st, const Ce
   }
   FluidFlux fl;
   fluid_flux(w, o.gamma, fr, fl);
-  const double bn = c.p(P_BN + d)[at];
-  const double b0[3] = {c.p(P_B0F + 3 * d)[at], c.p(P_B0F + 3 * d + 1)[at],
-                        c.p(P_B0F + 3 * d + 2)[at]};
-  MaxwellFlux mx;
-  if (!maxwell_flux(w, bb, bn, fr.n, b0, c.p(P_B0N + d)[at], o.boris, o.c,
-                    fl.un, mx)) {
-    c.fail(44, at);
-    return;
+  // hydrodynamic steps: a zero Maxwell flux (nonorthogonal_sweep.c:463-464)
+  MaxwellFlux mx = {{0.0, 0.0, 0.0}, 0.0, 0.0};
+  if (o.mhd) {
+    const double bn = c.p(P_BN + d)[at];
+    const double b0[3] = {c.p(P_B0F + 3 * d)[at], c.p(P_B0F + 3 * d + 1)[at],
+                          c.p(P_B0F + 3 * d + 2)[at]};
+    if (!maxwell_flux(w, bb, bn, fr.n, b0, c.p(P_B0N + d)[at], o.boris, o.c, fl.un, mx)) {
+      c.fail(44, at);
+      return;
+    }
   }
   if (o.hogs) hogs(fl, mx, o.hogs_h, o.hogs_m, o.boris, o.c);
   if (o.hogs && o.boris) {
@@ -980,9 +990,24 @@ GCS_DEV void face_flux_face(const K &c, int d, int f, long at, long st, const Ce
     s2 += dot3(w[0].v, w[0].v);
     s2 += dot3(w[1].v, w[1].v);
     if (sqrt(0.5 * s2) >= 1.5 * o.c) {
-      const long lc = at - st;
+      // U* of the two cells, by the predictor's own expression (the
+      // predicted state is not kept in memory for this rare branch)
+      double us[2][5];
+      for (int side = 0; side < 2; ++side) {
+        const long cell = at - (1 - side) * st;
+        double uo[5], uc[5];
+        for (int q = 0; q < 5; ++q) {
+          uo[q] = c.p(plane_u(c.old, q))[cell];
+          uc[q] = c.p(plane_u(c.cur, q))[cell];
+        }
+        if (!predict_cell(uo, uc, c.sc->ratio, o.upd_gamma, o.upd_rho_floor, o.upd_p_floor,
+                          us[side])) {
+          c.fail(45, at);
+          return;
+        }
+      }
       for (int q = 0; q < 5; ++q) {
-        const double lo_u = c.p(P_UPRED + q)[lc], up_u = c.p(P_UPRED + q)[at];
+        const double lo_u = us[0][q], up_u = us[1][q];
         if (!isfinite(lo_u) || !isfinite(up_u)) {
           c.fail(45, at);
           return;
@@ -1540,11 +1565,13 @@ GCS_DEV void face_flux_face_fp32(const K &c, int d, int f, long at, long st, con
   }
   for (int n = 0; n < 8; ++n) q8[n] = cell(4, n);
   up7_pdm_shared_r(vol, iv, riv, q8, pdm, w[0].p, w[1].p);
-  R bb[2][3];
-  for (int e = 0; e < 3; ++e) {
-    R q8[8];
-    for (int n = 0; n < 8; ++n) q8[n] = cell(6 + e, n);
-    up7_pdm_shared_r(vol, iv, riv, q8, pdm, bb[0][e], bb[1][e]);
+  R bb[2][3] = {{R(0), R(0), R(0)}, {R(0), R(0), R(0)}};
+  if (o.mhd) {
+    for (int e = 0; e < 3; ++e) {
+      R q8[8];
+      for (int n = 0; n < 8; ++n) q8[n] = cell(6 + e, n);
+      up7_pdm_shared_r(vol, iv, riv, q8, pdm, bb[0][e], bb[1][e]);
+    }
   }
   if (!valid_interface_r(w[0].rho, w[0].v, w[0].p) ||
       !valid_interface_r(w[1].rho, w[1].v, w[1].p)) {
@@ -1582,14 +1609,18 @@ GCS_DEV void face_flux_face_fp32(const K &c, int d, int f, long at, long st, con
   }
   FluidFluxR<R> fl;
   fluid_flux_r(w, gamma, fr, fl);
-  const R bn = c.sp(M_BN + d)[at];
-  const R b0[3] = {c.sp(fb + MF_B0)[at], c.sp(fb + MF_B0 + 1)[at], c.sp(fb + MF_B0 + 2)[at]};
   const R cl = (R)o.c;
   MaxwellFluxR<R> mx;
-  if (!maxwell_flux_r(w, bb, bn, fr.n, b0, c.sp(fb + MF_B0N)[at], o.boris != 0, cl, fl.un,
-                      mx)) {
-    c.fail(44, at);
-    return;
+  for (int e = 0; e < 3; ++e) mx.m[e] = R(0);
+  mx.va = mx.psum = R(0);
+  if (o.mhd) {
+    const R bn = c.sp(M_BN + d)[at];
+    const R b0[3] = {c.sp(fb + MF_B0)[at], c.sp(fb + MF_B0 + 1)[at], c.sp(fb + MF_B0 + 2)[at]};
+    if (!maxwell_flux_r(w, bb, bn, fr.n, b0, c.sp(fb + MF_B0N)[at], o.boris != 0, cl, fl.un,
+                        mx)) {
+      c.fail(44, at);
+      return;
+    }
   }
   if (o.hogs) hogs_r(fl, mx, (R)o.hogs_h, (R)o.hogs_m, o.boris != 0, cl);
   if (o.hogs && o.boris) {
@@ -1833,7 +1864,8 @@ __global__ void k_earth_emf_diffusive(K c, WindSeries w) {
   auto dens = [&](int dir, long fa, double &out) {
     const double area = c.p(P_FACE + FACE_PLANES * dir + FO_AREA)[fa];
     if (!(area > 0.0)) return false;
-    out = c.p(P_PHIPRED + dir)[fa] / area;
+    const double cf = c.p(plane_phi(c.cur, dir))[fa], of = c.p(plane_phi(c.old, dir))[fa];
+    out = (cf + c.sc->ratio * (cf - of)) / area;  // the predictor's Phi*
     return (bool)isfinite(out);
   };
   double bikp, bikm, bkip, bkim, bjip, bjim, bijp, bijm;
@@ -1917,7 +1949,7 @@ GCS_DEV double update_cell(const K &c, int i, int j, int k, long at, const int h
   const double dt = c.sc->dt;
   const double vol = c.p(P_VOL)[at];
   const double ivol = 1.0 / vol;
-  double rate[5], mr[3];
+  double rate[5], mr[3] = {0.0, 0.0, 0.0};
   for (int q = 0; q < 5; ++q) {
     double diff = 0.0;
     for (int d = 0; d < 3; ++d) {
@@ -1926,19 +1958,38 @@ GCS_DEV double update_cell(const K &c, int i, int j, int k, long at, const int h
     }
     rate[q] = diff * ivol;
   }
-  for (int m = 0; m < 3; ++m) {
-    double diff = 0.0;
-    for (int d = 0; d < 3; ++d) {
-      const Flux *mp = maxwell_plane<Flux>(c, d, m);
-      diff += (double)mp[at] - (double)mp[at + step[d]];
+  if (o.mhd) {
+    for (int m = 0; m < 3; ++m) {
+      double diff = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const Flux *mp = maxwell_plane<Flux>(c, d, m);
+        diff += (double)mp[at] - (double)mp[at + step[d]];
+      }
+      mr[m] = diff * ivol;
+      mr[m] += c.p(P_FORCE + m)[at];
     }
-    mr[m] = diff * ivol;
-    mr[m] += c.p(P_FORCE + m)[at];
+  }
+  if (o.rates) {  // storage hydro_rate / maxwell_rate (nonorthogonal_sweep.c:585-619)
+    for (int q = 0; q < 5; ++q) c.p(P_FVEL + q)[at] = rate[q];
+    for (int m = 0; m < 3; ++m) c.p(P_FVEL + 5 + m)[at] = mr[m];
   }
   double u0[5], u[5];
   for (int q = 0; q < 5; ++q) {
     u0[q] = c.p(plane_u(c.cur, q))[at];
     u[q] = u0[q] + dt * rate[q];
+  }
+  if (!o.mhd) {
+    // nonorthogonal_step.c:554-556: a hydrodynamic step ends with the
+    // Reynolds update; cell B and the face fluxes carry over unchanged.
+    for (int q = 0; q < 5; ++q) c.p(plane_u(c.old, q))[at] = u[q];
+    for (int m = 0; m < 3; ++m) c.p(plane_b(c.old, m))[at] = c.p(plane_b(c.cur, m))[at];
+    for (int d = 0; d < 3; ++d) {
+      c.p(plane_phi(c.old, d))[at] = c.p(plane_phi(c.cur, d))[at];
+      const int pos = d == 0 ? i : d == 1 ? j : k;
+      if (pos == hi[d] - 1)
+        c.p(plane_phi(c.old, d))[at + step[d]] = c.p(plane_phi(c.cur, d))[at + step[d]];
+    }
+    return DBL_MAX;
   }
   // CT: new flux of the six faces (identical arithmetic in both cells of a
   // shared face).
@@ -2023,8 +2074,12 @@ __global__ void k_keep_ghosts(K c) {
   const bool ai = i >= g && i < g + L.nr, aj = j >= g && j < g + L.n,
              ak = k >= g && k < g + L.n;
   const bool cell = i < L.NI && j < L.NJ && k < L.NK;
-  if (cell && !(ai && aj && ak))
+  if (cell && !(ai && aj && ak)) {
     for (int q = 0; q < 5; ++q) c.p(plane_u(c.old, q))[at] = c.p(plane_u(c.cur, q))[at];
+    // a hydrodynamic step recovers no B: ghost B carries over as well
+    if (!c.o.mhd)
+      for (int m = 0; m < 3; ++m) c.p(plane_b(c.old, m))[at] = c.p(plane_b(c.cur, m))[at];
+  }
   const bool fi = i >= g && i <= g + L.nr, fj = j >= g && j <= g + L.n,
              fk = k >= g && k <= g + L.n;
   const bool face[3] = {i <= L.NI && j < L.NJ && k < L.NK && !(fi && aj && ak),
@@ -2050,6 +2105,7 @@ static K make_k(Problem &P) {
   c.cur = P.level;
   c.old = 1 - P.level;
   c.o = P.opt;
+  c.o.rates = P.fidelity ? 1 : 0;
   c.mi = P.mi_view;
   return c;
 }
@@ -2212,7 +2268,9 @@ int enqueue_phase(Problem &P, Phase phase) {
         k_inner_trap<double><<<nblocks(6L * L.n * L.n, T), T, 0, s>>>(c);
     }
   } else if (phase == PHASE_EMF) {
-    // cubedsphere_step.c:99-126 up to the seam EMF average
+    // cubedsphere_step.c:99-126 up to the seam EMF average; a hydrodynamic
+    // step has no EMF
+    if (!P.opt.mhd) return 0;
     mark_stage(P, S_FACE_VELOCITY);
     const int act[3] = {L.nr, L.n, L.n};
     // FP64: the i-face velocities for the k-edges only; the j- and i-edges
@@ -2266,9 +2324,11 @@ int enqueue_phase(Problem &P, Phase phase) {
     }
   } else {
     // cubedsphere_step.c:125-152
-    mark_stage(P, S_SYNC_EDGES);
-    k_sync_edges<<<nblocks(P.n_edge_groups, T), T, 0, s>>>(c, P.edge_groups, P.edge_members,
-                                                           P.n_edge_groups);
+    if (P.opt.mhd) {
+      mark_stage(P, S_SYNC_EDGES);
+      k_sync_edges<<<nblocks(P.n_edge_groups, T), T, 0, s>>>(c, P.edge_groups, P.edge_members,
+                                                             P.n_edge_groups);
+    }
     mark_stage(P, S_UPDATE);
     // A driver step folds the new state's CFL dt into the slots that the
     // next step's dt rule reads (its seam cells after the final sync).
@@ -2279,23 +2339,28 @@ int enqueue_phase(Problem &P, Phase phase) {
     else
       k_update<false><<<nblocks(n_active, 128), 128, 0, s>>>(c, cfl);
     // Final seam average of the new face flux, then B of the touched cells.
+    // A hydrodynamic step changes no face flux (cubedsphere_step.c:142).
     K n = c;
     n.cur = c.old;
     n.old = c.cur;
-    mark_stage(P, S_FINAL_SYNC);
-    k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(n, P.seams, P.n_seams, 0);
-    mark_stage(P, S_SEAM_RECOVER);
-    k_recover_list<<<nblocks(P.n_seam_cells, T), T, 0, s>>>(n, P.seam_cells, P.n_seam_cells,
-                                                            plane_phi(n.cur, 0),
-                                                            plane_b(n.cur, 0), cfl);
+    if (P.opt.mhd) {
+      mark_stage(P, S_FINAL_SYNC);
+      k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(n, P.seams, P.n_seams, 0);
+      mark_stage(P, S_SEAM_RECOVER);
+      k_recover_list<<<nblocks(P.n_seam_cells, T), T, 0, s>>>(n, P.seam_cells, P.n_seam_cells,
+                                                              plane_phi(n.cur, 0),
+                                                              plane_b(n.cur, 0), cfl);
+    }
     if (P.fidelity) {
       mark_stage(P, S_FIDELITY);
       k_keep_ghosts<<<nblocks(n_box, T), T, 0, s>>>(c);
-      k_recover_ghosts<<<nblocks(n_cells - n_active, T), T, 0, s>>>(n, plane_phi(n.cur, 0),
-                                                                    plane_b(n.cur, 0), 0);
+      if (P.opt.mhd)
+        k_recover_ghosts<<<nblocks(n_cells - n_active, T), T, 0, s>>>(n, plane_phi(n.cur, 0),
+                                                                      plane_b(n.cur, 0), 0);
     }
     P.level = 1 - P.level;
     P.host_dirty = true;
+    P.rates_stored = c.o.rates != 0;
     P.old_halo_current = P.b_recovered;
     P.b_recovered = true;
     P.cfl_folded = cfl != nullptr;
@@ -2399,67 +2464,86 @@ struct PressureControlArgs {
       low_density_sound_speed;
 };
 
-__global__ void k_pressure_control(K c, PressureControlArgs a, unsigned long long *acc) {
+// One active cell (thread t of the active box) of the pressure control:
+// updates its U in place and reports whether each branch ran, its sound
+// speed and its pressure (zeros for a thread outside the box).
+GCS_DEV void pressure_control_cell(const K &c, const PressureControlArgs &a, long t,
+                                   bool &low, bool &hot, double &sound, double &pressure) {
   const Layout &L = c.L;
   const int lo[3] = {L.ghost, L.ghost, L.ghost};
   const int hi[3] = {L.ghost + L.nr, L.ghost + L.n, L.ghost + L.n};
   int f, i, j, k;
-  bool low = false, hot = false;
-  double sound = 0.0, pressure = 0.0;
-  if (box(tid(), lo, hi, f, i, j, k)) {
-    const long at = L.at(f, i, j, k);
-    double u[5];
-    for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
-    Prim w;
-    if (!cons_to_prim(u, a.gamma, a.density_floor, a.pressure_floor, w)) {
-      c.fail(81, at);
+  low = hot = false;
+  sound = pressure = 0.0;
+  if (!box(t, lo, hi, f, i, j, k)) return;
+  const long at = L.at(f, i, j, k);
+  double u[5];
+  for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
+  Prim w;
+  if (!cons_to_prim(u, a.gamma, a.density_floor, a.pressure_floor, w)) {
+    c.fail(81, at);
+    return;
+  }
+  bool changed = false, ok = true;
+  if (w.rho <= a.low_density_threshold) {
+    w.p = fmax(a.pressure_floor,
+               w.rho * a.low_density_sound_speed * a.low_density_sound_speed / a.gamma);
+    low = changed = true;
+  }
+  const double cs = sqrt(a.gamma * w.p / w.rho);
+  sound = cs;
+  pressure = w.p;
+  if (cs > 1.5 * a.light_speed) {
+    const double x = c.p(P_CEN)[at], y = c.p(P_CEN + 1)[at], z = c.p(P_CEN + 2)[at];
+    const double radius_square = x * x + y * y + z * z;
+    const double cylindrical_square = x * x + y * y;
+    if (!(radius_square > 0.0) || !(cylindrical_square > DBL_MIN)) {
+      c.fail(82, at);
+      ok = false;
     } else {
-      bool changed = false, ok = true;
-      if (w.rho <= a.low_density_threshold) {
-        w.p = fmax(a.pressure_floor,
-                   w.rho * a.low_density_sound_speed * a.low_density_sound_speed / a.gamma);
-        low = changed = true;
-      }
-      const double cs = sqrt(a.gamma * w.p / w.rho);
-      sound = cs;
-      pressure = w.p;
-      if (cs > 1.5 * a.light_speed) {
-        const double x = c.p(P_CEN)[at], y = c.p(P_CEN + 1)[at], z = c.p(P_CEN + 2)[at];
-        const double radius_square = x * x + y * y + z * z;
-        const double cylindrical_square = x * x + y * y;
-        if (!(radius_square > 0.0) || !(cylindrical_square > DBL_MIN)) {
-          c.fail(82, at);
-          ok = false;
-        } else {
-          const double radius = sqrt(radius_square);
-          const double dipole_l = radius * radius_square / cylindrical_square;
-          const double target =
-              w.rho * (1.5 * a.light_speed) * (1.5 * a.light_speed) / a.gamma;
-          const double bounce_time = dipole_l / cs;
-          w.p -= (a.dt / bounce_time) * (w.p - target);
-          hot = changed = true;
-        }
-      }
-      if (ok && changed) {
-        if (!prim_to_cons(w, a.gamma, a.density_floor, a.pressure_floor, u))
-          c.fail(83, at);
-        else
-          for (int q = 0; q < 5; ++q) c.p(plane_u(c.cur, q))[at] = u[q];
-      }
+      const double radius = sqrt(radius_square);
+      const double dipole_l = radius * radius_square / cylindrical_square;
+      const double target = w.rho * (1.5 * a.light_speed) * (1.5 * a.light_speed) / a.gamma;
+      const double bounce_time = dipole_l / cs;
+      w.p -= (a.dt / bounce_time) * (w.p - target);
+      hot = changed = true;
     }
   }
-  const unsigned long long n_low = __popcll(__ballot(low)), n_hot = __popcll(__ballot(hot));
+  if (ok && changed) {
+    if (!prim_to_cons(w, a.gamma, a.density_floor, a.pressure_floor, u))
+      c.fail(83, at);
+    else
+      for (int q = 0; q < 5; ++q) c.p(plane_u(c.cur, q))[at] = u[q];
+  }
+}
+
+// A wave's (low-density count, high-sound-speed count, maximum sound speed
+// bits, maximum pressure bits), complete in lane 0.
+GCS_DEV void pressure_control_wave(bool low, bool hot, double sound, double pressure,
+                                   unsigned long long out[4]) {
+  out[0] = __popcll(__ballot(low));
+  out[1] = __popcll(__ballot(hot));
   unsigned long long s_bits = __double_as_longlong(sound), p_bits = __double_as_longlong(pressure);
   for (int off = 32; off > 0; off >>= 1) {
     const unsigned long long so = __shfl_down(s_bits, off, 64), po = __shfl_down(p_bits, off, 64);
     s_bits = so > s_bits ? so : s_bits;
     p_bits = po > p_bits ? po : p_bits;
   }
+  out[2] = s_bits;
+  out[3] = p_bits;
+}
+
+__global__ void k_pressure_control(K c, PressureControlArgs a, unsigned long long *acc) {
+  bool low, hot;
+  double sound, pressure;
+  pressure_control_cell(c, a, tid(), low, hot, sound, pressure);
+  unsigned long long r[4];
+  pressure_control_wave(low, hot, sound, pressure, r);
   if ((threadIdx.x & 63) == 0) {
-    if (n_low) atomicAdd(&acc[0], n_low);
-    if (n_hot) atomicAdd(&acc[1], n_hot);
-    atomicMax(&acc[2], s_bits);
-    atomicMax(&acc[3], p_bits);
+    if (r[0]) atomicAdd(&acc[0], r[0]);
+    if (r[1]) atomicAdd(&acc[1], r[1]);
+    atomicMax(&acc[2], r[2]);
+    atomicMax(&acc[3], r[3]);
   }
 }
 
@@ -2492,6 +2576,70 @@ int pressure_control(Problem &P, const double args[7], unsigned long long counts
   counts[1] = host[1];
   std::memcpy(&maxima[0], &host[2], sizeof(double));
   std::memcpy(&maxima[1], &host[3], sizeof(double));
+  return 0;
+}
+
+// The Earth driver's end-of-run gate (test_cubedsphere_dipole_background.c:
+// 807-893): minimum density and pressure of the active cells' primitive
+// state (conversion floors 1e-12) and the maximum total div B. acc = {min
+// density bits, min pressure bits, -, -}; the div B maximum goes through
+// k_div_b. Positive doubles order like their bit patterns, so min and max
+// are exact in any order.
+__global__ void k_gate_init(unsigned long long *acc) {
+  acc[threadIdx.x] = threadIdx.x < 2 ? __double_as_longlong(DBL_MAX) : 0ull;
+}
+
+__global__ void k_gates(K c, double gamma, unsigned long long *acc) {
+  const Layout &L = c.L;
+  const int lo[3] = {L.ghost, L.ghost, L.ghost};
+  const int hi[3] = {L.ghost + L.nr, L.ghost + L.n, L.ghost + L.n};
+  int f, i, j, k;
+  double rho = DBL_MAX, p = DBL_MAX;
+  if (box(tid(), lo, hi, f, i, j, k)) {
+    const long at = L.at(f, i, j, k);
+    double u[5];
+    for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
+    Prim w;
+    if (!cons_to_prim(u, gamma, 1e-12, 1e-12, w)) {
+      c.fail(91, at);
+    } else {
+      rho = w.rho;
+      p = w.p;
+    }
+  }
+  unsigned long long rb = __double_as_longlong(rho), pb = __double_as_longlong(p);
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long ro = __shfl_down(rb, off, 64), po = __shfl_down(pb, off, 64);
+    rb = ro < rb ? ro : rb;
+    pb = po < pb ? po : pb;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&acc[0], rb);
+    atomicMin(&acc[1], pb);
+  }
+}
+
+int final_gates(Problem &P, double gamma, double out[3]) {
+  if (!(gamma > 1.0)) {
+    set_error("final gates: gamma must exceed 1");
+    return -1;
+  }
+  unsigned long long *acc = P.acc4, host[4] = {0, 0, 0, 0};
+  K c = make_k(P);
+  const long n = 6L * P.L.nr * P.L.n * P.L.n;
+  k_gate_init<<<1, 4, 0, P.stream>>>(acc);
+  k_gates<<<nblocks(n, 256), 256, 0, P.stream>>>(c, gamma, acc);
+  k_div_b<<<nblocks(n, 256), 256, 0, P.stream>>>(c, acc + 2);
+  const hipError_t e = hipGetLastError();
+  const hipError_t m = hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, P.stream);
+  const hipError_t w = hipStreamSynchronize(P.stream);
+  if (e != hipSuccess || m != hipSuccess || w != hipSuccess) {
+    set_error("final gates: device error");
+    return -1;
+  }
+  std::memcpy(&out[0], &host[0], sizeof(double));
+  std::memcpy(&out[1], &host[1], sizeof(double));
+  std::memcpy(&out[2], &host[2], sizeof(double));
   return 0;
 }
 

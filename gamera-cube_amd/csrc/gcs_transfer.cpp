@@ -30,6 +30,24 @@ int plane_to_host(Problem &P, int plane, int face, const size_t ext[3],
   return 0;
 }
 
+int box_planes_to_host(Problem &P, int first, int ncomp, int face, const size_t ext[3],
+                       const size_t lo[3], const size_t hi[3], double *host) {
+  const Layout &L = P.L;
+  std::vector<double> &buf = P.staging;
+  buf.resize(L.face_stride);
+  for (int c = 0; c < ncomp; ++c) {
+    GCS_HIP(hipMemcpy(buf.data(), P.plane(first + c) + face * L.face_stride,
+                      L.face_stride * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = lo[0]; i < hi[0]; ++i)
+      for (size_t j = lo[1]; j < hi[1]; ++j) {
+        const double *row = buf.data() + (i * L.SJ + j) * L.SK;
+        for (size_t k = lo[2]; k < hi[2]; ++k)
+          host[((i * ext[1] + j) * ext[2] + k) * ncomp + c] = row[k];
+      }
+  }
+  return 0;
+}
+
 int host_to_plane(Problem &P, int plane, int face, const size_t ext[3],
                   const double *host, size_t stride) {
   const Layout &L = P.L;

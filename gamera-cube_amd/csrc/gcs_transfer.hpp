@@ -20,5 +20,9 @@ int planes_to_host(Problem &P, int first, int ncomp, int face,
                    const size_t ext[3], double *host);
 int host_to_planes(Problem &P, int first, int ncomp, int face,
                    const size_t ext[3], const double *host);
+// planes_to_host for the elements of the box lo <= (i, j, k) < hi only; the
+// host array's other elements are left as they are.
+int box_planes_to_host(Problem &P, int first, int ncomp, int face, const size_t ext[3],
+                       const size_t lo[3], const size_t hi[3], double *host);
 
 }  // namespace gcs

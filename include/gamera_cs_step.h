@@ -66,9 +66,11 @@ void gcs_problem_counters(gcs_problem *problem, int face, double out[2]);
  * values, ghost B is recovered from them). 0: skip that work and the
  * ghost-cell B of the halo fill and of the Earth boundary's recovery (no
  * step computation reads ghost-cell B); ghost U and face fluxes are
- * rewritten at the start of each step, ghost B is not maintained, and the
- * active state, clock and ionosphere potential are bit-identical to
- * fidelity 1's. No reference counterpart. */
+ * rewritten at the start of each step, ghost B is not maintained, nor are
+ * the predicted face flux, the predicted momentum and energy (the step
+ * reads only the predicted density) and the stress rates; the active state,
+ * clock and ionosphere potential are bit-identical to fidelity 1's. No
+ * reference counterpart. */
 int gcs_problem_set_fidelity(gcs_problem *problem, int on);
 
 /* 0 (default): FP64 throughout, the drop-in's arithmetic. 1: the mixed
@@ -104,6 +106,24 @@ int gcs_problem_pressure_control(gcs_problem *problem, double dt, double gamma,
  * 0 (default): launch the step's kernels one by one. Results are identical
  * either way. No reference counterpart. */
 int gcs_problem_set_graph(gcs_problem *problem, int on);
+
+/* The serial Earth driver's end-of-run gate
+ * (test_cubedsphere_dipole_background.c:807-893) on the current state:
+ * out = {minimum density, minimum pressure (primitive conversion with
+ * floors 1e-12, 1e-12), maximum total (B1 + B0) div B per volume} over the
+ * active cells. The driver passes when out[0] > 0, out[1] > 0 and
+ * out[2] < 1e-9. Returns -1 on a cell the conversion rejects. */
+int gcs_problem_final_gates(gcs_problem *problem, double out[3]);
+
+/* Replace the problem's solar-wind series by `count` samples (any number;
+ * the config holds at most GCS_MAX_WIND_SAMPLES): time[n] in code time,
+ * state[8 n .. 8 n + 7] = density, pressure, velocity (3), magnetic field
+ * (3); linear != 0 interpolates between samples, else holds each sample
+ * (gamera_solar_wind_set, nonorthogonal/src/solar_wind.c:55-80; sampled
+ * with the ballistic delay of :234-275 at the series' reference point).
+ * Times must be finite and increasing, states finite. */
+int gcs_problem_set_wind(gcs_problem *problem, size_t count, const double *time,
+                         const double *state, int linear);
 
 /* 1: run every face-flux, face-velocity and edge-EMF sweep with one face or
  * edge per thread (the untiled kernels), in FP64 and in the mixed form.

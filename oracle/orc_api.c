@@ -28,6 +28,8 @@ typedef struct {
   orc_halo_plan *halo;
   orc_edge_plan *edges;
   orc_mi *mi;           /* created on the first step when mi_enabled */
+  double *series_time;  /* a series set by orc_set_wind (else NULL) */
+  gamera_solar_wind_state *series_state;
   int mi_updates;
   double time, dt, previous_dt;
   long steps;
@@ -71,6 +73,8 @@ void orc_destroy(orc_problem *p) {
   orc_halo_plan_free(p->halo);
   orc_edge_plan_free(p->edges);
   orc_mi_destroy(p->mi);
+  free(p->series_time);
+  free(p->series_state);
   free(p);
 }
 
@@ -252,6 +256,45 @@ void orc_clock(const orc_problem *p, double out[4]) {
   out[1] = p->dt;
   out[2] = p->previous_dt;
   out[3] = (double)p->steps;
+}
+
+/* Replace the problem's wind series (solar_wind.c:55-80, gamera_solar_wind_set
+ * without its Bx relation, as the series of the config): count samples,
+ * state[8 n ..] = density, pressure, velocity, magnetic. */
+int orc_set_wind(orc_problem *p, size_t count, const double *time,
+                 const double *state, int linear) {
+  if (p == NULL || count == 0 || time == NULL || state == NULL) return -1;
+  for (size_t n = 0; n < count; ++n) {
+    if (!isfinite(time[n]) || (n > 0 && !(time[n] > time[n - 1]))) return -1;
+    for (int q = 0; q < 8; ++q)
+      if (!isfinite(state[8 * n + q])) return -1;
+  }
+  double *t = malloc(count * sizeof(*t));
+  gamera_solar_wind_state *w = malloc(count * sizeof(*w));
+  if (t == NULL || w == NULL) {
+    free(t);
+    free(w);
+    return -1;
+  }
+  for (size_t n = 0; n < count; ++n) {
+    const double *x = &state[8 * n];
+    t[n] = time[n];
+    w[n].density = x[0];
+    w[n].pressure = x[1];
+    for (int c = 0; c < 3; ++c) {
+      w[n].velocity[c] = x[2 + c];
+      w[n].magnetic[c] = x[5 + c];
+    }
+  }
+  free(p->series_time);
+  free(p->series_state);
+  p->series_time = t;
+  p->series_state = w;
+  p->wind.count = count;
+  p->wind.time = t;
+  p->wind.state = w;
+  p->wind.linear_interpolation = count > 1 ? (linear != 0) : 1;
+  return 0;
 }
 
 void orc_set_clock(orc_problem *p, double time, double dt, double previous_dt,

@@ -380,6 +380,39 @@ fail:
   return NULL;
 }
 
+/* Replace the wind series with the reference's own setter
+ * (gamera_solar_wind_set, solar_wind.c:55-80): count samples,
+ * state[8 n ..] = density, pressure, velocity, magnetic. */
+int refh_set_wind(refh_problem *p, size_t count, const double *time,
+                  const double *state, int linear) {
+  if (p == NULL || count == 0 || time == NULL || state == NULL) return -1;
+  gamera_solar_wind_state *w = calloc(count, sizeof(*w));
+  if (w == NULL) return -1;
+  for (size_t n = 0; n < count; ++n) {
+    const double *x = &state[8 * n];
+    w[n].density = x[0];
+    w[n].pressure = x[1];
+    for (int c = 0; c < 3; ++c) {
+      w[n].velocity[c] = x[2 + c];
+      w[n].magnetic[c] = x[5 + c];
+    }
+  }
+  const int rc = gamera_solar_wind_set(&p->wind, count, time, w);
+  free(w);
+  if (rc != 0) return -1;
+  p->wind.linear_interpolation = count > 1 ? (linear != 0) : 1;
+  return 0;
+}
+
+/* 0: hydrodynamic steps (the step options' use_mhd = false in all three
+ * option groups, and the dt rule without the Alfven speed); 1: MHD. */
+void refh_set_mhd(refh_problem *p, int on) {
+  if (p == NULL) return;
+  p->options.stress.use_mhd = on != 0;
+  p->options.update.use_mhd = on != 0;
+  p->timestep.use_mhd = on != 0;
+}
+
 /* Clock of the driver loop (time, dt of the last step, previous dt, steps),
  * e.g. to continue a run another implementation started. */
 void refh_set_clock(refh_problem *p, double time, double dt, double previous_dt,

@@ -192,3 +192,33 @@ def max_divergence(p, cfg):
             net += phi[tuple(hi)] - phi[tuple(sl)]
         worst = max(worst, float(np.max(np.abs(net) / vol[g:g + nr, g:g + n, g:g + n])))
     return worst
+
+
+def wind_series(samples=40, seed=11):
+    """A long solar-wind series for the step's Earth boundary: times from
+    -20 to about +3.4 code units (the ballistic delay across the grid spans
+    about 15), density, pressure, speed and IMF varying between samples,
+    Bz switching sign in blocks. Returns (time, state[n][8] = rho, p, v, B)."""
+    rng = np.random.default_rng(seed)
+    time = -20.0 + 0.6 * np.arange(samples)
+    state = np.zeros((samples, 8))
+    state[:, 0] = 5.0 + rng.uniform(-2.0, 2.0, samples)
+    state[:, 1] = state[:, 0] * 0.08 * (1.0 + rng.uniform(-0.2, 0.2, samples))
+    state[:, 2] = 4.0 + rng.uniform(-0.5, 0.5, samples)
+    state[:, 5] = rng.uniform(-0.1, 0.1, samples)
+    state[:, 6] = rng.uniform(-0.3, 0.3, samples)
+    state[:, 7] = np.where((np.arange(samples) // 5) % 2 == 0, -1.09, 1.09)
+    return time, state
+
+
+def set_wind(p, time, state, linear=True):
+    """Give a problem of any of the three APIs the wind series (time, state)."""
+    fn = getattr(p._lib, f"{p.prefix}_set_wind")
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                   ctypes.c_int]
+    t = np.ascontiguousarray(time, dtype=np.float64)
+    s = np.ascontiguousarray(state, dtype=np.float64)
+    rc = fn(p._handle, t.size, t.ctypes.data, s.ctypes.data, 1 if linear else 0)
+    if rc != 0:
+        raise RuntimeError(f"{p.prefix}: set_wind failed")

@@ -28,6 +28,12 @@ CASES = [
     ("dipole_background", "dipole_background", []),
     ("dipole_background_72x16x16", "dipole_background", ["earth-local"]),
     ("earth_mi_dipole_zero_fac", "dipole_background", ["earth-mi-local", "1"]),
+    # hydrodynamic steps (use_mhd = false) and the face-centred MHD blast
+    # (cubedsphere/CMakeLists.txt:139-147)
+    ("hd_spherical_blast", "hd_spherical_blast", []),
+    ("hd_spherical_blast_six_faces", "hd_spherical_blast", ["-", "24", "1.5"]),
+    ("mhd_spherical_blast_face_center", "hd_spherical_blast",
+     ["-", "32", "1.5", "face-center-mhd", "0.1"]),
 ]
 
 

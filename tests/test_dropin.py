@@ -8,8 +8,8 @@ pointer; the reference's dt rule, Earth context and storage are used
 unchanged). Every array the step writes is compared after each step with the
 reference's scaled measure |a - b| / max(1, |a|).
 
-hydro_rate and maxwell_rate are not compared: the device fuses them into the
-update and leaves those host arrays untouched.
+The stress rates (hydro_rate, maxwell_rate) of the active cells, which the
+device update stores when it reproduces the storage, are compared as well.
 """
 import ctypes
 import os
@@ -33,7 +33,8 @@ WRITTEN = ([problem.CONSERVED, problem.OLD_CONSERVED, problem.PREDICTED_CONSERVE
            + [problem.PREDICTED_FACE_FLUX + d for d in range(3)]
            + [problem.EDGE_EMF + d for d in range(3)]
            + [problem.FLUID_FACE_FLUX + d for d in range(3)]
-           + [problem.MAXWELL_FACE_FLUX + d for d in range(3)])
+           + [problem.MAXWELL_FACE_FLUX + d for d in range(3)]
+           + [problem.HYDRO_RATE, problem.MAXWELL_RATE])
 
 
 @pytest.fixture(scope="module")
@@ -61,7 +62,8 @@ def address(lib, name):
 # Area-scaled face fluxes are intermediates of order 1e3 whose components
 # cancel; their differences are measured against the array's largest value.
 INTERMEDIATE = ([problem.FLUID_FACE_FLUX + d for d in range(3)]
-                + [problem.MAXWELL_FACE_FLUX + d for d in range(3)])
+                + [problem.MAXWELL_FACE_FLUX + d for d in range(3)]
+                + [problem.HYDRO_RATE, problem.MAXWELL_RATE])
 
 
 def worst(a, b, fields=WRITTEN):
@@ -84,13 +86,17 @@ def release(gpu_lib, ref_lib, p):
                                ref_lib.refh_storage(p._handle))
 
 
-def run_pair(ref_lib, gpu, name, steps, wrapped, step=None):
+def run_pair(ref_lib, gpu, name, steps, wrapped, step=None, mhd=True):
     """`step`: address of a function with the planned entry point's
-    prototype (default: libgcs.so's own)."""
+    prototype (default: libgcs.so's own); mhd=False: hydrodynamic steps."""
     lib = gpu.library()
     cfg = gcs_config.workload(name)
     a = problem.Problem(ref_lib, "refh", cfg)
     b = problem.Problem(ref_lib, "refh", cfg)
+    if not mhd:
+        ref_lib.refh_set_mhd.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        ref_lib.refh_set_mhd(a._handle, 0)
+        ref_lib.refh_set_mhd(b._handle, 0)
     if step is None:
         step = address(lib, "gamera_cs_advance_six_with_background_planned")
     ref_lib.refh_set_step(b._handle, step)
@@ -114,10 +120,19 @@ def run_pair(ref_lib, gpu, name, steps, wrapped, step=None):
         release(lib, ref_lib, b)
 
 
-@pytest.mark.parametrize("name,steps", [("tiny", 4), ("small", 8)])
+@pytest.mark.parametrize("name,steps", [("tiny", 4), ("small", 8), ("ghost5", 4)])
 def test_dropin_matches_reference_step(ref_global, gpu, name, steps):
-    """Earth callbacks recognised by address and run as device kernels."""
+    """Earth callbacks recognised by address and run as device kernels
+    (ghost5: five ghost layers, which the reference allows)."""
     run_pair(ref_global, gpu, name, steps, wrapped=False)
+
+
+@pytest.mark.parametrize("name,steps", [("small", 6), ("fast_wind", 4)])
+def test_dropin_hydrodynamic_steps_match_reference(ref_global, gpu, name, steps):
+    """use_mhd = false (nonorthogonal_step.c:554, nonorthogonal_sweep.c:
+    150-171): fluid fluxes only, no EMF or CT; B and the face fluxes are
+    carried over unchanged."""
+    run_pair(ref_global, gpu, name, steps, wrapped=False, mhd=False)
 
 
 # The planned entry point's prototype (cubedsphere_step.h:58-67).

@@ -1,13 +1,14 @@
 """The HIP step against the CPU oracle and the compiled reference, on the
 GPU. Setup must be bitwise the reference's; steps agree to the reference's
 scaled parity measure |a - b| / max(1, |a|) (tolerances below)."""
+import ctypes
 import time
 
 import numpy as np
 import pytest
 
 from conftest import (bits_equal, counters, gcs_config, max_divergence,
-                      oracle_from, problem, scaled_error)
+                      oracle_from, problem, scaled_error, set_wind, wind_series)
 
 pytestmark = pytest.mark.gpu
 
@@ -31,7 +32,7 @@ def worst_error(a, b, fields):
     return worst
 
 
-@pytest.mark.parametrize("name", ["tiny", "small", "ragged"])
+@pytest.mark.parametrize("name", ["tiny", "small", "ragged", "ghost5"])
 def test_setup_is_bitwise_the_reference_setup(gpu, ref, name):
     cfg = gcs_config.workload(name)
     print(f"setup parity {name}", flush=True)
@@ -45,7 +46,8 @@ def test_setup_is_bitwise_the_reference_setup(gpu, ref, name):
             assert np.array_equal(r.edge_valid(f, d), g.edge_valid(f, d))
 
 
-@pytest.mark.parametrize("name,steps", [("tiny", 10), ("small", 10), ("ragged", 10)])
+@pytest.mark.parametrize("name,steps", [("tiny", 10), ("small", 10), ("ragged", 10),
+                                        ("ghost5", 6)])
 def test_steps_match_oracle(gpu, orc, name, steps):
     cfg = gcs_config.workload(name)
     print(f"step parity {name} x{steps}", flush=True)
@@ -92,6 +94,26 @@ def test_emergency_diffusion_steps_match_oracle(gpu, orc):
     assert max_divergence(g, cfg) < 1e-10
 
 
+@pytest.mark.parametrize("linear", [True, False])
+def test_long_wind_series_steps_match_oracle(gpu, orc, linear):
+    """A 40-sample solar-wind series (any length, solar_wind.c:55-225) set on
+    the device problem and on the oracle: ballistic sampling, wind weights,
+    the outer ghosts and EMF over 8 steps."""
+    cfg = gcs_config.workload("small")
+    print(f"long wind series, linear={linear}", flush=True)
+    g = gpu.GpuProblem(cfg)
+    o = oracle_from(g, orc, cfg)
+    time, state = wind_series()
+    set_wind(g, time, state, linear)
+    set_wind(o, time, state, linear)
+    for s in range(8):
+        g.advance(1)
+        o.advance(1)
+        err = worst_error(o, g, STATE)
+        assert err <= STEP_TOL * (s + 1), (s, err)
+    print(f"  scaled error after 8 steps {err:.2e}", flush=True)
+
+
 def test_explicit_step_matches_oracle(gpu, orc):
     cfg = gcs_config.workload("small")
     g = gpu.GpuProblem(cfg)
@@ -130,3 +152,37 @@ def test_config0_200_steps_match_oracle(gpu, orc):
     for f in range(gcs_config.FACE_COUNT):
         u = g.field(f, problem.CONSERVED).reshape(-1, 5)
         assert np.all(u[:, 0] > 0)
+
+
+def test_final_gates_match_the_drivers_gate(gpu, orc):
+    """gcs_problem_final_gates against the Earth driver's end-of-run gate
+    (test_cubedsphere_dipole_background.c:807-893) evaluated on the oracle's
+    state after the same 20 steps: minimum density and pressure of the
+    primitive conversion with floors 1e-12, maximum total div B."""
+    cfg = gcs_config.workload("small")
+    g = gpu.GpuProblem(cfg)
+    o = oracle_from(g, orc, cfg)
+    g.advance(20)
+    o.advance(20)
+    out = (ctypes.c_double * 3)()
+    fn = gpu.library().gcs_problem_final_gates
+    fn.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+    assert fn(g._handle, out) == 0
+    ni, nj, nk = cfg.extent
+    gh, nr, n = cfg.ghost, cfg.radial_cells, cfg.angular_cells
+    gamma = cfg.startup.gamma
+    rho_min = p_min = np.inf
+    for f in range(gcs_config.FACE_COUNT):
+        u = o.field(f, problem.CONSERVED).reshape(ni, nj, nk, 5)[
+            gh:gh + nr, gh:gh + n, gh:gh + n].reshape(-1, 5)
+        v = u[:, 1:4] / u[:, 0:1]
+        v2 = (v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2]
+        p = np.maximum((gamma - 1.0) * (u[:, 4] - 0.5 * u[:, 0] * v2), 1e-12)
+        rho_min = min(rho_min, float(np.maximum(u[:, 0], 1e-12).min()))
+        p_min = min(p_min, float(p.min()))
+    div = max_divergence(o, cfg)
+    print(f"gates: GPU {list(out)}; oracle state ({rho_min}, {p_min}, {div})", flush=True)
+    assert abs(out[0] - rho_min) <= 1e-12 * rho_min
+    assert abs(out[1] - p_min) <= 1e-12 * p_min
+    assert abs(out[2] - div) <= 1e-6 * div
+    assert out[0] > 0 and out[1] > 0 and out[2] < 1e-9

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from conftest import (bits_equal, counters, gcs_config, max_divergence,
-                      oracle_from, problem, reference_from)
+                      oracle_from, problem, reference_from, set_wind, wind_series)
 
 CHECKED = (problem.STATE_FIELDS
            + [problem.PREDICTED_CONSERVED, problem.PREDICTED_CELL_MAGNETIC]
@@ -15,7 +15,7 @@ CHECKED = (problem.STATE_FIELDS
 
 
 @pytest.mark.parametrize("name,steps", [("tiny", 4), ("small", 12), ("ragged", 6),
-                                        ("fast_wind", 5)])
+                                        ("fast_wind", 5), ("ghost5", 4)])
 def test_oracle_steps_bitwise_equal_reference(ref, orc, name, steps):
     cfg = gcs_config.workload(name)
     r = problem.Problem(ref, "refh", cfg)
@@ -88,3 +88,23 @@ def test_oracle_config0_200_steps_bitwise_equal_reference(ref, orc):
             for f in range(gcs_config.FACE_COUNT):
                 assert bits_equal(r.field(f, fid), o.field(f, fid)), (chunk, fid, f)
     assert max_divergence(o, cfg) < 1e-9
+
+
+@pytest.mark.parametrize("linear", [True, False])
+def test_oracle_long_wind_series_bitwise_equal_reference(ref, orc, linear):
+    """A 40-sample wind series (the reference accepts any length,
+    solar_wind.c:55-80): the Earth boundary's ballistic sampling, wind
+    weights and outer EMF over a run, oracle and reference bit for bit."""
+    cfg = gcs_config.workload("small")
+    r = problem.Problem(ref, "refh", cfg)
+    o = oracle_from(r, orc, cfg)
+    time, state = wind_series()
+    set_wind(r, time, state, linear)
+    set_wind(o, time, state, linear)
+    for _ in range(6):
+        r.advance(1)
+        o.advance(1)
+        assert r.clock() == o.clock()
+        for fid in problem.STATE_FIELDS:
+            for f in range(gcs_config.FACE_COUNT):
+                assert bits_equal(r.field(f, fid), o.field(f, fid)), (fid, f)
