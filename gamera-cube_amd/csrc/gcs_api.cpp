@@ -532,6 +532,20 @@ int gcs_problem_pressure_control(Problem *P, double dt, double gamma, double den
   return s.failed ? gcs::check_failure(*P) : 0;
 }
 
+int gcs_problem_pressure_control_noalloc(Problem *P, double dt, double gamma,
+                                         double density_floor, double pressure_floor,
+                                         double light_speed, double low_density_threshold,
+                                         double low_density_sound_speed,
+                                         unsigned long long counts[2], double maxima[2]) {
+  if (P == nullptr || counts == nullptr || maxima == nullptr) return -1;
+  const double args[7] = {dt, gamma, density_floor, pressure_floor, light_speed,
+                          low_density_threshold, low_density_sound_speed};
+  if (gcs::pressure_control_noalloc(*P, args, counts, maxima) != 0) return -1;
+  gcs::StepScalars s;
+  if (gcs::read_scalars(*P, s) != 0) return -1;
+  return s.failed ? gcs::check_failure(*P) : 0;
+}
+
 int gcs_problem_final_gates(Problem *P, double out[3]) {
   if (P == nullptr || out == nullptr) return -1;
   if (gcs::final_gates(*P, P->cfg.startup.gamma, out) != 0) return -1;

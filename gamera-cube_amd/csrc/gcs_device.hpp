@@ -291,6 +291,10 @@ int final_gates(Problem &P, double gamma, double out[3]);
 // rho, p, v, B) into the problem's device buffer, growing it when needed,
 // and point P.wind_series at it; the other fields are set by the caller.
 int upload_wind(Problem &P, int count, const double *time, const double *state);
+// pressure_control with no device allocation: scratch in a static device
+// array (at most 16,777,216 active cells; calls overlapping on one GPU race).
+int pressure_control_noalloc(Problem &P, const double args[7], unsigned long long counts[2],
+                             double maxima[2]);
 int enqueue_phase(Problem &P, Phase phase);
 // Constants derived from uploaded geometry and background: recovery
 // pivots, edge-B solve, B0 normal and edge-tangential components.

@@ -2579,6 +2579,136 @@ int pressure_control(Problem &P, const double args[7], unsigned long long counts
   return 0;
 }
 
+// ---- pressure control without device allocation ---------------------------------
+//
+// gcs_problem_pressure_control_noalloc: the same cells and results as
+// pressure_control, for callers that cannot allocate device memory. Its
+// scratch is this file-scope array, one copy per GPU for the life of the
+// process: 4 words of partial results per block of 256 active cells, then
+// the 4 words of the total. Every block writes its own partials before the
+// total is formed, so nothing needs clearing between calls; calls that
+// overlap on one GPU share the array and race.
+constexpr int kScratchBlocks = 65536;  // at most 16,777,216 active cells
+static __device__ __align__(256) unsigned long long scratch[4 * (kScratchBlocks + 1)];
+
+__global__ void __launch_bounds__(256) k_pressure_control_blocks(K c, PressureControlArgs a,
+                                                                 unsigned long long *partial) {
+  bool low, hot;
+  double sound, pressure;
+  pressure_control_cell(c, a, tid(), low, hot, sound, pressure);
+  unsigned long long r[4];
+  pressure_control_wave(low, hot, sound, pressure, r);
+  __shared__ unsigned long long waves[4][4];
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int q = 0; q < 4; ++q) waves[wave][q] = r[q];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int q = threadIdx.x;
+    unsigned long long v = waves[0][q];
+    for (int w = 1; w < 4; ++w) v = q < 2 ? v + waves[w][q] : (waves[w][q] > v ? waves[w][q] : v);
+    partial[4L * blockIdx.x + q] = v;
+  }
+}
+
+// One block of 256 threads folds n blocks' partials into total[0..3].
+__global__ void __launch_bounds__(256) k_pressure_control_total(
+    const unsigned long long *partial, long n, unsigned long long *total) {
+  unsigned long long r[4] = {0, 0, 0, 0};
+  for (long b = threadIdx.x; b < n; b += blockDim.x) {
+    r[0] += partial[4 * b];
+    r[1] += partial[4 * b + 1];
+    r[2] = partial[4 * b + 2] > r[2] ? partial[4 * b + 2] : r[2];
+    r[3] = partial[4 * b + 3] > r[3] ? partial[4 * b + 3] : r[3];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    r[0] += __shfl_down(r[0], off, 64);
+    r[1] += __shfl_down(r[1], off, 64);
+    const unsigned long long so = __shfl_down(r[2], off, 64), po = __shfl_down(r[3], off, 64);
+    r[2] = so > r[2] ? so : r[2];
+    r[3] = po > r[3] ? po : r[3];
+  }
+  __shared__ unsigned long long waves[4][4];
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int q = 0; q < 4; ++q) waves[wave][q] = r[q];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int q = threadIdx.x;
+    unsigned long long v = waves[0][q];
+    for (int w = 1; w < 4; ++w) v = q < 2 ? v + waves[w][q] : (waves[w][q] > v ? waves[w][q] : v);
+    total[q] = v;
+  }
+}
+
+namespace {
+// Device address of `scratch` on each GPU, looked up once per device with
+// that device current (host code never passes the array's name itself).
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_lock;
+unsigned long long *g_scratch[kMaxScratchDevices] = {};
+
+int device_scratch(unsigned long long **out) {
+  int dev = -1;
+  GCS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices) {
+    set_error("pressure control (no allocation): device index out of range");
+    return -1;
+  }
+  std::lock_guard<std::mutex> guard(g_scratch_lock);
+  if (g_scratch[dev] == nullptr) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr) {
+      set_error(std::string("pressure control (no allocation): scratch address: ") +
+                hipGetErrorString(e));
+      return -1;
+    }
+    g_scratch[dev] = static_cast<unsigned long long *>(ptr);
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+}  // namespace
+
+int pressure_control_noalloc(Problem &P, const double args[7], unsigned long long counts[2],
+                             double maxima[2]) {
+  const PressureControlArgs a{args[0], args[1], args[2], args[3], args[4], args[5], args[6]};
+  if (!(a.dt > 0.0) || !(a.gamma > 1.0) || !(a.density_floor > 0.0) ||
+      !(a.pressure_floor > 0.0) || !(a.light_speed > 0.0) ||
+      !(a.low_density_threshold > 0.0) || !(a.low_density_sound_speed >= 0.0)) {
+    set_error("pressure control: invalid arguments");
+    return -1;
+  }
+  const long cells = 6L * P.L.nr * P.L.n * P.L.n;
+  const long blocks = (cells + 255) / 256;
+  if (blocks > kScratchBlocks) {
+    set_error("pressure control (no allocation): " + std::to_string(cells) +
+              " active cells exceed the scratch limit of " +
+              std::to_string(256L * kScratchBlocks) + " (use gcs_problem_pressure_control)");
+    return -1;
+  }
+  unsigned long long *ptr = nullptr, host[4] = {0, 0, 0, 0};
+  if (device_scratch(&ptr) != 0) return -1;
+  unsigned long long *total = ptr + 4L * kScratchBlocks;
+  k_pressure_control_blocks<<<(unsigned)blocks, 256, 0, P.stream>>>(make_k(P), a, ptr);
+  k_pressure_control_total<<<1, 256, 0, P.stream>>>(ptr, blocks, total);
+  const hipError_t e = hipGetLastError();
+  const hipError_t m = hipMemcpyAsync(host, total, sizeof(host), hipMemcpyDeviceToHost, P.stream);
+  const hipError_t w = hipStreamSynchronize(P.stream);
+  if (e != hipSuccess || m != hipSuccess || w != hipSuccess) {
+    set_error("pressure control (no allocation): device error");
+    return -1;
+  }
+  P.cfl_folded = false;
+  P.host_dirty = true;
+  counts[0] = host[0];
+  counts[1] = host[1];
+  std::memcpy(&maxima[0], &host[2], sizeof(double));
+  std::memcpy(&maxima[1], &host[3], sizeof(double));
+  return 0;
+}
+
 // The Earth driver's end-of-run gate (test_cubedsphere_dipole_background.c:
 // 807-893): minimum density and pressure of the active cells' primitive
 // state (conversion floors 1e-12) and the maximum total div B. acc = {min

@@ -100,6 +100,20 @@ int gcs_problem_pressure_control(gcs_problem *problem, double dt, double gamma,
                                  double low_density_sound_speed,
                                  unsigned long long counts[2], double maxima[2]);
 
+/* gcs_problem_pressure_control without any device allocation, for callers
+ * that cannot allocate (code under stream capture, tight driver loops that
+ * apply the control after every step). No counterpart in the reference.
+ * Same arguments, cells and results. Its scratch is a static device array
+ * of 2 MiB per GPU, held until the process ends: at most 16,777,216 active
+ * cells (6 x 96^2 x 192 fits; larger problems are rejected with an error and
+ * use gcs_problem_pressure_control). Calls that overlap on one GPU share the
+ * array and race. */
+int gcs_problem_pressure_control_noalloc(
+    gcs_problem *problem, double dt, double gamma, double density_floor,
+    double pressure_floor, double light_speed, double low_density_threshold,
+    double low_density_sound_speed, unsigned long long counts[2],
+    double maxima[2]);
+
 /* 1: run each driver step (outside profiled timing) as one hipGraph launch,
  * captured per time-level parity on first use and recaptured when the
  * precision, fidelity, ghost-recovery state or ionosphere view changes.
