@@ -40,9 +40,9 @@ struct StepScalars {
   unsigned long long emergency_speed_bits[6];  // max, as IEEE bits (>= 0)
 };
 
-// Solar-wind series for device sampling (solar_wind.h:24-35).
-// A solar-wind time series of any length (gamera_solar_wind_series,
-// solar_wind.h): the samples live in a device buffer of the problem.
+// A solar-wind time series of any length for device sampling
+// (gamera_solar_wind_series, solar_wind.h:24-35): the samples live in a
+// device buffer of the problem.
 struct WindSeries {
   int count, linear;
   const double *time;   // device, count sample times
@@ -263,6 +263,32 @@ struct MiReport {
 };
 void mi_report(const Problem &P, MiReport &out);
 int mi_download(Problem &P, int what, std::vector<double> &out);  // 0 potential, 1 fac
+// The MI state as the reference's gamera_cs_mi_dipole_* accessors report it.
+struct MiInfo {
+  bool ready = false;
+  int updates = 0;
+  double epoch_s = 0.0, next_update_s = 0.0, sample_radius = 0.0, max_colat = 0.0,
+         potential_norm = 0.0;
+  int iterations[2] = {0, 0};
+  double initial_residual[2] = {0.0, 0.0}, final_residual[2] = {0.0, 0.0};
+  int converged[2] = {0, 0};
+  const double *potential = nullptr;  // device [2][nt][np]
+};
+void mi_info(const Problem &P, MiInfo &out);
+// Let the step kernels read the potential (on: when the state is ready) or
+// not (the Earth context of a call has no ionosphere).
+void mi_view_enable(Problem &P, bool on);
+// Replace the potential ([2][nt][np], V) and the update clock; ready after.
+int mi_import_potential(Problem &P, const double *values, double epoch_s,
+                        double next_update_s);
+// J-parallel, magnetic latitude, longitude of the active sample-shell cells
+// at the last update, [3][6][n][n].
+int mi_shell_current(Problem &P, std::vector<double> &out);
+// Evaluate gamera_cs_mi_dipole_potential (what 0) or _ghost_velocity (what
+// 1) at one point on the device from a potential view; out = {value or
+// velocity[3]}; returns 0, or -1 where the reference's function would.
+int mi_point(const MiView &view, int what, const double x[3], const double v[3],
+             double *result, double out[3]);
 
 // Setup: geometry, background, initial state and plans.
 int setup_problem(Problem &P);

@@ -33,6 +33,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -41,6 +42,7 @@
 #include <vector>
 
 #include "gamera_cs_dropin.h"
+#include "gamera_cs_mi_dropin.h"
 #include "gamera_cs_mpi_dropin.h"
 #include "gcs_device.hpp"
 #include "gcs_transfer.hpp"
@@ -71,12 +73,38 @@ struct Mirror {
   Problem *P = nullptr;
   bool device_current = false;  // resident mode: device state is newest
   std::vector<double> wind;     // the wind samples last uploaded
+  struct MiHandle *mi = nullptr;  // its ionosphere state, if any
+};
+
+// A gamera_cs_mi_dipole of this library: the MI state of a mirror's problem
+// (gamera_cs_mi_dipole_create), or a potential alone on the device
+// (gamera_cs_mi_dipole_create_potential_view).
+struct MiHandle {
+  Mirror *m = nullptr;  // null for a potential view, or once the mirror is gone
+  gamera_cs_mi_dipole_config cfg{};
+  gamera_cs_earth_startup startup{};
+  size_t active[3] = {0, 0, 0}, ghost = 0;
+  // potential view: its potential on the device, clock and readiness
+  double *view_potential = nullptr;
+  MiView view;
+  bool view_ready = false;
+  double view_epoch = 0.0, view_next = 0.0;
+  double *query = nullptr;  // 4 device doubles for point queries
+  std::vector<double> host_fac, host_potential;  // analysis-view copies
 };
 
 std::mutex g_lock;
+std::vector<MiHandle *> g_mi;
+
+MiHandle *find_mi(const void *state) {
+  for (MiHandle *h : g_mi)
+    if (h == state) return h;
+  return nullptr;
+}
 std::vector<Mirror *> g_mirrors;
 Callbacks g_cb;
 bool g_resident = false;
+long g_host_rounds = 0;  // callbacks run on the host with the state moved there and back
 
 // cubedsphere_step.c:7-11
 bool same_value(double a, double b) {
@@ -328,6 +356,7 @@ int state_to_host(Problem &P, gamera_no_storage *st) {
 
 void release(Mirror *m) {
   if (m == nullptr) return;
+  if (m->mi != nullptr) m->mi->m = nullptr;  // its MI state goes with the problem
   free_problem(m->P);
   delete m;
 }
@@ -447,14 +476,18 @@ void set_options(Problem &P, const gamera_no_advance_options &o) {
 // The Earth context can run on the device when it matches what the mirror
 // holds (cubedsphere_earth_boundary.c:9-14 for the rest of the checks).
 bool earth_on_device(const Mirror &m, const gamera_cs_earth_boundary *b,
-                     const gamera_no_background_field *background, bool need_no_mi) {
+                     const gamera_no_background_field *background) {
   if (b == nullptr || b->startup == nullptr || b->wind == nullptr ||
       b->background == nullptr || b->background != background ||
       !m.has_background || !std::isfinite(b->density_floor) ||
       !(b->density_floor > 0.0) || !std::isfinite(b->pressure_floor) ||
       !(b->pressure_floor > 0.0))
     return false;
-  if (need_no_mi && b->mi != nullptr) return false;
+  // an ionosphere on the device only when it is this mirror's own
+  if (b->mi != nullptr) {
+    const MiHandle *h = find_mi(b->mi);
+    if (h == nullptr || h->m != &m) return false;
+  }
   const gamera_solar_wind_series &w = *b->wind;
   return w.count >= 1 && w.count <= (size_t)INT32_MAX && w.time != nullptr &&
          w.state != nullptr;
@@ -534,7 +567,7 @@ int advance(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
   bool host_radial = false, host_flux = false, host_emf = false;
   const gamera_cs_earth_boundary *wall = nullptr;
   if (radial != nullptr) {
-    if (radial == g_cb.radial && earth_on_device(*m, rb, bg, true)) {
+    if (radial == g_cb.radial && earth_on_device(*m, rb, bg)) {
       bd.earth_cells = true;  // rewrites every radial ghost the outflow fill writes
       wall = rb;
     } else {
@@ -549,7 +582,7 @@ int advance(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
     else host_flux = true;
   }
   if (o->edge_emf_sync != nullptr) {
-    if (o->edge_emf_sync == g_cb.emf && earth_on_device(*m, eb, bg, true)) {
+    if (o->edge_emf_sync == g_cb.emf && earth_on_device(*m, eb, bg)) {
       bd.earth_emf = true;
       if (wall == nullptr) wall = eb;
     } else {
@@ -570,6 +603,9 @@ int advance(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
     s.strict_wall = wall->strict_inner_gas_wall != 0;
     if (set_wind(P, m->wind, *wall) != 0) return -1;
   }
+  // the wall reads the potential only when its context has this mirror's
+  // ionosphere (cubedsphere_earth_boundary.c:58, :440)
+  mi_view_enable(P, wall != nullptr && wall->mi != nullptr);
 
   // Scalars of this call.
   StepScalars sc;
@@ -602,6 +638,7 @@ int advance(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
   const size_t lower[3] = {ghost, ghost, ghost};
   const size_t upper[3] = {ghost + active[0], ghost + active[1], ghost + active[2]};
   auto host_round = [&](auto &&call) -> int {
+    ++g_host_rounds;
     if (state_to_host(P, storage) != 0) return -1;
     const int status = call();
     if (state_to_device(P, storage) != 0) return -1;
@@ -837,6 +874,475 @@ int gcs_dropin_release(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
 int gcs_dropin_mirror_count(void) {
   std::lock_guard<std::mutex> guard(gcs::g_lock);
   return (int)gcs::g_mirrors.size();
+}
+
+long gcs_dropin_host_round_trips(void) {
+  std::lock_guard<std::mutex> guard(gcs::g_lock);
+  return gcs::g_host_rounds;
+}
+
+}  // extern "C"
+
+// ---- the dipole ionosphere coupling (cubedsphere/cubedsphere_mi_dipole.h) -------
+
+namespace gcs {
+namespace {
+
+// cubedsphere_mi_dipole.c:45-57
+bool valid_mi_config(const gamera_cs_mi_dipole_config *c) {
+  return c != nullptr && c->azimuth_points >= 8U && c->colatitude_points >= 3U &&
+         c->current_shell < 64U && std::isfinite(c->ionosphere_radius_re) &&
+         c->ionosphere_radius_re > 0.0 && std::isfinite(c->pedersen_siemens) &&
+         c->pedersen_siemens > 0.0 && std::isfinite(c->coupling_interval_s) &&
+         c->coupling_interval_s > 0.0 && std::isfinite(c->length_norm_m) &&
+         c->length_norm_m > 0.0 && std::isfinite(c->speed_norm_m_s) &&
+         c->speed_norm_m_s > 0.0 && std::isfinite(c->mass_density_norm_kg_m3) &&
+         c->mass_density_norm_kg_m3 > 0.0 && c->maximum_iterations > 0 &&
+         std::isfinite(c->relative_tolerance) && c->relative_tolerance > 0.0;
+}
+
+// The constants of a potential view (cubedsphere_mi_dipole.c:59-85) in the
+// form the device point queries read.
+bool make_view(const gamera_cs_earth_startup &st, const gamera_cs_mi_dipole_config &c,
+               MiView &v) {
+  if (!valid_mi_config(&c) || !(c.ionosphere_radius_re < st.inner_radius) ||
+      c.azimuth_points > SIZE_MAX / c.colatitude_points)
+    return false;
+  const double mu0 = 4.0e-7 * std::acos(-1.0);
+  const double b_norm = c.speed_norm_m_s * std::sqrt(mu0 * c.mass_density_norm_kg_m3);
+  v = MiView();
+  v.np = (int)c.azimuth_points;
+  v.nt = (int)c.colatitude_points;
+  v.ri = c.ionosphere_radius_re;
+  v.max_colat = std::asin(std::sqrt(c.ionosphere_radius_re / st.inner_radius));
+  v.potential_norm = c.speed_norm_m_s * b_norm * c.length_norm_m;
+  v.length_norm = c.length_norm_m;
+  v.speed_norm = c.speed_norm_m_s;
+  v.b_norm = b_norm;
+  for (int e = 0; e < 3; ++e) v.moment[e] = st.dipole.moment.value[e];
+  v.cut_radius = st.cut_radius;
+  v.cut_length = st.cut_length;
+  return v.potential_norm > 0.0 && std::isfinite(v.max_colat);
+}
+
+void destroy_handle(MiHandle *h) {
+  if (h == nullptr) return;
+  if (h->m != nullptr) {
+    mi_destroy(*h->m->P);
+    h->m->P->mi_view = MiView();
+    h->m->mi = nullptr;
+  }
+  (void)hipFree(h->view_potential);
+  (void)hipFree(h->query);
+  for (size_t n = 0; n < g_mi.size(); ++n)
+    if (g_mi[n] == h) g_mi.erase(g_mi.begin() + (long)n);
+  delete h;
+}
+
+MiHandle *new_handle() {
+  MiHandle *h = new MiHandle();
+  if (hipMalloc(&h->query, 4 * sizeof(double)) != hipSuccess) {
+    h->query = nullptr;
+    delete h;
+    set_error("MI state: device allocation failed");
+    return nullptr;
+  }
+  g_mi.push_back(h);
+  return h;
+}
+
+bool handle_ready(const MiHandle *h) {
+  if (h == nullptr) return false;
+  if (h->m == nullptr) return h->view_potential != nullptr && h->view_ready;
+  MiInfo info;
+  mi_info(*h->m->P, info);
+  return info.ready;
+}
+
+// The view point queries read: the mirror problem's or the potential view's.
+bool query_view(const MiHandle *h, MiView &v) {
+  if (!handle_ready(h)) return false;
+  if (h->m != nullptr) {
+    // the problem's view with its potential, whether or not a step reads it
+    MiInfo info;
+    mi_info(*h->m->P, info);
+    v = h->m->P->mi_view;
+    v.potential = info.potential;
+    return v.potential != nullptr;
+  }
+  v = h->view;
+  v.potential = h->view_potential;
+  return true;
+}
+
+int point_query(const MiHandle *h, int what, const double x[3], const double vel[3],
+                double out[3]) {
+  MiView v;
+  if (!query_view(h, v)) return -1;
+  return mi_point(v, what, x, vel, h->query, out);
+}
+
+int polar_arrays(MiHandle *h, std::vector<double> &fac, std::vector<double> &pot) {
+  if (h->m == nullptr) return -1;
+  return mi_download(*h->m->P, 1, fac) != 0 || mi_download(*h->m->P, 0, pot) != 0 ? -1 : 0;
+}
+
+}  // namespace
+}  // namespace gcs
+
+extern "C" {
+
+int gamera_cs_mi_dipole_create(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+                               gamera_no_storage storage[GAMERA_CS_FACE_COUNT],
+                               const gamera_no_background_field background[GAMERA_CS_FACE_COUNT],
+                               const gamera_cs_earth_startup *startup, const size_t active[3],
+                               size_t ghost, const gamera_cs_mi_dipole_config *config,
+                               gamera_cs_mi_dipole **result) {
+  using namespace gcs;
+  if (result == nullptr) return -1;
+  *result = nullptr;
+  // cubedsphere_mi_dipole.c:165-170
+  if (grid == nullptr || storage == nullptr || background == nullptr || startup == nullptr ||
+      active == nullptr || ghost < 4U || active[0] < 3U || active[1] < 8U ||
+      active[1] != active[2] || !valid_mi_config(config) ||
+      config->current_shell + 1U >= active[0] ||
+      !(config->ionosphere_radius_re < startup->inner_radius) ||
+      config->azimuth_points > SIZE_MAX / config->colatitude_points)
+    return -1;
+  std::lock_guard<std::mutex> guard(g_lock);
+  Mirror *m = mirror_for(grid, storage, active, ghost, background);
+  if (m == nullptr) return -1;
+  if (m->mi != nullptr) destroy_handle(m->mi);  // one ionosphere per mirror
+  MiHandle *h = new_handle();
+  if (h == nullptr) return -1;
+  Problem &P = *m->P;
+  P.cfg.mi = *config;
+  P.cfg.hall_siemens = 0.0;  // the reference's adapter solves Pedersen only (:407-418)
+  P.cfg.startup = *startup;
+  P.cfg.radial_cells = active[0];
+  P.cfg.angular_cells = active[1];
+  P.cfg.ghost = ghost;
+  P.cfg.mi_enabled = 1;
+  if (mi_create(P) != 0) {
+    destroy_handle(h);
+    return -1;
+  }
+  P.mi_view.potential = nullptr;  // read by steps only once ready
+  h->m = m;
+  h->cfg = *config;
+  h->startup = *startup;
+  std::memcpy(h->active, active, sizeof(h->active));
+  h->ghost = ghost;
+  m->mi = h;
+  *result = reinterpret_cast<gamera_cs_mi_dipole *>(h);
+  return 0;
+}
+
+void gamera_cs_mi_dipole_destroy(gamera_cs_mi_dipole *state) {
+  std::lock_guard<std::mutex> guard(gcs::g_lock);
+  gcs::destroy_handle(gcs::find_mi(state));
+}
+
+int gamera_cs_mi_dipole_create_potential_view(const gamera_cs_earth_startup *startup,
+                                              const gamera_cs_mi_dipole_config *config,
+                                              gamera_cs_mi_dipole **result) {
+  using namespace gcs;
+  if (result == nullptr) return -1;
+  *result = nullptr;
+  MiView v;
+  if (startup == nullptr || !make_view(*startup, *config, v)) return -1;
+  std::lock_guard<std::mutex> guard(g_lock);
+  MiHandle *h = new_handle();
+  if (h == nullptr) return -1;
+  const size_t count = 2 * config->azimuth_points * config->colatitude_points;
+  if (hipMalloc(&h->view_potential, count * sizeof(double)) != hipSuccess ||
+      hipMemset(h->view_potential, 0, count * sizeof(double)) != hipSuccess) {
+    destroy_handle(h);
+    return -1;
+  }
+  h->view = v;
+  h->cfg = *config;
+  h->startup = *startup;
+  *result = reinterpret_cast<gamera_cs_mi_dipole *>(h);
+  return 0;
+}
+
+int gamera_cs_mi_dipole_export_potential(const gamera_cs_mi_dipole *state, double *values,
+                                         size_t count, double *epoch_s,
+                                         double *next_update_s) {
+  using namespace gcs;
+  std::lock_guard<std::mutex> guard(g_lock);
+  MiHandle *h = find_mi(state);
+  if (!handle_ready(h) || values == nullptr || epoch_s == nullptr || next_update_s == nullptr)
+    return -1;
+  const size_t per = h->cfg.azimuth_points * h->cfg.colatitude_points;
+  if (count != 2 * per) return -1;
+  if (h->m != nullptr) {
+    std::vector<double> pot;
+    if (mi_download(*h->m->P, 0, pot) != 0) return -1;
+    std::memcpy(values, pot.data(), count * sizeof(double));
+    MiInfo info;
+    mi_info(*h->m->P, info);
+    *epoch_s = info.epoch_s;
+    *next_update_s = info.next_update_s;
+    return 0;
+  }
+  if (hipMemcpy(values, h->view_potential, count * sizeof(double), hipMemcpyDeviceToHost) !=
+      hipSuccess)
+    return -1;
+  *epoch_s = h->view_epoch;
+  *next_update_s = h->view_next;
+  return 0;
+}
+
+int gamera_cs_mi_dipole_import_potential(gamera_cs_mi_dipole *state, const double *values,
+                                         size_t count, double epoch_s, double next_update_s) {
+  using namespace gcs;
+  std::lock_guard<std::mutex> guard(g_lock);
+  MiHandle *h = find_mi(state);
+  // cubedsphere_mi_dipole.c:138-147
+  if (h == nullptr || values == nullptr || !std::isfinite(epoch_s) ||
+      !std::isfinite(next_update_s) || epoch_s < 0.0 || next_update_s <= epoch_s)
+    return -1;
+  const size_t per = h->cfg.azimuth_points * h->cfg.colatitude_points;
+  if (count != 2 * per) return -1;
+  for (size_t q = 0; q < count; ++q)
+    if (!std::isfinite(values[q])) return -1;
+  if (h->m != nullptr) return mi_import_potential(*h->m->P, values, epoch_s, next_update_s);
+  if (h->view_potential == nullptr ||
+      hipMemcpy(h->view_potential, values, count * sizeof(double), hipMemcpyHostToDevice) !=
+          hipSuccess)
+    return -1;
+  h->view_epoch = epoch_s;
+  h->view_next = next_update_s;
+  h->view_ready = true;
+  return 0;
+}
+
+// cubedsphere_mi_dipole.c:372-378
+int gamera_cs_mi_dipole_update_due(const gamera_cs_mi_dipole *state, double time_code) {
+  using namespace gcs;
+  std::lock_guard<std::mutex> guard(g_lock);
+  const MiHandle *h = find_mi(state);
+  if (h == nullptr || !std::isfinite(time_code) || time_code < 0.0) return -1;
+  const double time_s = time_code * h->cfg.length_norm_m / h->cfg.speed_norm_m_s;
+  if (h->m == nullptr) return !h->view_ready || time_s + 1e-10 >= h->view_next;
+  MiInfo info;
+  mi_info(*h->m->P, info);
+  return !info.ready || time_s + 1e-10 >= info.next_update_s;
+}
+
+// cubedsphere_mi_dipole.c:380-436: the shell current, polar samples and
+// Pedersen solves on the device, from the state the storage holds now.
+int gamera_cs_mi_dipole_maybe_update(gamera_cs_mi_dipole *state, double time_code) {
+  using namespace gcs;
+  const int due = gamera_cs_mi_dipole_update_due(state, time_code);
+  if (due <= 0) return due;
+  std::lock_guard<std::mutex> guard(g_lock);
+  MiHandle *h = find_mi(state);
+  if (h == nullptr || h->m == nullptr) return -1;
+  Mirror *m = h->m;
+  if (!(g_resident && m->device_current)) {
+    if (state_to_device(*m->P, m->storage) != 0) return -1;
+    m->device_current = true;
+  }
+  const int rc = mi_maybe_update(*m->P, time_code);
+  m->P->mi_view.potential = nullptr;  // steps enable it per call
+  return rc;
+}
+
+int gamera_cs_mi_dipole_ready(const gamera_cs_mi_dipole *state) {
+  std::lock_guard<std::mutex> guard(gcs::g_lock);
+  return gcs::handle_ready(gcs::find_mi(state)) ? 1 : 0;
+}
+
+int gamera_cs_mi_dipole_potential(const gamera_cs_mi_dipole *state, gamera_no_vec3 point,
+                                  double *potential_v) {
+  using namespace gcs;
+  if (potential_v == nullptr) return -1;
+  std::lock_guard<std::mutex> guard(g_lock);
+  const double zero[3] = {0.0, 0.0, 0.0};
+  double out[3];
+  if (point_query(find_mi(state), 0, point.value, zero, out) != 0) return -1;
+  *potential_v = out[0];
+  return 0;
+}
+
+int gamera_cs_mi_dipole_ghost_velocity(const gamera_cs_mi_dipole *state, gamera_no_vec3 point,
+                                       gamera_no_vec3 source_velocity,
+                                       gamera_no_vec3 *ghost_velocity) {
+  using namespace gcs;
+  if (ghost_velocity == nullptr) return -1;
+  std::lock_guard<std::mutex> guard(g_lock);
+  double out[3];
+  if (point_query(find_mi(state), 1, point.value, source_velocity.value, out) != 0) return -1;
+  for (int c = 0; c < 3; ++c) ghost_velocity->value[c] = out[c];
+  return 0;
+}
+
+// cubedsphere_mi_dipole.c:478-503 on a face's host arrays: each valid wall
+// edge's EMF from the potential at its two ends, evaluated on the device.
+int gamera_cs_mi_dipole_apply_inner_emf(const gamera_cs_mi_dipole *state,
+                                        gamera_no_storage *storage,
+                                        const gamera_no_grid *grid, size_t wall_i) {
+  using namespace gcs;
+  if (storage == nullptr || grid == nullptr) return -1;
+  std::lock_guard<std::mutex> guard(g_lock);
+  const MiHandle *h = find_mi(state);
+  MiView v;
+  if (!query_view(h, v)) return -1;
+  const double zero[3] = {0.0, 0.0, 0.0};
+  for (int d = 1; d <= 2; ++d) {
+    const gamera_no_edge_grid &eg = grid->edge[d];
+    if (wall_i >= eg.extent[0]) return -1;
+    for (size_t j = 0; j < eg.extent[1]; ++j)
+      for (size_t k = 0; k < eg.extent[2]; ++k) {
+        const size_t edge = (wall_i * eg.extent[1] + j) * eg.extent[2] + k;
+        if (!eg.valid[edge]) continue;
+        size_t end[3] = {wall_i, j, k};
+        ++end[d];
+        const size_t *ve = grid->vertex_extent;
+        const gamera_no_vec3 &a = grid->vertex[(wall_i * ve[1] + j) * ve[2] + k];
+        const gamera_no_vec3 &b = grid->vertex[(end[0] * ve[1] + end[1]) * ve[2] + end[2]];
+        double pa[3], pb[3];
+        if (mi_point(v, 0, a.value, zero, h->query, pa) != 0 ||
+            mi_point(v, 0, b.value, zero, h->query, pb) != 0)
+          return -1;
+        storage->edge_emf[d][edge] = -(pb[0] - pa[0]) / v.potential_norm;
+      }
+  }
+  return 0;
+}
+
+int gamera_cs_mi_dipole_stats(const gamera_cs_mi_dipole *state, double *fac_min,
+                              double *fac_max, double *potential_min_v,
+                              double *potential_max_v, gamera_mi_solver_stats solve[2]) {
+  using namespace gcs;
+  std::lock_guard<std::mutex> guard(g_lock);
+  MiHandle *h = find_mi(state);
+  // cubedsphere_mi_dipole.c:545-569
+  if (!handle_ready(h) || h->m == nullptr || fac_min == nullptr || fac_max == nullptr ||
+      potential_min_v == nullptr || potential_max_v == nullptr || solve == nullptr)
+    return -1;
+  std::vector<double> fac, pot;
+  if (polar_arrays(h, fac, pot) != 0) return -1;
+  *fac_min = INFINITY;
+  *fac_max = -INFINITY;
+  *potential_min_v = INFINITY;
+  *potential_max_v = -INFINITY;
+  for (size_t q = 0; q < fac.size(); ++q) {
+    *fac_min = std::fmin(*fac_min, fac[q]);
+    *fac_max = std::fmax(*fac_max, fac[q]);
+    *potential_min_v = std::fmin(*potential_min_v, pot[q]);
+    *potential_max_v = std::fmax(*potential_max_v, pot[q]);
+  }
+  MiInfo info;
+  mi_info(*h->m->P, info);
+  for (int hemi = 0; hemi < 2; ++hemi)
+    solve[hemi] = gamera_mi_solver_stats{info.iterations[hemi], info.initial_residual[hemi],
+                                         info.final_residual[hemi], info.converged[hemi]};
+  return 0;
+}
+
+int gamera_cs_mi_dipole_analysis_view(const gamera_cs_mi_dipole *state,
+                                      gamera_cs_mi_analysis_view *view) {
+  using namespace gcs;
+  std::lock_guard<std::mutex> guard(g_lock);
+  MiHandle *h = find_mi(state);
+  if (!handle_ready(h) || h->m == nullptr || view == nullptr) return -1;
+  if (polar_arrays(h, h->host_fac, h->host_potential) != 0) return -1;
+  MiInfo info;
+  mi_info(*h->m->P, info);
+  const size_t per = h->cfg.azimuth_points * h->cfg.colatitude_points;
+  view->azimuth_points = h->cfg.azimuth_points;
+  view->colatitude_points = h->cfg.colatitude_points;
+  view->shell_j_cells = h->active[1];
+  view->shell_k_cells = h->active[2];
+  view->maximum_colatitude_rad = info.max_colat;
+  view->epoch_s = info.epoch_s;
+  view->sample_radius_re = info.sample_radius;
+  view->pedersen_siemens = h->cfg.pedersen_siemens;
+  for (int hemi = 0; hemi < 2; ++hemi) {
+    view->fac_parallel_a_m2[hemi] = h->host_fac.data() + hemi * per;
+    view->potential_v[hemi] = h->host_potential.data() + hemi * per;
+  }
+  return 0;
+}
+
+int gamera_cs_mi_dipole_export_shell_current(const gamera_cs_mi_dipole *state,
+                                             double *jparallel_a_m2, double *mlat_deg,
+                                             double *phi_deg, size_t count) {
+  using namespace gcs;
+  std::lock_guard<std::mutex> guard(g_lock);
+  MiHandle *h = find_mi(state);
+  if (!handle_ready(h) || h->m == nullptr || jparallel_a_m2 == nullptr ||
+      mlat_deg == nullptr || phi_deg == nullptr ||
+      count != (size_t)GAMERA_CS_FACE_COUNT * h->active[1] * h->active[2])
+    return -1;
+  std::vector<double> shell;
+  if (mi_shell_current(*h->m->P, shell) != 0) return -1;
+  std::memcpy(jparallel_a_m2, shell.data(), count * sizeof(double));
+  std::memcpy(mlat_deg, shell.data() + count, count * sizeof(double));
+  std::memcpy(phi_deg, shell.data() + 2 * count, count * sizeof(double));
+  return 0;
+}
+
+// cubedsphere_mi_dipole.c:620-648: the reference's CSV format.
+int gamera_cs_mi_dipole_write_csv(const gamera_cs_mi_dipole *state, const char *prefix) {
+  using namespace gcs;
+  std::lock_guard<std::mutex> guard(g_lock);
+  MiHandle *h = find_mi(state);
+  if (!handle_ready(h) || h->m == nullptr || prefix == nullptr || std::strlen(prefix) > 4000U)
+    return -1;
+  std::vector<double> fac, pot;
+  if (polar_arrays(h, fac, pot) != 0) return -1;
+  MiInfo info;
+  mi_info(*h->m->P, info);
+  const size_t np = h->cfg.azimuth_points, nt = h->cfg.colatitude_points;
+  for (int hemi = 0; hemi < 2; ++hemi) {
+    const char *name = hemi == 0 ? "north" : "south";
+    char path[4096];
+    if (std::snprintf(path, sizeof(path), "%s.mi_%s.csv", prefix, name) >= (int)sizeof(path))
+      return -1;
+    FILE *file = std::fopen(path, "w");
+    if (file == nullptr) return -1;
+    std::fputs("time_s,hemisphere,theta_deg,phi_deg,fac_A_m2,potential_V\n", file);
+    for (size_t t = 0; t < nt; ++t)
+      for (size_t p = 0; p < np; ++p) {
+        const size_t q = hemi * np * nt + t * np + p;
+        std::fprintf(file, "%.17g,%s,%.17g,%.17g,%.17g,%.17g\n", info.epoch_s, name,
+                     info.max_colat * (double)t / (double)(nt - 1U) * 180.0 / std::acos(-1.0),
+                     360.0 * (double)p / (double)np, fac[q], pot[q]);
+      }
+    if (std::fclose(file) != 0) return -1;
+  }
+  return 0;
+}
+
+// cubedsphere_mi_dipole.c:650-689
+int gamera_cs_mi_dipole_write_shell_current_csv(const gamera_cs_mi_dipole *state,
+                                                const char *prefix) {
+  using namespace gcs;
+  std::lock_guard<std::mutex> guard(g_lock);
+  MiHandle *h = find_mi(state);
+  if (!handle_ready(h) || h->m == nullptr || prefix == nullptr) return -1;
+  char path[4096];
+  if (std::snprintf(path, sizeof(path), "%s.shell_j.csv", prefix) >= (int)sizeof(path))
+    return -1;
+  std::vector<double> shell;
+  if (mi_shell_current(*h->m->P, shell) != 0) return -1;
+  FILE *file = std::fopen(path, "w");
+  if (file == nullptr) return -1;
+  std::fputs("face,j,k,mlat_deg,phi_deg,jparallel_A_m2\n", file);
+  const size_t nj = h->active[1], nk = h->active[2], per = 6 * nj * nk;
+  for (int face = 0; face < GAMERA_CS_FACE_COUNT; ++face)
+    for (size_t j = 0; j < nj; ++j)
+      for (size_t k = 0; k < nk; ++k) {
+        const size_t q = ((size_t)face * nj + j) * nk + k;
+        std::fprintf(file, "%d,%zu,%zu,%.17g,%.17g,%.17g\n", face, j, k, shell[per + q],
+                     shell[2 * per + q], shell[q]);
+      }
+  return std::fclose(file) == 0 ? 0 : -1;
 }
 
 }  // extern "C"

@@ -66,10 +66,16 @@ struct MiState {
   bool ready = false;
   int updates = 0;
   int iterations[2] = {0, 0};
+  double initial_residual[2] = {0.0, 0.0};
   double final_residual[2] = {0.0, 0.0};
+  int converged[2] = {0, 0};
   double solve_ms = 0.0;
   // device arrays
   double *shell_fac = nullptr;  // [6][S][S]
+  // J.b0/|b0| (A/m^2, before the dipole mapping), magnetic latitude and
+  // longitude (degrees) of the active sample-shell cells at the last update,
+  // [3][6][n][n]
+  double *shell_current = nullptr;
   double *fac = nullptr;        // [2][nt][np]
   double *potential = nullptr;  // [2][nt][np]
   double *work = nullptr;       // [2][kVectors][n] + ring scratch
@@ -189,7 +195,7 @@ GCS_DEV bool face_circulation(const MiK &c, int dir, long at, double &out) {
 }
 
 // FAC of the active shell cells (:277-307).
-__global__ void k_mi_shell_fac(MiK c) {
+__global__ void k_mi_shell_fac(MiK c, double *shell_current) {
   const Layout &L = c.L;
   const int n = L.n;
   const long t = gtid();
@@ -245,6 +251,16 @@ __global__ void k_mi_shell_fac(MiK c) {
     return;
   }
   c.shell_fac[(long)f * c.S * c.S + lj * c.S + lk] = fac;
+  // the shell's own J-parallel, magnetic latitude and longitude
+  // (cubedsphere_mi_dipole.c:590-618)
+  if (shell_current != nullptr) {
+    const long q = ((long)f * n + (lj - 1)) * n + (lk - 1), per = 6L * n * n;
+    double phi = atan2(x[1], x[0]) * 180.0 / kPi;
+    if (phi < 0.0) phi += 360.0;
+    shell_current[q] = jb / bm * c.current_norm;
+    shell_current[per + q] = asin(x[2] / rad) * 180.0 / kPi;
+    shell_current[2 * per + q] = phi;
+  }
 }
 
 // FAC fringe (:308-342) and polar samples (:344-370, :391-406).
@@ -1370,6 +1386,8 @@ int solve(Problem &P, bool tensor) {
     const double *sl = m.host_scal + h * SL_COUNT;
     m.iterations[h] = (int)sl[SL_ITER];
     m.final_residual[h] = sl[SL_NORM];
+    m.initial_residual[h] = sl[SL_FIRST];
+    m.converged[h] = sl[SL_OK] != 0.0 ? 1 : 0;
     ok = ok && sl[SL_OK] != 0.0;
   }
   if (!ok) {
@@ -1525,6 +1543,8 @@ int mi_create(Problem &P) {
     return -1;
   const long polar = 2L * nt * np;
   GCS_HIP(hipMalloc(&m.shell_fac, 6L * S * S * sizeof(double)));
+  GCS_HIP(hipMalloc(&m.shell_current, 18L * L.n * L.n * sizeof(double)));
+  GCS_HIP(hipMemset(m.shell_current, 0, 18L * L.n * L.n * sizeof(double)));
   GCS_HIP(hipMalloc(&m.fac, polar * sizeof(double)));
   GCS_HIP(hipMalloc(&m.potential, polar * sizeof(double)));
   GCS_HIP(hipMalloc(&m.work, 2L * kVectors * m.n * sizeof(double)));
@@ -1576,6 +1596,7 @@ void mi_destroy(Problem &P) {
   MiState *m = P.mi;
   if (m == nullptr) return;
   (void)hipFree(m->shell_fac);
+  (void)hipFree(m->shell_current);
   (void)hipFree(m->fac);
   (void)hipFree(m->potential);
   (void)hipFree(m->work);
@@ -1621,7 +1642,10 @@ int mi_sample_current(Problem &P) {
   const Layout &L = P.L;
   k_mi_copy_shell<<<blocks_of(6L * 3 * L.NJ * L.NK), kT, 0, s>>>(c);
   k_mi_fringe_b<<<blocks_of(m.n_bfringe), kT, 0, s>>>(c, m.bfringe, m.n_bfringe);
-  k_mi_shell_fac<<<blocks_of(6L * L.n * L.n), kT, 0, s>>>(c);
+  k_mi_shell_fac<<<blocks_of(6L * L.n * L.n), kT, 0, s>>>(c, m.shell_current);
+  // the copy above used the face-velocity planes, where a fidelity step's
+  // update leaves the stress rates
+  P.rates_stored = false;
   k_mi_bilinear<<<blocks_of(m.n_ffringe), kT, 0, s>>>(m.ffringe, m.n_ffringe, m.shell_fac,
                                                      m.shell_fac);
   k_mi_bilinear<<<blocks_of(m.n_samples), kT, 0, s>>>(m.samples, m.n_samples, m.shell_fac,
@@ -1648,6 +1672,59 @@ void mi_report(const Problem &P, MiReport &out) {
   }
   out.epoch_s = m.epoch;
   out.solve_ms = m.solve_ms;
+}
+
+void mi_info(const Problem &P, MiInfo &out) {
+  out = MiInfo();
+  if (P.mi == nullptr) return;
+  const MiState &m = *P.mi;
+  out.ready = m.ready;
+  out.updates = m.updates;
+  out.epoch_s = m.epoch;
+  out.next_update_s = m.next_update;
+  out.sample_radius = m.sample_radius;
+  out.max_colat = m.max_colat;
+  out.potential_norm = m.potential_norm;
+  out.potential = m.potential;
+  for (int h = 0; h < 2; ++h) {
+    out.iterations[h] = m.iterations[h];
+    out.initial_residual[h] = m.initial_residual[h];
+    out.final_residual[h] = m.final_residual[h];
+    out.converged[h] = m.converged[h];
+  }
+}
+
+void mi_view_enable(Problem &P, bool on) {
+  P.mi_view.potential = on && P.mi != nullptr && P.mi->ready ? P.mi->potential : nullptr;
+}
+
+int mi_import_potential(Problem &P, const double *values, double epoch_s,
+                        double next_update_s) {
+  if (P.mi == nullptr) {
+    set_error("MI is not enabled");
+    return -1;
+  }
+  MiState &m = *P.mi;
+  GCS_HIP(hipStreamSynchronize(P.stream));
+  GCS_HIP(hipMemcpy(m.potential, values, 2L * m.nt * m.np * sizeof(double),
+                    hipMemcpyHostToDevice));
+  m.epoch = epoch_s;
+  m.next_update = next_update_s;
+  m.ready = true;
+  P.mi_view.potential = m.potential;
+  return 0;
+}
+
+int mi_shell_current(Problem &P, std::vector<double> &out) {
+  if (P.mi == nullptr) {
+    set_error("MI is not enabled");
+    return -1;
+  }
+  out.resize(18L * P.L.n * P.L.n);
+  GCS_HIP(hipStreamSynchronize(P.stream));
+  GCS_HIP(hipMemcpy(out.data(), P.mi->shell_current, out.size() * sizeof(double),
+                    hipMemcpyDeviceToHost));
+  return 0;
 }
 
 int mi_download(Problem &P, int what, std::vector<double> &out) {

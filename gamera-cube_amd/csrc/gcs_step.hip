@@ -598,6 +598,18 @@ GCS_DEV bool mi_ghost_velocity(const MiView &m, const double x[3], const double 
   return true;
 }
 
+// One point query of the potential (what 0) or the MI ghost velocity (what 1)
+// for the drop-in's gamera_cs_mi_dipole_potential / _ghost_velocity;
+// out = {value or velocity, ok}.
+__global__ void k_mi_point(MiView m, int what, double x0, double x1, double x2, double v0,
+                           double v1, double v2, double *out) {
+  const double x[3] = {x0, x1, x2}, v[3] = {v0, v1, v2};
+  double r[3] = {0.0, 0.0, 0.0};
+  const bool ok = what == 0 ? mi_potential(m, x, r[0]) : mi_ghost_velocity(m, x, v, r);
+  for (int c = 0; c < 3; ++c) out[c] = r[c];
+  out[3] = ok ? 1.0 : 0.0;
+}
+
 // ---- Earth radial boundary -------------------------------------------------
 
 // cubedsphere_earth_boundary.c:27-112: radial ghost cells of both levels.
@@ -972,7 +984,9 @@ GCS_DEV void face_flux_face(const K &c, int d, int f, long at, long st, const Ce
   }
   FluidFlux fl;
   fluid_flux(w, o.gamma, fr, fl);
-  // hydrodynamic steps: a zero Maxwell flux (nonorthogonal_sweep.c:463-464)
+  // hydrodynamic steps: a zero Maxwell flux and neither HOGS nor emergency
+  // diffusion, which the reference applies inside its MHD branch
+  // (nonorthogonal_sweep.c:462-527)
   MaxwellFlux mx = {{0.0, 0.0, 0.0}, 0.0, 0.0};
   if (o.mhd) {
     const double bn = c.p(P_BN + d)[at];
@@ -983,8 +997,8 @@ GCS_DEV void face_flux_face(const K &c, int d, int f, long at, long st, const Ce
       return;
     }
   }
-  if (o.hogs) hogs(fl, mx, o.hogs_h, o.hogs_m, o.boris, o.c);
-  if (o.hogs && o.boris) {
+  if (o.mhd && o.hogs) hogs(fl, mx, o.hogs_h, o.hogs_m, o.boris, o.c);
+  if (o.mhd && o.hogs && o.boris) {
     // nonorthogonal_flux.c:206-241
     double s2 = 0.0;
     s2 += dot3(w[0].v, w[0].v);
@@ -1622,8 +1636,8 @@ GCS_DEV void face_flux_face_fp32(const K &c, int d, int f, long at, long st, con
       return;
     }
   }
-  if (o.hogs) hogs_r(fl, mx, (R)o.hogs_h, (R)o.hogs_m, o.boris != 0, cl);
-  if (o.hogs && o.boris) {
+  if (o.mhd && o.hogs) hogs_r(fl, mx, (R)o.hogs_h, (R)o.hogs_m, o.boris != 0, cl);
+  if (o.mhd && o.hogs && o.boris) {
     R s2 = R(0);
     s2 += dot3r(w[0].v, w[0].v);
     s2 += dot3r(w[1].v, w[1].v);
@@ -2747,6 +2761,19 @@ __global__ void k_gates(K c, double gamma, unsigned long long *acc) {
     atomicMin(&acc[0], rb);
     atomicMin(&acc[1], pb);
   }
+}
+
+int mi_point(const MiView &view, int what, const double x[3], const double v[3],
+             double *result, double out[3]) {
+  if (view.potential == nullptr) return -1;
+  GCS_HIP(hipDeviceSynchronize());  // the potential is written on a problem's stream
+  k_mi_point<<<1, 1>>>(view, what, x[0], x[1], x[2], v[0], v[1], v[2], result);
+  GCS_HIP(hipGetLastError());
+  double host[4];
+  GCS_HIP(hipMemcpy(host, result, sizeof(host), hipMemcpyDeviceToHost));
+  if (host[3] == 0.0) return -1;
+  for (int c = 0; c < 3; ++c) out[c] = host[c];
+  return 0;
 }
 
 int final_gates(Problem &P, double gamma, double out[3]) {

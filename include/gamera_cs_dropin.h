@@ -56,7 +56,8 @@ int gamera_cs_advance_six_with_background(
 
 /* cubedsphere/cubedsphere_step.h:58-67 (cubedsphere_step.c:13-154). The
  * halo plan is accepted and not read: the device builds its own plan from
- * the same grid. Requires ghost = 4 and options->update.use_mhd. */
+ * the same grid. Any ghost >= 4; MHD and hydrodynamic
+ * (options->update.use_mhd = false) steps. */
 int gamera_cs_advance_six_with_background_planned(
     const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
     gamera_no_storage storage[GAMERA_CS_FACE_COUNT],
@@ -98,6 +99,11 @@ int gcs_dropin_mark_host_modified(
 int gcs_dropin_release(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
                        gamera_no_storage storage[GAMERA_CS_FACE_COUNT]);
 int gcs_dropin_mirror_count(void);
+
+/* Callbacks that ran on the host (the state downloaded before and uploaded
+ * after) since the library was loaded; zero for a step whose boundary work
+ * all runs on the device. No reference counterpart. */
+long gcs_dropin_host_round_trips(void);
 
 #ifdef __cplusplus
 }

@@ -80,6 +80,10 @@ typedef struct {
   int mi_updates;
   refh_step_fn step;                          /* the six-face step called */
   gamera_cs_radial_boundary_function radial;  /* its radial boundary */
+  /* an ionosphere of another implementation (refh_use_mi): its state and
+   * its gamera_cs_mi_dipole_maybe_update */
+  void *ext_mi;
+  int (*ext_update)(void *mi, double time_code);
 } refh_problem;
 
 /* Wrappers of the reference's Earth callbacks at addresses of their own:
@@ -380,6 +384,26 @@ fail:
   return NULL;
 }
 
+/* Couple the driver loop to another implementation's ionosphere state `mi`
+ * (with the gamera_cs_mi_dipole_* prototypes): the Earth context's mi
+ * becomes it and each step calls `update` in place of the reference's
+ * gamera_cs_mi_dipole_maybe_update. NULL restores the reference's. */
+void refh_use_mi(refh_problem *p, void *mi, int (*update)(void *, double)) {
+  if (p == NULL) return;
+  p->ext_mi = mi;
+  p->ext_update = update;
+  p->boundary.mi = mi != NULL ? (const struct gamera_cs_mi_dipole *)mi : p->mi;
+}
+
+/* The grid's background, the startup and the active box of the problem,
+ * for creating an ionosphere state on it. */
+const gamera_no_background_field *refh_background(const refh_problem *p) {
+  return p->background;
+}
+const gamera_cs_earth_startup *refh_startup(const refh_problem *p) {
+  return &p->cfg.startup;
+}
+
 /* Replace the wind series with the reference's own setter
  * (gamera_solar_wind_set, solar_wind.c:55-80): count samples,
  * state[8 n ..] = density, pressure, velocity, magnetic. */
@@ -439,7 +463,11 @@ static int advance_one(refh_problem *p) {
     if (!(limit > 0.0)) return -1;
     p->dt = fmin(p->cfg.dt_fraction * limit, 1.25 * p->previous_dt);
   }
-  if (p->mi != NULL) {
+  if (p->ext_mi != NULL) {
+    const int updated = p->ext_update(p->ext_mi, p->time);
+    if (updated < 0) return -1;
+    p->mi_updates += updated;
+  } else if (p->mi != NULL) {
     const int updated = gamera_cs_mi_dipole_maybe_update(p->mi, p->time);
     if (updated < 0) return -1;
     p->mi_updates += updated;
@@ -627,6 +655,11 @@ void refh_counters(const refh_problem *p, int face, double out[2]) {
 }
 
 int refh_mi_updates(const refh_problem *p) { return p->mi_updates; }
+
+/* The reference's own ionosphere state of the problem (NULL without MI). */
+gamera_cs_mi_dipole *refh_mi(const refh_problem *p) {
+  return p != NULL ? p->mi : NULL;
+}
 
 /* Both hemispheres' sampled field-aligned current [hemi][colatitude]
  * [azimuth] in A/m^2 (cubedsphere_mi_dipole.h:30-41). */
