@@ -12,9 +12,11 @@ potential at zero. The state is resident in HBM before timing starts.
 Prints one JSON line: cell-updates/s over K driver steps (CFL dt rule + full
 six-face step), the step's fraction of the HBM roofline under the
 1216 B/cell-update model, the dominant kernel's roofline from HIP events on
-the step's stream, and the CPU rate of the oracle (a C restatement of the
-reference step, OpenMP on all host threads) on a two-step sample of the
-same workload, started from the GPU problem's own arrays.
+the step's stream next to a measured copy-kernel bandwidth, the CPU rate of
+the compiled reference (OpenMP on all host threads; --cpu-kind port: the
+oracle) on a two-step sample of the same workload started from the GPU
+problem's own arrays, and the rate of the reference's own driver loop
+calling the drop-in entry points with the state resident on the device.
 """
 import argparse
 import ctypes
@@ -61,6 +63,111 @@ REFERENCE_FLAGS = ("GAMERA_CS_HALO_COMPACT", "GAMERA_CS_HALO_INDEX_CACHE",
                    "GAMERA_NO_EMF_FACE_CACHE", "GAMERA_NO_PRIMITIVE_CACHE")
 
 
+def copy_of(lib, prefix, gpu_problem, cfg, prob):
+    """A problem of `lib` (the compiled reference through its harness,
+    prefix "refh", allocated without the reference's setup; or the oracle,
+    "orc") holding the GPU problem's geometry, background, state and clock,
+    without the MI coupling."""
+    set_clock = getattr(lib, f"{prefix}_set_clock")
+    set_clock.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                          ctypes.c_double, ctypes.c_long]
+    ocfg = type(cfg).from_buffer_copy(cfg)
+    ocfg.mi_enabled = 0
+    if prefix == "refh":
+        o = prob.Problem(lib, prefix, ocfg, create="create_shell")
+    else:
+        o = prob.Problem(lib, prefix, ocfg)
+    o.copy_fields(gpu_problem, prob.GEOMETRY_FIELDS + prob.BACKGROUND_FIELDS
+                  + prob.STATE_FIELDS)
+    for f in range(6):
+        for d in range(3):
+            o.edge_valid(f, d)[:] = gpu_problem.edge_valid(f, d)
+    c = gpu_problem.clock()
+    set_clock(o._handle, c["time"], c["dt"], c["previous_dt"], c["steps"])
+    return o
+
+
+def dropin_resident(gpu_problem, cfg, steps, warmup=3):
+    """The reference's own driver loop (oracle/_ref harness: the serial Earth
+    driver's dt rule and MI cadence) on a copy of the GPU problem, calling
+    libgcs.so's gamera_cs_advance_six_with_background_planned,
+    gamera_no_local_timestep and gamera_cs_mi_dipole_* in resident mode: the
+    state stays on the device, and each step's dt comes from six device
+    reductions read back to the host, as the reference's driver reads its
+    local dt. Timed over `steps` driver steps after `warmup` (mirror build and
+    first upload)."""
+    path = os.path.join(ROOT, "oracle", "_ref", "libgamera_ref.so")
+    if not os.path.exists(path):
+        return None
+    prob = importlib.import_module("gamera-cube_amd.problem")
+    native = importlib.import_module("gamera-cube_amd.native")
+    gcs = native.library()
+    ref = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)  # Earth callbacks found by name
+    vp = ctypes.c_void_p
+    for name in ("refh_grid", "refh_storage", "refh_background", "refh_startup"):
+        getattr(ref, name).restype = vp
+        getattr(ref, name).argtypes = [vp]
+    ref.refh_set_step.argtypes = [vp, vp]
+    ref.refh_set_timestep.argtypes = [vp, vp]
+    ref.refh_use_mi.argtypes = [vp, vp, vp]
+    gcs.gcs_dropin_release.argtypes = [vp, vp]
+    gcs.gcs_dropin_sync_to_host.argtypes = [vp, vp]
+    gcs.gcs_dropin_host_round_trips.restype = ctypes.c_long
+    gcs.gcs_dropin_local_timestep_device_calls.restype = ctypes.c_long
+    gcs.gamera_cs_mi_dipole_create.argtypes = [vp, vp, vp, vp, vp, ctypes.c_size_t, vp,
+                                               ctypes.POINTER(vp)]
+    gcs.gamera_cs_mi_dipole_destroy.argtypes = [vp]
+
+    def addr(name):
+        return ctypes.cast(getattr(gcs, name), vp).value
+
+    o = copy_of(ref, "refh", gpu_problem, cfg, prob)
+    h = vp()
+    grid, storage = ref.refh_grid(o._handle), ref.refh_storage(o._handle)
+    try:
+        ref.refh_set_step(o._handle, addr("gamera_cs_advance_six_with_background_planned"))
+        ref.refh_set_timestep(o._handle, addr("gamera_no_local_timestep"))
+        if cfg.mi_enabled:
+            active = (ctypes.c_size_t * 3)(cfg.radial_cells, cfg.angular_cells,
+                                           cfg.angular_cells)
+            if gcs.gamera_cs_mi_dipole_create(grid, storage, ref.refh_background(o._handle),
+                                              ref.refh_startup(o._handle), active, cfg.ghost,
+                                              ctypes.byref(cfg.mi), ctypes.byref(h)) != 0:
+                return {"error": "gamera_cs_mi_dipole_create failed"}
+            ref.refh_use_mi(o._handle, h, addr("gamera_cs_mi_dipole_maybe_update"))
+        gcs.gcs_dropin_set_resident(1)
+        o.advance(warmup)
+        gcs.gcs_device_synchronize()
+        rounds = gcs.gcs_dropin_host_round_trips()
+        served = gcs.gcs_dropin_local_timestep_device_calls()
+        t0 = time.perf_counter()
+        o.advance(steps)
+        gcs.gcs_device_synchronize()
+        seconds = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        if gcs.gcs_dropin_sync_to_host(grid, storage) != 0:
+            return {"error": "gcs_dropin_sync_to_host failed"}
+        sync_s = time.perf_counter() - t1
+        return {"ms_per_step": seconds * 1e3 / steps,
+                "cell_updates_per_s": cfg.active_cells * steps / seconds,
+                "steps": steps, "warmup": warmup,
+                "dt_reductions_on_device": gcs.gcs_dropin_local_timestep_device_calls() - served,
+                "host_round_trips": gcs.gcs_dropin_host_round_trips() - rounds,
+                "sync_to_host_s": sync_s,
+                "mi": ("libgcs.so's gamera_cs_mi_dipole_* (the reference interface has "
+                       "no Hall term: Pedersen only)") if cfg.mi_enabled else "off",
+                "loop": "the reference's driver loop (harness) calling the drop-in entry "
+                        "points: dt from gamera_no_local_timestep per face, MI cadence, "
+                        "planned six-face step; state resident on the device"}
+    finally:
+        gcs.gcs_dropin_set_resident(0)
+        ref.refh_use_mi(o._handle, None, None)
+        if h.value:
+            gcs.gamera_cs_mi_dipole_destroy(h)
+        gcs.gcs_dropin_release(grid, storage)
+        o.close()
+
+
 def cpu_baseline(gpu_problem, cfg, steps=2, kind="reference"):
     """The CPU step on the same workload, started from the GPU's arrays:
     kind "reference" is the reference library compiled from its sources
@@ -76,24 +183,10 @@ def cpu_baseline(gpu_problem, cfg, steps=2, kind="reference"):
     prob = importlib.import_module("gamera-cube_amd.problem")
     lib = ctypes.CDLL(path)
     prefix = "refh" if kind == "reference" else "orc"
-    set_clock = getattr(lib, f"{prefix}_set_clock")
-    set_clock.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
-                          ctypes.c_double, ctypes.c_long]
-    ocfg = type(cfg).from_buffer_copy(cfg)
-    ocfg.mi_enabled = 0
     if kind == "reference":
         for flag in REFERENCE_FLAGS:
             os.environ[flag] = "1"
-        o = prob.Problem(lib, prefix, ocfg, create="create_shell")
-    else:
-        o = prob.Problem(lib, prefix, ocfg)
-    o.copy_fields(gpu_problem, prob.GEOMETRY_FIELDS + prob.BACKGROUND_FIELDS
-                  + prob.STATE_FIELDS)
-    for f in range(6):
-        for d in range(3):
-            o.edge_valid(f, d)[:] = gpu_problem.edge_valid(f, d)
-    c = gpu_problem.clock()
-    set_clock(o._handle, c["time"], c["dt"], c["previous_dt"], c["steps"])
+    o = copy_of(lib, prefix, gpu_problem, cfg, prob)
     o.advance(1)  # first step builds the halo plans: not timed
     t0 = time.perf_counter()
     o.advance(steps)
@@ -154,23 +247,16 @@ def cpu_only(workload):
 
 def copy_peak(gib=2.0, iters=10):
     """HBM bandwidth of a plain device-to-device copy (read + write bytes per
-    second) on this GPU, beside the 8 TB/s specification."""
-    import torch
-    n = int(gib * (1 << 30)) // 8
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record()
-    for _ in range(iters):
-        b.copy_(a)
-    stop.record()
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop)
-    del a, b
-    torch.cuda.empty_cache()
-    return 2 * 8 * n * iters / (ms * 1e-3) / 1e9
+    second, libgcs.so's copy kernel timed with HIP events) on this GPU, beside
+    the 8 TB/s specification."""
+    native = importlib.import_module("gamera-cube_amd.native")
+    lib = native.library()
+    lib.gcs_copy_bandwidth.argtypes = [ctypes.c_size_t, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double)]
+    out = ctypes.c_double()
+    if lib.gcs_copy_bandwidth(int(gib * (1 << 30)) // 8, iters, ctypes.byref(out)) != 0:
+        raise RuntimeError(f"copy kernel failed: {native.last_error()}")
+    return out.value
 
 
 def active_state(p, cfg, prob):
@@ -376,19 +462,25 @@ def main():
     ap.add_argument("--warm-start", type=int, default=0, metavar="STEPS",
                     help="after the timed steps, train the MI warm-start network for "
                          "STEPS steps and report its iterations/ms against the plain solve")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the drop-in resident leg (the reference's driver loop "
+                         "calling libgcs.so's entry points)")
+    ap.add_argument("--dropin-steps", type=int, default=20,
+                    help="driver steps timed in the drop-in resident leg")
     ap.add_argument("--large-run", type=int, default=0, metavar="STEPS",
                     help="only run config 4 (FP64 and mixed, Bz switch) for STEPS steps")
     args = ap.parse_args()
     if args.cpu_only:
         return cpu_only(args.cpu_only)
     if args.copy_calibration:
-        import torch
-        a = torch.ones(args.copy_calibration, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        for _ in range(3):
-            torch.neg(a, out=b)
-        torch.cuda.synchronize()
+        # libgcs.so's copy kernel (k_copy): one untimed and three timed launches
+        native = importlib.import_module("gamera-cube_amd.native")
+        lib = native.library()
+        lib.gcs_copy_bandwidth.argtypes = [ctypes.c_size_t, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_double)]
+        gbs = ctypes.c_double()
+        if lib.gcs_copy_bandwidth(args.copy_calibration, 3, ctypes.byref(gbs)) != 0:
+            raise RuntimeError(native.last_error())
         print(json.dumps({"copy_calibration_doubles": args.copy_calibration,
                           "bytes_read": 8 * args.copy_calibration,
                           "bytes_written": 8 * args.copy_calibration}))
@@ -514,6 +606,13 @@ def main():
         print("bench: CPU baseline (compiled reference, same workload) ...",
               file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(g, cfg, kind=args.cpu_kind)
+    if not args.no_dropin:
+        print("bench: drop-in resident leg (reference driver loop, libgcs.so entry "
+              "points) ...", file=sys.stderr, flush=True)
+        leg = dropin_resident(g, cfg, args.dropin_steps)
+        if leg is not None and "cell_updates_per_s" in leg:
+            leg["fraction_of_value"] = leg["cell_updates_per_s"] / value
+        result["dropin_resident"] = leg
     g.close()
     print(json.dumps(result))
     return 0

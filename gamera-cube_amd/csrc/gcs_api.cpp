@@ -584,6 +584,18 @@ int gcs_problem_set_wind(Problem *P, size_t count, const double *time, const dou
   return 0;
 }
 
+int gcs_copy_bandwidth(size_t doubles, int iterations, double *gb_per_s) {
+  return gcs::copy_bandwidth(doubles, iterations, gb_per_s);
+}
+
+int gcs_device_synchronize(void) {
+  if (hipDeviceSynchronize() != hipSuccess) {
+    gcs::set_error("device synchronize failed");
+    return -1;
+  }
+  return 0;
+}
+
 int gcs_problem_set_plain_kernels(Problem *P, int on) {
   if (P == nullptr) return -1;
   P->plain = on != 0;

@@ -313,6 +313,29 @@ int max_div_b(Problem &P, double *out);
 // {min density, min pressure, max total div B} of the current state's
 // active cells (the Earth driver's end-of-run gate).
 int final_gates(Problem &P, double gamma, double out[3]);
+// Read + write bytes per second of a plain device copy of `doubles` doubles
+// on the current device, timed with HIP events over `iterations` launches.
+int copy_bandwidth(size_t doubles, int iterations, double *gb_per_s);
+// The options of gamera_no_local_timestep (gamera_no_timestep_options,
+// nonorthogonal_step.h:66-76).
+struct LocalDtOptions {
+  double gamma, density_floor, pressure_floor, cfl, light_speed;
+  int mhd, boris, background;
+};
+// Minimum local CFL dt (nonorthogonal_step.c:449-525) over the cells
+// lo <= (i, j, k) < hi of one cube face, from the current level's device
+// planes; *invalid when a cell's state or dt is invalid (where the
+// reference returns -1). Returns -1 on a device error.
+int local_dt_planes(Problem &P, int face, const int lo[3], const int hi[3],
+                    const LocalDtOptions &o, double *dt, bool *invalid);
+// The same over device copies of arrays in the reference's host layout:
+// u 5 and b, b0 3 values per cell of the extent ext (b, b0 may be null
+// where the options do not read them), len the box cells' three CFL
+// lengths (k fastest); acc is 2 device words of scratch.
+int local_dt_arrays(const double *u, const double *b, const double *b0, const double *len,
+                    const long ext[3], const int lo[3], const int hi[3],
+                    const LocalDtOptions &o, unsigned long long *acc, double *dt,
+                    bool *invalid);
 // Upload a wind series (count samples: time[n], state[8 n .. 8 n + 7] =
 // rho, p, v, B) into the problem's device buffer, growing it when needed,
 // and point P.wind_series at it; the other fields are set by the caller.

@@ -23,11 +23,12 @@
 // Callbacks. The reference's own Earth callbacks
 // (cubedsphere_earth_boundary.h:26-58: gamera_cs_earth_fill_radial_ghosts,
 // gamera_cs_earth_inner_fluid_flux, gamera_cs_earth_no_mi_emf) are
-// recognised by address and run as device kernels. Any other callback, and
-// the Earth ones when their context needs something the device path does
-// not hold (MI coupling, a different background, more wind samples), runs
-// on the host at its place in the step: the state is downloaded, the
-// callback called, and the state uploaded again.
+// recognised by address and run as device kernels, with an ionosphere
+// state of this library's gamera_cs_mi_dipole_create as well. Any other
+// callback, and the Earth ones when their context needs something the
+// device path does not hold (another implementation's MI state, a different
+// background), runs on the host at its place in the step: the state is
+// downloaded, the callback called, and the state uploaded again.
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
@@ -1343,6 +1344,150 @@ int gamera_cs_mi_dipole_write_shell_current_csv(const gamera_cs_mi_dipole *state
                      shell[2 * per + q], shell[q]);
       }
   return std::fclose(file) == 0 ? 0 : -1;
+}
+
+}  // extern "C"
+
+// ---- the driver's local CFL dt (nonorthogonal_step.h:79-84) -------------------------
+
+namespace gcs {
+namespace {
+
+// Device copies of the caller's arrays for gamera_no_local_timestep calls
+// the mirror planes cannot serve, grown as needed.
+struct DtStaging {
+  double *u = nullptr, *b = nullptr, *b0 = nullptr, *len = nullptr;
+  size_t cap[4] = {0, 0, 0, 0};  // doubles
+  unsigned long long *acc = nullptr;
+};
+DtStaging g_dt;
+long g_dt_device_calls = 0;  // served from a resident mirror's planes
+
+int stage(double *&dev, size_t &cap, const void *host, size_t count) {
+  if (count > cap) {
+    (void)hipFree(dev);
+    dev = nullptr;
+    cap = 0;
+    if (hipMalloc(&dev, count * sizeof(double)) != hipSuccess) {
+      dev = nullptr;
+      set_error("local timestep: device allocation failed");
+      return -1;
+    }
+    cap = count;
+  }
+  if (host != nullptr &&
+      hipMemcpy(dev, host, count * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("local timestep: upload failed");
+    return -1;
+  }
+  return 0;
+}
+
+// A resident mirror whose device state is newest and whose face `f` is this
+// grid with these state arrays: its planes hold what the host arrays would.
+Mirror *resident_face(const gamera_no_grid *grid, const double *conserved,
+                      const gamera_no_vec3 *cell_magnetic, const gamera_no_vec3 *background,
+                      const LocalDtOptions &o, int &face) {
+  if (!g_resident) return nullptr;
+  for (Mirror *m : g_mirrors) {
+    if (!m->device_current || m->P == nullptr) continue;
+    for (int f = 0; f < GAMERA_CS_FACE_COUNT; ++f) {
+      if (m->grid + f != grid) continue;
+      const gamera_no_storage &st = m->storage[f];
+      if (st.conserved != conserved) continue;
+      if (o.mhd && st.cell_magnetic != cell_magnetic) continue;
+      if (o.mhd && o.background &&
+          !(m->has_background && m->background[f].cell_magnetic == background))
+        continue;
+      face = f;
+      return m;
+    }
+  }
+  return nullptr;
+}
+
+}  // namespace
+}  // namespace gcs
+
+extern "C" {
+
+// nonorthogonal_step.c:449-525 on the GPU. A face of a resident mirror whose
+// device state is newest is read from the device planes (no transfer);
+// otherwise the caller's arrays are copied to the device and reduced there.
+int gamera_no_local_timestep(const gamera_no_grid *grid, const double *conserved,
+                             const gamera_no_vec3 *cell_magnetic, const size_t active_lower[3],
+                             const size_t active_upper[3],
+                             const gamera_no_timestep_options *options,
+                             const gamera_no_vec3 *background_cell_magnetic,
+                             double *local_dt) {
+  // nonorthogonal_step.c:454-471
+  if (grid == nullptr || conserved == nullptr || options == nullptr || local_dt == nullptr ||
+      !std::isfinite(options->gamma) || options->gamma <= 1.0 ||
+      !std::isfinite(options->density_floor) || options->density_floor <= 0.0 ||
+      !std::isfinite(options->pressure_floor) || options->pressure_floor <= 0.0 ||
+      !std::isfinite(options->cfl) || options->cfl <= 0.0 ||
+      (options->use_mhd && cell_magnetic == nullptr) ||
+      (options->use_background && background_cell_magnetic == nullptr) ||
+      (options->use_boris &&
+       (!std::isfinite(options->light_speed) || options->light_speed <= DBL_MIN)))
+    return -1;
+  if (active_lower == nullptr || active_upper == nullptr) return -1;
+  for (int a = 0; a < 3; ++a)
+    if (active_lower[a] >= active_upper[a] || active_upper[a] > grid->cell_extent[a]) return -1;
+  gcs::LocalDtOptions o{options->gamma,       options->density_floor, options->pressure_floor,
+                        options->cfl,         options->light_speed,   options->use_mhd ? 1 : 0,
+                        options->use_boris ? 1 : 0, options->use_background ? 1 : 0};
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = (int)active_lower[a];
+    hi[a] = (int)active_upper[a];
+  }
+  std::lock_guard<std::mutex> guard(gcs::g_lock);
+  double dt = 0.0;
+  bool invalid = false;
+  int face = 0;
+  if (gcs::Mirror *m =
+          gcs::resident_face(grid, conserved, cell_magnetic, background_cell_magnetic, o, face)) {
+    if (gcs::local_dt_planes(*m->P, face, lo, hi, o, &dt, &invalid) != 0 || invalid) return -1;
+    ++gcs::g_dt_device_calls;
+    *local_dt = dt;
+    return 0;
+  }
+  if (grid->cell == nullptr) return -1;
+  const size_t cells = grid->cell_extent[0] * grid->cell_extent[1] * grid->cell_extent[2];
+  gcs::DtStaging &s = gcs::g_dt;
+  const bool need_b = o.mhd != 0, need_b0 = o.mhd && o.background;
+  std::vector<double> len;
+  len.reserve(3 * (size_t)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]));
+  for (int i = lo[0]; i < hi[0]; ++i)
+    for (int j = lo[1]; j < hi[1]; ++j)
+      for (int k = lo[2]; k < hi[2]; ++k) {
+        const size_t h = ((size_t)i * grid->cell_extent[1] + j) * grid->cell_extent[2] + k;
+        for (int d = 0; d < 3; ++d) len.push_back(grid->cell[h].cfl_length[d]);
+      }
+  if (s.acc == nullptr && hipMalloc(&s.acc, 2 * sizeof(unsigned long long)) != hipSuccess) {
+    s.acc = nullptr;
+    gcs::set_error("local timestep: device allocation failed");
+    return -1;
+  }
+  if (gcs::stage(s.u, s.cap[0], conserved, 5 * cells) != 0 ||
+      (need_b && gcs::stage(s.b, s.cap[1], cell_magnetic, 3 * cells) != 0) ||
+      (need_b0 && gcs::stage(s.b0, s.cap[2], background_cell_magnetic, 3 * cells) != 0) ||
+      gcs::stage(s.len, s.cap[3], len.data(), len.size()) != 0)
+    return -1;
+  const long ext[3] = {(long)grid->cell_extent[0], (long)grid->cell_extent[1],
+                       (long)grid->cell_extent[2]};
+  if (gcs::local_dt_arrays(s.u, need_b ? s.b : nullptr, need_b0 ? s.b0 : nullptr, s.len, ext,
+                           lo, hi, o, s.acc, &dt, &invalid) != 0 ||
+      invalid)
+    return -1;
+  *local_dt = dt;
+  return 0;
+}
+
+long gcs_dropin_local_timestep_device_calls(void) {
+  std::lock_guard<std::mutex> guard(gcs::g_lock);
+  return gcs::g_dt_device_calls;
 }
 
 }  // extern "C"

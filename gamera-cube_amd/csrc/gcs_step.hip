@@ -2800,6 +2800,156 @@ int final_gates(Problem &P, double gamma, double out[3]) {
   return 0;
 }
 
+// ---- gamera_no_local_timestep -----------------------------------------------------
+
+// nonorthogonal_step.c:480-515 for one cell: the CFL dt from the conserved
+// state u, the residual cell B b and background B b0 (read only as the
+// options say) and the three CFL lengths; false where the reference fails.
+GCS_DEV bool local_dt_cell(const double u[5], const double b[3], const double b0[3],
+                           const double len[3], const LocalDtOptions &o, double &dt) {
+  Prim w;
+  if (!cons_to_prim(u, o.gamma, o.density_floor, o.pressure_floor, w)) return false;
+  const double cs = sqrt(o.gamma * w.p / w.rho);
+  double va = 0.0;
+  if (o.mhd) {
+    double m[3];
+    for (int d = 0; d < 3; ++d) m[d] = o.background ? b[d] + b0[d] : b[d];
+    va = sqrt(dot3(m, m)) / sqrt(w.rho);
+    if (o.boris) va = o.light_speed * va / hypot(o.light_speed, va);
+  }
+  const double speed = sqrt(dot3(w.v, w.v)) + hypot(cs, va);
+  double l = len[0];
+  l = fmin(l, len[1]);
+  l = fmin(l, len[2]);
+  dt = o.cfl * l / speed;
+  return isfinite(dt) && dt > 0.0;
+}
+
+// Wavefront minimum of dt (as IEEE bits: every candidate is positive) into
+// acc[0] and any failure into acc[1]. Every lane of the wave calls it.
+GCS_DEV void local_dt_fold(double dt, bool bad, unsigned long long *acc) {
+  unsigned long long bits = __double_as_longlong(dt);
+  int any = bad ? 1 : 0;
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long other = __shfl_down(bits, off, 64);
+    bits = other < bits ? other : bits;
+    any |= __shfl_down(any, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&acc[0], bits);
+    if (any) atomicOr(&acc[1], 1ull);
+  }
+}
+
+__global__ void k_local_dt_init(unsigned long long *acc) {
+  acc[0] = __double_as_longlong(DBL_MAX);
+  acc[1] = 0ull;
+}
+
+// Box cell t -> (i, j, k), k fastest.
+struct DtBox {
+  int lo[3], n[3];
+  GCS_DEV bool cell(long t, int &i, int &j, int &k) const {
+    if (t >= (long)n[0] * n[1] * n[2]) return false;
+    k = lo[2] + (int)(t % n[2]);
+    const long r = t / n[2];
+    j = lo[1] + (int)(r % n[1]);
+    i = lo[0] + (int)(r / n[1]);
+    return true;
+  }
+};
+
+__global__ void __launch_bounds__(256) k_local_dt_planes(K c, int f, DtBox bx, LocalDtOptions o,
+                                                         unsigned long long *acc) {
+  int i, j, k;
+  double dt = DBL_MAX;
+  bool bad = false;
+  if (bx.cell(tid(), i, j, k)) {
+    const long at = c.L.at(f, i, j, k);
+    double u[5], b[3] = {0.0, 0.0, 0.0}, b0[3] = {0.0, 0.0, 0.0}, len[3];
+    for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
+    for (int d = 0; d < 3; ++d) {
+      if (o.mhd) b[d] = c.p(plane_b(c.cur, d))[at];
+      if (o.mhd && o.background) b0[d] = c.p(P_B0C + d)[at];
+      len[d] = c.p(P_CFL + d)[at];
+    }
+    bad = !local_dt_cell(u, b, b0, len, o, dt);
+    if (bad) dt = DBL_MAX;
+  }
+  local_dt_fold(dt, bad, acc);
+}
+
+__global__ void __launch_bounds__(256) k_local_dt_arrays(const double *u, const double *b,
+                                                         const double *b0, const double *len,
+                                                         long ej, long ek, DtBox bx,
+                                                         LocalDtOptions o,
+                                                         unsigned long long *acc) {
+  int i, j, k;
+  const long t = tid();
+  double dt = DBL_MAX;
+  bool bad = false;
+  if (bx.cell(t, i, j, k)) {
+    const long h = ((long)i * ej + j) * ek + k;  // gamera_no_index3
+    double uc[5], bc[3] = {0.0, 0.0, 0.0}, b0c[3] = {0.0, 0.0, 0.0}, lc[3];
+    for (int q = 0; q < 5; ++q) uc[q] = u[5 * h + q];
+    for (int d = 0; d < 3; ++d) {
+      if (o.mhd) bc[d] = b[3 * h + d];
+      if (o.mhd && o.background) b0c[d] = b0[3 * h + d];
+      lc[d] = len[3 * t + d];
+    }
+    bad = !local_dt_cell(uc, bc, b0c, lc, o, dt);
+    if (bad) dt = DBL_MAX;
+  }
+  local_dt_fold(dt, bad, acc);
+}
+
+namespace {
+int finish_local_dt(unsigned long long *acc, hipStream_t s, double *dt, bool *invalid) {
+  unsigned long long host[2] = {0, 0};
+  const hipError_t e = hipGetLastError();
+  const hipError_t m = hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, s);
+  const hipError_t w = hipStreamSynchronize(s);
+  if (e != hipSuccess || m != hipSuccess || w != hipSuccess) {
+    set_error("local timestep: device error");
+    return -1;
+  }
+  std::memcpy(dt, &host[0], sizeof(double));
+  *invalid = host[1] != 0;
+  return 0;
+}
+
+DtBox dt_box(const int lo[3], const int hi[3]) {
+  DtBox bx;
+  for (int a = 0; a < 3; ++a) {
+    bx.lo[a] = lo[a];
+    bx.n[a] = hi[a] - lo[a];
+  }
+  return bx;
+}
+}  // namespace
+
+int local_dt_planes(Problem &P, int face, const int lo[3], const int hi[3],
+                    const LocalDtOptions &o, double *dt, bool *invalid) {
+  const DtBox bx = dt_box(lo, hi);
+  const long n = (long)bx.n[0] * bx.n[1] * bx.n[2];
+  unsigned long long *acc = P.acc4;
+  k_local_dt_init<<<1, 1, 0, P.stream>>>(acc);
+  k_local_dt_planes<<<nblocks(n, 256), 256, 0, P.stream>>>(make_k(P), face, bx, o, acc);
+  return finish_local_dt(acc, P.stream, dt, invalid);
+}
+
+int local_dt_arrays(const double *u, const double *b, const double *b0, const double *len,
+                    const long ext[3], const int lo[3], const int hi[3],
+                    const LocalDtOptions &o, unsigned long long *acc, double *dt,
+                    bool *invalid) {
+  const DtBox bx = dt_box(lo, hi);
+  const long n = (long)bx.n[0] * bx.n[1] * bx.n[2];
+  k_local_dt_init<<<1, 1, 0, nullptr>>>(acc);
+  k_local_dt_arrays<<<nblocks(n, 256), 256, 0, nullptr>>>(u, b, b0, len, ext[1], ext[2], bx, o,
+                                                          acc);
+  return finish_local_dt(acc, nullptr, dt, invalid);
+}
+
 int max_div_b(Problem &P, double *out) {
   unsigned long long *dev = P.acc4, host = 0;
   k_zero4<<<1, 4, 0, P.stream>>>(dev);
@@ -2848,6 +2998,67 @@ int enqueue_step(Problem &P, bool explicit_dt) {
   mark_stage(P, S_END);
   GCS_HIP(hipGetLastError());
   return 0;
+}
+
+}  // namespace gcs
+
+// ---- measurement: a plain copy kernel ----------------------------------------------
+
+namespace gcs {
+
+// Each thread copies four 16-byte elements 256 apart (every wave's accesses
+// contiguous), all four loads issued before the first store.
+constexpr int kCopyUnroll = 4;
+
+__global__ void __launch_bounds__(256) k_copy(const double2 *__restrict__ a,
+                                              double2 *__restrict__ b, long n) {
+  const long base = (long)blockIdx.x * (256 * kCopyUnroll) + threadIdx.x;
+  double2 v[kCopyUnroll];
+#pragma unroll
+  for (int r = 0; r < kCopyUnroll; ++r) {
+    const long t = base + 256L * r;
+    if (t < n) v[r] = a[t];
+  }
+#pragma unroll
+  for (int r = 0; r < kCopyUnroll; ++r) {
+    const long t = base + 256L * r;
+    if (t < n) b[t] = v[r];
+  }
+}
+
+int copy_bandwidth(size_t doubles, int iterations, double *gb_per_s) {
+  if (doubles < 2 || iterations < 1 || gb_per_s == nullptr) {
+    set_error("copy bandwidth: need at least 2 doubles and 1 iteration");
+    return -1;
+  }
+  const long n2 = (long)(doubles / 2);
+  double2 *a = nullptr, *b = nullptr;
+  hipEvent_t start = nullptr, stop = nullptr;
+  int status = -1;
+  float ms = 0.0f;
+  if (hipMalloc(&a, n2 * sizeof(double2)) != hipSuccess ||
+      hipMalloc(&b, n2 * sizeof(double2)) != hipSuccess ||
+      hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) {
+    set_error("copy bandwidth: allocation failed");
+  } else {
+    const unsigned blocks = (unsigned)((n2 + 256 * kCopyUnroll - 1) / (256 * kCopyUnroll));
+    k_copy<<<blocks, 256>>>(a, b, n2);  // first touch, not timed
+    (void)hipEventRecord(start, nullptr);
+    for (int r = 0; r < iterations; ++r) k_copy<<<blocks, 256>>>(a, b, n2);
+    (void)hipEventRecord(stop, nullptr);
+    if (hipGetLastError() == hipSuccess && hipEventSynchronize(stop) == hipSuccess &&
+        hipEventElapsedTime(&ms, start, stop) == hipSuccess && ms > 0.0f) {
+      *gb_per_s = 2.0 * sizeof(double2) * (double)n2 * iterations / (ms * 1e-3) / 1e9;
+      status = 0;
+    } else {
+      set_error("copy bandwidth: device error");
+    }
+  }
+  if (start) (void)hipEventDestroy(start);
+  if (stop) (void)hipEventDestroy(stop);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return status;
 }
 
 }  // namespace gcs

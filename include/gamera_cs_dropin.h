@@ -69,6 +69,20 @@ int gamera_cs_advance_six_with_background_planned(
     gamera_cs_radial_boundary_function radial_boundary,
     double current_time, double previous_time, void *boundary_context);
 
+/* NO/nonorthogonal_step.h:79-84 (nonorthogonal_step.c:449-525): the local
+ * CFL dt of one face's box, computed on the GPU. A face of a resident
+ * mirror whose device state is newest (gcs_dropin_set_resident) is read
+ * from the device, with no transfer; otherwise the caller's arrays are
+ * copied to the device first. Same validation and return values. A program
+ * that also links the reference's nonorthogonal_step.c keeps the
+ * reference's definition; see the integration notes for taking this one. */
+int gamera_no_local_timestep(const gamera_no_grid *grid, const double *conserved,
+                             const gamera_no_vec3 *cell_magnetic,
+                             const size_t active_lower[3], const size_t active_upper[3],
+                             const gamera_no_timestep_options *options,
+                             const gamera_no_vec3 *background_cell_magnetic,
+                             double *local_dt);
+
 /* ---- mirror control (no reference counterpart) ------------------------- */
 
 /* Callbacks that run as device kernels. By default libgcs.so looks up the
@@ -104,6 +118,11 @@ int gcs_dropin_mirror_count(void);
  * after) since the library was loaded; zero for a step whose boundary work
  * all runs on the device. No reference counterpart. */
 long gcs_dropin_host_round_trips(void);
+
+/* gamera_no_local_timestep calls served from a resident mirror's device
+ * state (no transfer) since the library was loaded. No reference
+ * counterpart. */
+long gcs_dropin_local_timestep_device_calls(void);
 
 #ifdef __cplusplus
 }

@@ -168,6 +168,13 @@ int gcs_problem_time_steps(gcs_problem *problem, long steps, int profile,
 int gcs_stage_count(void);
 const char *gcs_stage_name(int stage);
 
+/* Measurement helpers, no reference counterpart: the HBM bandwidth of a
+ * plain device-to-device copy of `doubles` doubles on the current device
+ * (read + write bytes per second over `iterations` launches, HIP events),
+ * and a wait for all work queued on the current device. */
+int gcs_copy_bandwidth(size_t doubles, int iterations, double *gb_per_s);
+int gcs_device_synchronize(void);
+
 /* Ionosphere coupling of a problem created with mi_enabled (the driver's
  * gamera_cs_mi_dipole_*, cubedsphere_mi_dipole.h:51-98): ints = {updates,
  * ready, solver iterations north, south}; doubles = {final residual north,

@@ -62,6 +62,13 @@ typedef int (*refh_step_fn)(
     gamera_cs_radial_boundary_function radial_boundary, double current_time,
     double previous_time, void *boundary_context);
 
+/* The prototype of gamera_no_local_timestep (nonorthogonal_step.h:79-84). */
+typedef int (*refh_timestep_fn)(
+    const gamera_no_grid *grid, const double *conserved,
+    const gamera_no_vec3 *cell_magnetic, const size_t active_lower[3],
+    const size_t active_upper[3], const gamera_no_timestep_options *options,
+    const gamera_no_vec3 *background_cell_magnetic, double *local_dt);
+
 typedef struct {
   refh_config cfg;
   size_t active[3], ghost, extent[3];
@@ -72,13 +79,14 @@ typedef struct {
   gamera_solar_wind_series wind;
   gamera_cs_earth_boundary boundary;
   gamera_no_advance_options options;
-  gamera_no_timestep_options timestep;
+  gamera_no_timestep_options timestep_options;
   gamera_cs_halo_plan *halo_plan;
   gamera_cs_mi_dipole *mi;
   double time, previous_dt, dt;
   long steps;
   int mi_updates;
   refh_step_fn step;                          /* the six-face step called */
+  refh_timestep_fn timestep;                  /* the local CFL dt called */
   gamera_cs_radial_boundary_function radial;  /* its radial boundary */
   /* an ionosphere of another implementation (refh_use_mi): its state and
    * its gamera_cs_mi_dipole_maybe_update */
@@ -155,10 +163,11 @@ static double cfl_limit(refh_problem *p) {
   double limit = INFINITY;
   for (int face = 0; face < GAMERA_CS_FACE_COUNT; ++face) {
     double local;
-    if (gamera_no_local_timestep(&p->grid[face], p->storage[face].conserved,
-                                 p->storage[face].cell_magnetic, lower, upper,
-                                 &p->timestep, p->owned[face].cell_magnetic,
-                                 &local) != 0)
+    const refh_timestep_fn fn =
+        p->timestep != NULL ? p->timestep : gamera_no_local_timestep;
+    if (fn(&p->grid[face], p->storage[face].conserved,
+           p->storage[face].cell_magnetic, lower, upper, &p->timestep_options,
+           p->owned[face].cell_magnetic, &local) != 0)
       return NAN;
     limit = fmin(limit, local);
   }
@@ -232,7 +241,7 @@ refh_problem *refh_create(const refh_config *cfg) {
       goto fail;
   }
 
-  p->timestep = (gamera_no_timestep_options){
+  p->timestep_options = (gamera_no_timestep_options){
       gamma, 1e-6, 1e-8, cfg->cfl, 10.0, true, cfg->use_boris != 0, true};
   p->options = (gamera_no_advance_options){
       .stress = {gamma, 1e-6, 1e-8, 1.0, 0.25, 0.25, 10.0, true, true,
@@ -349,7 +358,7 @@ refh_problem *refh_create_shell(const refh_config *cfg) {
     if (!ok || gamera_no_storage_create(p->extent, &p->storage[face]) != 0) goto fail;
     p->background[face] = b->field;
   }
-  p->timestep = (gamera_no_timestep_options){
+  p->timestep_options = (gamera_no_timestep_options){
       gamma, 1e-6, 1e-8, cfg->cfl, 10.0, true, cfg->use_boris != 0, true};
   p->options = (gamera_no_advance_options){
       .stress = {gamma, 1e-6, 1e-8, 1.0, 0.25, 0.25, 10.0, true, true,
@@ -434,7 +443,7 @@ void refh_set_mhd(refh_problem *p, int on) {
   if (p == NULL) return;
   p->options.stress.use_mhd = on != 0;
   p->options.update.use_mhd = on != 0;
-  p->timestep.use_mhd = on != 0;
+  p->timestep_options.use_mhd = on != 0;
 }
 
 /* Clock of the driver loop (time, dt of the last step, previous dt, steps),
@@ -510,6 +519,24 @@ void refh_set_step(refh_problem *p, void *step) {
   if (p == NULL) return;
   p->step = step != NULL ? (refh_step_fn)step
                          : gamera_cs_advance_six_with_background_planned;
+}
+
+/* One call of a local CFL dt function with gamera_no_local_timestep's
+ * prototype (NULL: the reference's) on face `face`'s arrays. */
+int refh_local_timestep(refh_problem *p, void *fn, int face, const size_t lower[3],
+                        const size_t upper[3], const gamera_no_timestep_options *options,
+                        double *dt) {
+  if (p == NULL || face < 0 || face >= GAMERA_CS_FACE_COUNT) return -2;
+  const refh_timestep_fn call = fn != NULL ? (refh_timestep_fn)fn : gamera_no_local_timestep;
+  return call(&p->grid[face], p->storage[face].conserved, p->storage[face].cell_magnetic,
+              lower, upper, options, p->owned[face].cell_magnetic, dt);
+}
+
+/* The local CFL dt function of the driver's dt rule (the prototype of
+ * gamera_no_local_timestep); NULL restores the reference's. */
+void refh_set_timestep(refh_problem *p, void *timestep) {
+  if (p == NULL) return;
+  p->timestep = (refh_timestep_fn)timestep;
 }
 
 /* 1: pass the Earth callbacks through wrappers (see wrapped_radial). */

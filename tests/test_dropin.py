@@ -235,3 +235,66 @@ def test_invalid_arguments_are_rejected_before_writing(ref_global, gpu):
     for f in range(6):
         assert (b.field(f, problem.CONSERVED) == before[f]).all()
     assert lib.gcs_dropin_mirror_count() == 0
+
+
+class TimestepOptions(ctypes.Structure):
+    """gamera_no_timestep_options (nonorthogonal_step.h:66-76)."""
+    _fields_ = [("gamma", ctypes.c_double), ("density_floor", ctypes.c_double),
+                ("pressure_floor", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("light_speed", ctypes.c_double), ("use_mhd", ctypes.c_bool),
+                ("use_boris", ctypes.c_bool), ("use_background", ctypes.c_bool)]
+
+
+def test_local_timestep_matches_reference(ref_global, gpu):
+    """gamera_no_local_timestep (nonorthogonal_step.c:449-525) of libgcs.so
+    against the reference's on every face of a stepped problem, for MHD and
+    hydrodynamic options, with and without Boris and background, on the
+    active box and a ragged sub-box; invalid options, boxes and states give
+    -1 from both."""
+    lib = gpu.library()
+    cfg = gcs_config.workload("small")
+    a = problem.Problem(ref_global, "refh", cfg)
+    a.advance(3)
+    fn = ref_global.refh_local_timestep
+    fn.restype = ctypes.c_int
+    size3 = ctypes.c_size_t * 3
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, size3, size3,
+                   ctypes.POINTER(TimestepOptions), ctypes.POINTER(ctypes.c_double)]
+    ours = address(lib, "gamera_no_local_timestep")
+    g = cfg.ghost
+    active = (size3(g, g, g), size3(g + cfg.radial_cells, g + cfg.angular_cells,
+                                    g + cfg.angular_cells))
+    ragged = (size3(g + 1, g + 3, g), size3(g + cfg.radial_cells - 2, g + 5,
+                                            g + cfg.angular_cells - 1))
+    extent = [cfg.radial_cells + 2 * g, cfg.angular_cells + 2 * g, cfg.angular_cells + 2 * g]
+
+    def both(face, box, opts):
+        da, db = ctypes.c_double(-1.0), ctypes.c_double(-1.0)
+        ra = fn(a._handle, None, face, box[0], box[1], ctypes.byref(opts), ctypes.byref(da))
+        rb = fn(a._handle, ours, face, box[0], box[1], ctypes.byref(opts), ctypes.byref(db))
+        return ra, rb, da.value, db.value
+
+    worst_rel = 0.0
+    for mhd, boris, bg in [(True, True, True), (True, False, True), (True, True, False),
+                           (False, False, False)]:
+        opts = TimestepOptions(5.0 / 3.0, 1e-6, 1e-8, cfg.cfl, 10.0, mhd, boris, bg)
+        for face in range(6):
+            for box in (active, ragged):
+                ra, rb, da, db = both(face, box, opts)
+                assert ra == rb == 0, (mhd, boris, bg, face)
+                worst_rel = max(worst_rel, abs(da - db) / da)
+    print(f"local timestep: worst relative difference {worst_rel:.2e}", flush=True)
+    assert worst_rel <= 1e-14
+    # nonorthogonal_step.c:454-471: options and boxes rejected by both
+    bad = TimestepOptions(5.0 / 3.0, 1e-6, 1e-8, 0.0, 10.0, True, True, True)
+    assert both(0, active, bad)[:2] == (-1, -1)
+    empty = (size3(g, g, g), size3(g, g + 4, g + 4))
+    good = TimestepOptions(5.0 / 3.0, 1e-6, 1e-8, cfg.cfl, 10.0, True, True, True)
+    assert both(0, empty, good)[:2] == (-1, -1)
+    beyond = (size3(g, g, g), size3(extent[0] + 1, extent[1], extent[2]))
+    assert both(0, beyond, good)[:2] == (-1, -1)
+    # an invalid state inside the box
+    u = a.field(5, problem.CONSERVED).reshape(-1, 5)
+    u[(g * extent[1] + g) * extent[2] + g, 0] = -1.0
+    assert both(5, active, good)[:2] == (-1, -1)
+    assert lib.gcs_dropin_mirror_count() == 0

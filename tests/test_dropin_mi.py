@@ -208,3 +208,49 @@ def test_point_queries_match_the_reference_on_one_potential(libs):
             assert (w != 0.0).any() and np.abs(w - g).max() <= 1e-12 * np.abs(w).max()
     finally:
         gcs.gamera_cs_mi_dipole_destroy(view)
+
+
+def test_resident_driver_loop_moves_no_state(libs):
+    """The reference's driver loop with libgcs.so's step, ionosphere and
+    local CFL dt in resident mode: 20 steps with no host round trip, every
+    dt after the first served from the device state, and the state synced
+    once at the end equal to the reference's."""
+    ref, gcs = libs
+    cfg = mi_config(gcs_config.workload("small_mi"))
+    a = problem.Problem(ref, "refh", cfg)
+    b = problem.Problem(ref, "refh", cfg)
+    ref.refh_set_step(b._handle, ctypes.cast(gcs.gamera_cs_advance_six_with_background_planned,
+                                             VP).value)
+    ref.refh_set_timestep.argtypes = [VP, VP]
+    ref.refh_set_timestep(b._handle, ctypes.cast(gcs.gamera_no_local_timestep, VP).value)
+    h = create_gcs_mi(ref, gcs, b, cfg)
+    ref.refh_use_mi(b._handle, h, ctypes.cast(gcs.gamera_cs_mi_dipole_maybe_update, VP).value)
+    gcs.gcs_dropin_local_timestep_device_calls.restype = ctypes.c_long
+    gcs.gcs_dropin_sync_to_host.argtypes = [VP, VP]
+    rounds = gcs.gcs_dropin_host_round_trips()
+    served = gcs.gcs_dropin_local_timestep_device_calls()
+    steps = 20
+    assert gcs.gcs_dropin_set_resident(1) == 0
+    try:
+        for s in range(steps):
+            a.advance(1)
+            b.advance(1)
+            ca, cb = a.clock(), b.clock()
+            assert abs(ca["dt"] - cb["dt"]) <= STEP_TOL * (s + 1) * ca["dt"], s
+        assert gcs.gcs_dropin_sync_to_host(ref.refh_grid(b._handle),
+                                           ref.refh_storage(b._handle)) == 0
+        worst = max(scaled_error(a.field(f, fid), b.field(f, fid))
+                    for fid in problem.STATE_FIELDS for f in range(gcs_config.FACE_COUNT))
+        rel = max(state_relative_errors(a, b).values())
+        served = gcs.gcs_dropin_local_timestep_device_calls() - served
+        print(f"resident loop: {steps} steps scaled {worst:.2e} relative {rel:.2e}; "
+              f"dt calls served on the device {served}; host round trips "
+              f"{gcs.gcs_dropin_host_round_trips() - rounds}", flush=True)
+        assert worst <= STEP_TOL * steps
+        assert served >= 6 * (steps - 1)
+        assert gcs.gcs_dropin_host_round_trips() == rounds
+    finally:
+        gcs.gcs_dropin_set_resident(0)
+        ref.refh_use_mi(b._handle, None, None)
+        gcs.gamera_cs_mi_dipole_destroy(h)
+        gcs.gcs_dropin_release(ref.refh_grid(b._handle), ref.refh_storage(b._handle))
