@@ -55,6 +55,15 @@ STAGE_MODEL = {
 }
 
 
+# The FP64 kernel of each modelled stage (rocprofv3's names, templates kept).
+STAGE_KERNEL = {
+    "face_flux_i": "k_face_flux_tiled<0, 8, 16>", "face_flux_j": "k_face_flux_tiled<1, 8, 16>",
+    "face_flux_k": "k_face_flux", "edge_emf_i": "k_edge_emf_tiled<0>",
+    "edge_emf_j": "k_edge_emf_tiled<1>", "edge_emf_k": "k_edge_emf",
+    "update": "k_update<false>", "predict": "k_predict<false>",
+}
+
+
 # The reference's production switches (scripts/run_earth.py:18-20): result-
 # preserving caches and the parallel planned halo fill, read with getenv by
 # the reference library while it steps.
@@ -424,7 +433,7 @@ def measured_traffic(workload, stage):
     FETCH_SIZE/WRITE_SIZE summary of this workload (calibrated against a copy
     kernel), or None when there is none."""
     path = os.path.join(ROOT, "profiles", f"step_{workload}_hbm_traffic.json")
-    kernel = "k_" + stage.rsplit("_", 1)[0] if stage.endswith(("_i", "_j", "_k")) else "k_" + stage
+    kernel = STAGE_KERNEL.get(stage)
     try:
         with open(path) as f:
             entry = json.load(f)["kernels"][kernel]
