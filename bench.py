@@ -428,6 +428,91 @@ def warm_start_report(g, cfg, steps):
     return out
 
 
+def logged_warm_start(runs, cfgm, native, held_out=2, samples=40, every=36, lag=10,
+                      train_steps=2000):
+    """The learned warm start trained on in-run data: `runs` config-1 MHD runs
+    (6 x 96^2 x 192, MI on its 2 x 128 x 256 grids, Pedersen 10 S, Hall 5 S)
+    over seeded solar-wind variations (warm_start.WIND_RANGES) log the
+    field-aligned current every `every` steps (gcs_problem_mi_sample_current);
+    the plain FP64 Krylov solve to 1e-10 gives each current's potential.
+    Networks train on the samples and on their changes over `lag` samples
+    (about the 10 s MI cadence), then `held_out` further runs compare, for
+    each current and the potential `lag` samples earlier: the plain solve
+    from that previous potential against the solve from the network's
+    correction of it."""
+    import numpy as np
+    ws = importlib.import_module("gamera-cube_amd.warm_start")
+    data, winds, target_iters = [], [], []
+    keep = None
+    t0 = time.perf_counter()
+    for r in range(runs + held_out):
+        wind = ws.wind_variation(1000 + r)
+        cfg = cfgm.earth_problem(192, 96, perturbation=0.01, seed=1 + r, mi=True,
+                                 hall_siemens=5.0, **wind)
+        g = native.GpuProblem(cfg)
+        g.set_fidelity(False)
+        g.set_graph(True)
+        cur, times = ws.log_currents(g, samples, every)
+        pot, iters = ws.solve_targets(g, cur)
+        data.append((cur, pot, times))
+        winds.append(wind)
+        target_iters += iters
+        if keep is not None:
+            keep.close()
+        keep = g
+        print(f"bench: logged run {r + 1}/{runs + held_out} ({wind}), "
+              f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+    log_s = time.perf_counter() - t0
+    train = [ws.hemisphere_pairs(c, p, lag=l) for c, p, _ in data[:runs] for l in (0, lag)]
+    cur_t = np.concatenate([t[0] for t in train])
+    pot_t = np.concatenate([t[1] for t in train])
+    hemi_t = np.concatenate([t[2] for t in train])
+    nt, np_ = keep.config.mi.colatitude_points, keep.config.mi.azimuth_points
+    out = {"runs": runs, "held_out_runs": held_out, "samples_per_run": samples,
+           "steps_between_samples": every, "lag_samples": lag,
+           "lag_seconds": float(np.mean([(t[lag:] - t[:-lag]).mean() for _, _, t in data])
+                                * cfgm.TIME_NORM_S),
+           "winds": winds, "training_pairs": int(cur_t.shape[0]), "logging_s": log_s,
+           "target_solve_iterations_mean": float(np.mean(target_iters)), "networks": {}}
+    for label, pool, steps in (("pooled_4x4", 4, train_steps), ("full_grid", 1, train_steps // 2)):
+        net = ws.WarmStartNet(nt // pool, np_ // pool, 8, seed=3)
+        t1 = time.perf_counter()
+        losses = ws.train_on(keep, net, cur_t, pot_t, hemi_t, steps=steps, batch=8,
+                             learning_rate=1e-3, seed=5)
+        train_s = time.perf_counter() - t1
+        ws.attach(keep, net)
+        plain_it, plain_ms, learn_it, learn_ms, diff = [], [], [], [], 0.0
+        for cur, pot, _ in data[runs:]:
+            for k in range(lag, samples):
+                p, it_p, ms_p = ws.solve_current(keep, cur[k], pot[k - lag], use_network=False)
+                q, it_l, ms_l = ws.solve_current(keep, cur[k], pot[k - lag], use_network=True)
+                plain_it.append(sum(it_p) / 2)
+                learn_it.append(sum(it_l) / 2)
+                plain_ms.append(ms_p)
+                learn_ms.append(ms_l)
+                diff = max(diff, float(np.abs(q - p).max() / max(np.abs(p).max(), 1e-300)))
+        ws.attach(keep, None)
+        net.close()
+        out["networks"][label] = {
+            "grid": f"2 x {nt // pool} x {np_ // pool}", "train_steps": steps,
+            "train_s": train_s, "relative_loss_first": float(losses[:10].mean()),
+            "relative_loss_last": float(losses[-50:].mean()),
+            "held_out_solves": len(plain_it),
+            "previous_potential_iterations_mean": float(np.mean(plain_it)),
+            "previous_potential_ms_mean": float(np.mean(plain_ms)),
+            "learned_iterations_mean": float(np.mean(learn_it)),
+            "learned_ms_mean": float(np.mean(learn_ms)),
+            "learned_fewer_iterations_fraction": float(np.mean(np.array(learn_it) <
+                                                               np.array(plain_it))),
+            "max_potential_difference_rel": diff}
+        print(f"bench: {label}: iterations {np.mean(plain_it):.1f} -> {np.mean(learn_it):.1f}, "
+              f"ms {np.mean(plain_ms):.1f} -> {np.mean(learn_ms):.1f}", file=sys.stderr,
+              flush=True)
+    keep.close()
+    print(json.dumps({"logged_warm_start": out}))
+    return 0
+
+
 def measured_traffic(workload, stage):
     """HBM bytes per launch of the stage's kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary of this workload (calibrated against a copy
@@ -476,6 +561,10 @@ def main():
                          "calling libgcs.so's entry points)")
     ap.add_argument("--dropin-steps", type=int, default=20,
                     help="driver steps timed in the drop-in resident leg")
+    ap.add_argument("--warm-start-logged", type=int, default=0, metavar="RUNS",
+                    help="only train the MI warm start on currents logged from RUNS config-1 "
+                         "MHD runs over seeded solar-wind variations and compare it with the "
+                         "previous-potential start on two held-out runs")
     ap.add_argument("--large-run", type=int, default=0, metavar="STEPS",
                     help="only run config 4 (FP64 and mixed, Bz switch) for STEPS steps")
     args = ap.parse_args()
@@ -504,6 +593,8 @@ def main():
         return mixed_error(args, cfgm, native, importlib.import_module("gamera-cube_amd.problem"))
     if args.large_run:
         return large_run(args, cfgm, native, importlib.import_module("gamera-cube_amd.problem"))
+    if args.warm_start_logged:
+        return logged_warm_start(args.warm_start_logged, cfgm, native)
     cfg = cfgm.workload(args.workload)
     print(f"bench: workload {args.workload} {cfg.active} x 6, setup ...",
           file=sys.stderr, flush=True)

@@ -1990,6 +1990,13 @@ int mi_train_warm_start(Problem &P, unet::Net &n, const gcs_warm_train_config &t
   const dim3 g(blocks_of((long)n.H * n.W), t.batch);
   int rc = 0;
   for (long step = 0; step < t.steps && rc == 0; ++step) {
+    // the previous step's copy of the pinned samples must finish before
+    // they are redrawn
+    if (step > 0 && loss_history == nullptr && hipStreamSynchronize(s) != hipSuccess) {
+      set_error("warm-start training: stream error");
+      rc = -1;
+      break;
+    }
     draw_samples(rng, t.batch, m.max_colat, t.pedersen_min, t.pedersen_max, t.hall_min,
                  t.hall_max, pb.host);
     if (make_pairs(pb, m.nt, m.np, m.max_colat, r2, s) != 0) {
@@ -2040,6 +2047,131 @@ int mi_train_warm_start(Problem &P, unet::Net &n, const gcs_warm_train_config &t
     set_error("warm-start training: stream error");
     rc = -1;
   }
+  return rc;
+}
+
+// Training on given pairs (for instance currents logged from MHD runs and
+// the potentials the plain solve gives for them): the same loss, Adam
+// steps and learning-rate schedule as mi_train_warm_start, each batch drawn
+// uniformly (seeded) from the `count` pairs, which stay on the device. The
+// conductances are the problem's (Pedersen of its MI configuration, its
+// Hall conductance).
+int mi_train_warm_start_on(Problem &P, unet::Net &n, const gcs_warm_train_config &t,
+                           const double *fac, const double *phi, const int *hemi, int count,
+                           float *loss_history) {
+  if (P.mi == nullptr) {
+    set_error("MI is not enabled");
+    return -1;
+  }
+  const MiState &m = *P.mi;
+  if (coarsening(m, n) <= 0 || unet::check_batch(n, t.batch) != 0 || t.steps <= 0 ||
+      !(t.learning_rate > 0.0) || count <= 0 || fac == nullptr || phi == nullptr ||
+      hemi == nullptr) {
+    set_error("invalid warm-start training configuration or data");
+    return -1;
+  }
+  for (int b = 0; b < count; ++b)
+    if (hemi[b] != GAMERA_MI_NORTH && hemi[b] != GAMERA_MI_SOUTH) {
+      set_error("hemisphere must be GAMERA_MI_NORTH or GAMERA_MI_SOUTH");
+      return -1;
+    }
+  const double rm = m.cfg.ionosphere_radius_re * m.cfg.length_norm_m;
+  const double r2 = rm * rm;
+  const size_t per = (size_t)m.nt * m.np;
+  hipStream_t s = n.stream;
+  PairBatch pb;
+  if (pb.allocate(t.batch, m.nt, m.np) != 0) return -1;
+  double *data_fac = nullptr, *data_phi = nullptr;
+  float *host_loss = nullptr;
+  int rc = 0;
+  if (hipMalloc(&data_fac, count * per * sizeof(double)) != hipSuccess ||
+      hipMalloc(&data_phi, count * per * sizeof(double)) != hipSuccess ||
+      hipHostMalloc(&host_loss, sizeof(float)) != hipSuccess ||
+      hipMemcpy(data_fac, fac, count * per * sizeof(double), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(data_phi, phi, count * per * sizeof(double), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    set_error("warm-start training: device buffers for the pairs");
+    rc = -1;
+  }
+  std::mt19937_64 rng(t.seed);
+  std::uniform_int_distribution<int> pick(0, count - 1);
+  const int f = coarsening(m, n);
+  const dim3 g(blocks_of((long)n.H * n.W), t.batch);
+  for (long step = 0; step < t.steps && rc == 0; ++step) {
+    // the pinned samples of the previous step must be copied before reuse
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      set_error("warm-start training: stream error");
+      rc = -1;
+      break;
+    }
+    for (int b = 0; b < t.batch && rc == 0; ++b) {
+      const int idx = pick(rng);
+      GenSample &gs = pb.host[b];
+      gs.hemi = hemi[idx];
+      gs.sp = m.cfg.pedersen_siemens;
+      gs.sh = m.hall;
+      if (hipMemcpyAsync(pb.fac + b * per, data_fac + idx * per, per * sizeof(double),
+                         hipMemcpyDeviceToDevice, s) != hipSuccess ||
+          hipMemcpyAsync(pb.phi + b * per, data_phi + idx * per, per * sizeof(double),
+                         hipMemcpyDeviceToDevice, s) != hipSuccess) {
+        set_error("warm-start training: batch gather");
+        rc = -1;
+      }
+    }
+    if (rc != 0) break;
+    if (hipMemcpyAsync(pb.dev, pb.host, t.batch * sizeof(GenSample), hipMemcpyHostToDevice,
+                       s) != hipSuccess) {
+      set_error("warm-start training: sample upload");
+      rc = -1;
+      break;
+    }
+    k_gen_scale<<<t.batch, kT, 0, s>>>(m.nt, m.np, pb.fac, pb.scale);
+    if (n.adam_t == 0 && step == 0) {
+      // output scale: unit RMS of the first batch's targets
+      k_gen_features<<<g, kT, 0, s>>>(pb.dev, m.nt, m.np, f, r2, pb.fac, pb.phi, pb.scale,
+                                      1.0, n.input, n.target);
+      std::vector<float> tg((size_t)t.batch * n.H * n.W);
+      if (hipMemcpyAsync(tg.data(), n.target, tg.size() * sizeof(float),
+                         hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        set_error("warm-start training: target readback failed");
+        rc = -1;
+        break;
+      }
+      double ss = 0.0;
+      for (float v : tg) ss += (double)v * v;
+      n.out_scale = std::sqrt(ss / (double)tg.size());
+      if (!(n.out_scale > 0.0) || !std::isfinite(n.out_scale)) n.out_scale = 1.0;
+    }
+    k_gen_features<<<g, kT, 0, s>>>(pb.dev, m.nt, m.np, f, r2, pb.fac, pb.phi, pb.scale,
+                                    n.out_scale, n.input, n.target);
+    const double warm = std::min(1.0, (double)(step + 1) / 50.0);
+    const double frac = (double)step / (double)std::max(1L, t.steps - 1);
+    const double lr = t.learning_rate * warm * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(kPi * frac)));
+    if (unet::forward_device(n, t.batch) != 0 || unet::loss_relative_device(n, t.batch) != 0 ||
+        unet::backward_device(n, t.batch) != 0 || unet::adam_device(n, (float)lr) != 0) {
+      rc = -1;
+      break;
+    }
+    if (loss_history != nullptr) {
+      if (hipMemcpyAsync(host_loss, n.loss, sizeof(float), hipMemcpyDeviceToHost, s) !=
+              hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        set_error("warm-start training: loss readback failed");
+        rc = -1;
+        break;
+      }
+      loss_history[step] = *host_loss;
+    }
+  }
+  if (hipStreamSynchronize(s) != hipSuccess && rc == 0) {
+    set_error("warm-start training: stream error");
+    rc = -1;
+  }
+  if (host_loss) (void)hipHostFree(host_loss);
+  (void)hipFree(data_fac);
+  (void)hipFree(data_phi);
   return rc;
 }
 

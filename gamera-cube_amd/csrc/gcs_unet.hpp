@@ -77,6 +77,9 @@ int mi_make_pairs(Problem &P, uint64_t seed, int count, double sp, double sh, do
                   double *phi_out);
 int mi_train_warm_start(Problem &P, unet::Net &n, const gcs_warm_train_config &t,
                         float *loss_history);
+int mi_train_warm_start_on(Problem &P, unet::Net &n, const gcs_warm_train_config &t,
+                           const double *fac, const double *phi, const int *hemi, int count,
+                           float *loss_history);
 int mi_solve_current(Problem &P, const double *fac, double *potential, int use_net,
                      int iterations[2], double *ms);
 

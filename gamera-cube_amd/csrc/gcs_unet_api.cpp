@@ -129,6 +129,15 @@ int gcs_problem_mi_train_warm_start(gcs_problem *problem, gcs_unet *net,
   return gcs::mi_train_warm_start(*as_problem(problem), *as_net(net), *cfg, loss_history);
 }
 
+int gcs_problem_mi_train_warm_start_on(gcs_problem *problem, gcs_unet *net,
+                                       const gcs_warm_train_config *cfg, const double *current,
+                                       const double *potential, const int *hemisphere,
+                                       int count, float *loss_history) {
+  if (problem == nullptr || net == nullptr || cfg == nullptr) return -1;
+  return gcs::mi_train_warm_start_on(*as_problem(problem), *as_net(net), *cfg, current,
+                                     potential, hemisphere, count, loss_history);
+}
+
 int gcs_problem_mi_make_pairs(gcs_problem *problem, unsigned long long seed, int count,
                               double pedersen, double hall, double *current,
                               double *potential) {

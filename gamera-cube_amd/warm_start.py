@@ -48,6 +48,9 @@ def _declare(lib):
     lib.gcs_problem_mi_attach_warm_start.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     lib.gcs_problem_mi_train_warm_start.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
                                                     ctypes.POINTER(TrainConfig), _F32P]
+    lib.gcs_problem_mi_train_warm_start_on.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(TrainConfig), _F64P, _F64P,
+        ctypes.POINTER(ctypes.c_int), ctypes.c_int, _F32P]
     lib.gcs_problem_mi_make_pairs.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong,
                                               ctypes.c_int, ctypes.c_double,
                                               ctypes.c_double, _F64P, _F64P]
@@ -179,6 +182,84 @@ def train(problem, net, steps, batch, learning_rate=1e-3, seed=1, pedersen=(5.0,
     return losses
 
 
+def train_on(problem, net, current, potential, hemisphere, steps, batch, learning_rate=1e-3,
+             seed=1):
+    """Train `net` on given pairs (current, potential [count][nt][np], each
+    one hemisphere: GAMERA_MI_NORTH = -1 or GAMERA_MI_SOUTH = 1); per-step
+    losses."""
+    lib = problem._lib
+    _declare(lib)
+    current, potential = _f64(current), _f64(potential)
+    hemi = np.ascontiguousarray(hemisphere, dtype=np.int32)
+    count = current.shape[0]
+    if potential.shape != current.shape or hemi.shape != (count,):
+        raise ValueError("current, potential and hemisphere must describe the same pairs")
+    cfg = TrainConfig(steps, batch, learning_rate, seed, 0.0, 0.0, 0.0, 0.0)
+    losses = np.zeros(steps, dtype=np.float32)
+    _check(lib.gcs_problem_mi_train_warm_start_on(
+        problem._handle, net._handle, ctypes.byref(cfg), current.ctypes.data_as(_F64P),
+        potential.ctypes.data_as(_F64P), hemi.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+        count, losses.ctypes.data_as(_F32P)), "gcs_problem_mi_train_warm_start_on")
+    return losses
+
+
+# Solar-wind ranges of the logged runs: Bz in [-10, 5] nT, n in [2, 20] cm^-3,
+# V in [300, 700] km/s, uniform.
+WIND_RANGES = {"bz_nt": (-10.0, 5.0), "n_cm3": (2.0, 20.0), "speed_km_s": (300.0, 700.0)}
+
+
+def wind_variation(seed):
+    """One seeded draw of the solar wind from WIND_RANGES."""
+    rng = np.random.default_rng(seed)
+    return {k: float(rng.uniform(lo, hi)) for k, (lo, hi) in WIND_RANGES.items()}
+
+
+def log_currents(problem, samples, every, skip=0):
+    """Advance `skip` driver steps, then `samples` times: `every` steps and a
+    sample of the field-aligned current onto the MI grids. Returns
+    [samples][2][nt][np] currents and their simulated times (code units)."""
+    nt = problem.config.mi.colatitude_points
+    np_ = problem.config.mi.azimuth_points
+    if skip:
+        problem.advance(skip)
+    out = np.zeros((samples, 2, nt, np_))
+    times = np.zeros(samples)
+    for k in range(samples):
+        problem.advance(every)
+        problem.mi_sample_current()
+        out[k] = problem.mi_field(1).reshape(2, nt, np_)
+        times[k] = problem.clock()["time"]
+    return out, times
+
+
+def solve_targets(problem, currents):
+    """The plain FP64 Krylov solve of each current [k][2][nt][np] to the
+    problem's MI tolerance, each started from the previous solution; returns
+    the potentials and the (north, south) iterations of each solve."""
+    pots = np.zeros_like(currents)
+    iters = []
+    guess = np.zeros_like(currents[0])
+    for k in range(currents.shape[0]):
+        pots[k], it, _ = solve_current(problem, currents[k], guess, use_network=False)
+        iters.append(it)
+        guess = pots[k]
+    return pots, iters
+
+
+def hemisphere_pairs(currents, potentials, lag=0):
+    """Training pairs from logged (current, potential) samples [k][2][nt][np]:
+    lag 0 the samples themselves, lag L > 0 their differences L samples
+    apart (what the warm start sees: the residual of the potential L
+    samples earlier is the current's change, its error the potential's
+    change). Each hemisphere is one pair."""
+    if lag:
+        currents = currents[lag:] - currents[:-lag]
+        potentials = potentials[lag:] - potentials[:-lag]
+    k, _, nt, np_ = currents.shape
+    hemi = np.tile(np.array([-1, 1], dtype=np.int32), k)
+    return currents.reshape(2 * k, nt, np_), potentials.reshape(2 * k, nt, np_), hemi
+
+
 def make_pairs(problem, seed, count, pedersen, hall):
     """(current, potential), each [count][nt][np]; even samples north."""
     lib = problem._lib
@@ -210,5 +291,6 @@ def solve_current(problem, current, guess, use_network):
     return pot, (iters[0], iters[1]), ms.value
 
 
-__all__ = ["WarmStartNet", "GpuProblem", "attach", "train", "make_pairs", "solve_current",
-           "gemm_check", "IN_CHANNELS"]
+__all__ = ["WarmStartNet", "GpuProblem", "attach", "train", "train_on", "make_pairs",
+           "solve_current", "log_currents", "solve_targets", "hemisphere_pairs",
+           "wind_variation", "WIND_RANGES", "gemm_check", "IN_CHANNELS"]

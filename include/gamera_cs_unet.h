@@ -95,6 +95,18 @@ int gcs_problem_mi_attach_warm_start(gcs_problem *problem, gcs_unet *net);
 int gcs_problem_mi_train_warm_start(gcs_problem *problem, gcs_unet *net,
                                     const gcs_warm_train_config *cfg,
                                     float *loss_history);
+/* Train on given pairs: current and potential [count][nt][np] (each one
+ * hemisphere, hemisphere[b] = GAMERA_MI_NORTH or GAMERA_MI_SOUTH), e.g.
+ * currents logged from MHD runs (gcs_problem_mi_sample_current) with the
+ * potentials the plain solve gives for them, or the differences of two
+ * such pairs. Uses cfg's steps, batch, learning rate and seed, the same
+ * loss and schedule as gcs_problem_mi_train_warm_start, and the problem's
+ * conductances; the pairs stay on the device for the run. */
+int gcs_problem_mi_train_warm_start_on(gcs_problem *problem, gcs_unet *net,
+                                       const gcs_warm_train_config *cfg,
+                                       const double *current, const double *potential,
+                                       const int *hemisphere, int count,
+                                       float *loss_history);
 /* `count` generated pairs for the problem's MI grid with uniform
  * conductances: current and potential [count][nt][np]; sample b is in the
  * northern hemisphere for even b, the southern for odd b. */

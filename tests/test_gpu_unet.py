@@ -214,3 +214,59 @@ def test_pooled_warm_start_on_the_config1_polar_grid(ws, gpu):
     assert np.abs(learned - plain).max() <= 1e-5 * np.abs(plain).max()
     assert sum(it_net) < sum(it_plain)
     ws.attach(p, None)
+
+
+def test_warm_start_on_logged_in_run_pairs(ws, gpu):
+    """The in-run data path: MHD runs over seeded solar-wind variations log
+    their field-aligned current (gcs_problem_mi_sample_current), the plain
+    FP64 solve gives each current's potential, the network trains on those
+    pairs and on their changes (gcs_problem_mi_train_warm_start_on), and on
+    a held-out run the solve from its correction of the previous potential
+    reaches the same potential as the plain solve from the previous
+    potential. Iterations of both are printed."""
+    cfgm = gcs_config
+    runs, samples, every, lag = 3, 12, 6, 4
+    data = []
+    keep = None
+    for r in range(runs + 1):
+        wind = ws.wind_variation(500 + r)
+        assert all(lo <= wind[k] <= hi for k, (lo, hi) in ws.WIND_RANGES.items())
+        cfg = cfgm.earth_problem(16, 12, perturbation=0.01, seed=7 + r, mi=True,
+                                 hall_siemens=5.0, mi_azimuth=64, mi_colatitude=32, **wind)
+        p = gpu.GpuProblem(cfg)
+        cur, times = ws.log_currents(p, samples, every)
+        pot, _ = ws.solve_targets(p, cur)
+        assert np.all(np.diff(times) > 0) and np.abs(cur).max() > 0
+        data.append((cur, pot))
+        if keep is not None:
+            keep.close()
+        keep = p
+    pairs = [ws.hemisphere_pairs(c, q, lag=l) for c, q in data[:runs] for l in (0, lag)]
+    cur_t = np.concatenate([t[0] for t in pairs])
+    pot_t = np.concatenate([t[1] for t in pairs])
+    hemi_t = np.concatenate([t[2] for t in pairs])
+    assert cur_t.shape[0] == 2 * runs * (samples + samples - lag)
+    nt, np_ = keep.config.mi.colatitude_points, keep.config.mi.azimuth_points
+    net = ws.WarmStartNet(nt, np_, 8, seed=3)
+    losses = ws.train_on(keep, net, cur_t, pot_t, hemi_t, steps=400, batch=8, seed=5)
+    assert np.all(np.isfinite(losses)) and losses[-50:].mean() < losses[:10].mean()
+    ws.attach(keep, net)
+    cur, pot = data[runs]
+    it_plain, it_net = [], []
+    for k in range(lag, samples):
+        plain, a, _ = ws.solve_current(keep, cur[k], pot[k - lag], use_network=False)
+        learned, b, _ = ws.solve_current(keep, cur[k], pot[k - lag], use_network=True)
+        assert np.abs(learned - plain).max() <= 1e-5 * np.abs(plain).max(), k
+        it_plain.append(sum(a))
+        it_net.append(sum(b))
+    print(f"logged pairs: {cur_t.shape[0]} training pairs, relative loss "
+          f"{losses[:10].mean():.3e} -> {losses[-50:].mean():.3e}; held-out iterations "
+          f"(north + south) from the previous potential {it_plain}, learned {it_net}",
+          flush=True)
+    ws.attach(keep, None)
+    # a hemisphere other than GAMERA_MI_NORTH / GAMERA_MI_SOUTH is rejected
+    # before any device work
+    with pytest.raises(RuntimeError):
+        ws.train_on(keep, net, cur_t[:2], pot_t[:2], np.array([0, 1]), steps=1, batch=2)
+    net.close()
+    keep.close()
