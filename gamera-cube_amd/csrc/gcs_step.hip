@@ -2825,9 +2825,16 @@ GCS_DEV bool local_dt_cell(const double u[5], const double b[3], const double b0
   return isfinite(dt) && dt > 0.0;
 }
 
-// Wavefront minimum of dt (as IEEE bits: every candidate is positive) into
-// acc[0] and any failure into acc[1]. Every lane of the wave calls it.
+// Block minimum of dt (as IEEE bits: every candidate is positive) into
+// acc[0] and any failure into acc[1], one atomic per block: atomics of every
+// wave on one address serialise at the L2. Every thread of the 256-thread
+// block calls it.
+constexpr int kLocalDtThreads = 256;
+constexpr unsigned kLocalDtBlocks = 2048;  // grid-stride: 8 blocks per CU
+
 GCS_DEV void local_dt_fold(double dt, bool bad, unsigned long long *acc) {
+  __shared__ unsigned long long s_min[kLocalDtThreads / 64];
+  __shared__ int s_bad[kLocalDtThreads / 64];
   unsigned long long bits = __double_as_longlong(dt);
   int any = bad ? 1 : 0;
   for (int off = 32; off > 0; off >>= 1) {
@@ -2835,7 +2842,17 @@ GCS_DEV void local_dt_fold(double dt, bool bad, unsigned long long *acc) {
     bits = other < bits ? other : bits;
     any |= __shfl_down(any, off, 64);
   }
+  const int wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    s_min[wave] = bits;
+    s_bad[wave] = any;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kLocalDtThreads / 64; ++w) {
+      bits = s_min[w] < bits ? s_min[w] : bits;
+      any |= s_bad[w];
+    }
     atomicMin(&acc[0], bits);
     if (any) atomicOr(&acc[1], 1ull);
   }
@@ -2859,12 +2876,16 @@ struct DtBox {
   }
 };
 
-__global__ void __launch_bounds__(256) k_local_dt_planes(K c, int f, DtBox bx, LocalDtOptions o,
-                                                         unsigned long long *acc) {
-  int i, j, k;
-  double dt = DBL_MAX;
+// Grid-stride over the box's cells; every thread reaches the block fold.
+__global__ void __launch_bounds__(kLocalDtThreads) k_local_dt_planes(K c, int f, DtBox bx,
+                                                                     LocalDtOptions o,
+                                                                     unsigned long long *acc) {
+  double best = DBL_MAX;
   bool bad = false;
-  if (bx.cell(tid(), i, j, k)) {
+  const long n = (long)bx.n[0] * bx.n[1] * bx.n[2];
+  for (long t = tid(); t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    bx.cell(t, i, j, k);
     const long at = c.L.at(f, i, j, k);
     double u[5], b[3] = {0.0, 0.0, 0.0}, b0[3] = {0.0, 0.0, 0.0}, len[3];
     for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
@@ -2873,22 +2894,24 @@ __global__ void __launch_bounds__(256) k_local_dt_planes(K c, int f, DtBox bx, L
       if (o.mhd && o.background) b0[d] = c.p(P_B0C + d)[at];
       len[d] = c.p(P_CFL + d)[at];
     }
-    bad = !local_dt_cell(u, b, b0, len, o, dt);
-    if (bad) dt = DBL_MAX;
+    double dt;
+    if (local_dt_cell(u, b, b0, len, o, dt))
+      best = fmin(best, dt);
+    else
+      bad = true;
   }
-  local_dt_fold(dt, bad, acc);
+  local_dt_fold(best, bad, acc);
 }
 
-__global__ void __launch_bounds__(256) k_local_dt_arrays(const double *u, const double *b,
-                                                         const double *b0, const double *len,
-                                                         long ej, long ek, DtBox bx,
-                                                         LocalDtOptions o,
-                                                         unsigned long long *acc) {
-  int i, j, k;
-  const long t = tid();
-  double dt = DBL_MAX;
+__global__ void __launch_bounds__(kLocalDtThreads) k_local_dt_arrays(
+    const double *u, const double *b, const double *b0, const double *len, long ej, long ek,
+    DtBox bx, LocalDtOptions o, unsigned long long *acc) {
+  double best = DBL_MAX;
   bool bad = false;
-  if (bx.cell(t, i, j, k)) {
+  const long n = (long)bx.n[0] * bx.n[1] * bx.n[2];
+  for (long t = tid(); t < n; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    bx.cell(t, i, j, k);
     const long h = ((long)i * ej + j) * ek + k;  // gamera_no_index3
     double uc[5], bc[3] = {0.0, 0.0, 0.0}, b0c[3] = {0.0, 0.0, 0.0}, lc[3];
     for (int q = 0; q < 5; ++q) uc[q] = u[5 * h + q];
@@ -2897,10 +2920,13 @@ __global__ void __launch_bounds__(256) k_local_dt_arrays(const double *u, const 
       if (o.mhd && o.background) b0c[d] = b0[3 * h + d];
       lc[d] = len[3 * t + d];
     }
-    bad = !local_dt_cell(uc, bc, b0c, lc, o, dt);
-    if (bad) dt = DBL_MAX;
+    double dt;
+    if (local_dt_cell(uc, bc, b0c, lc, o, dt))
+      best = fmin(best, dt);
+    else
+      bad = true;
   }
-  local_dt_fold(dt, bad, acc);
+  local_dt_fold(best, bad, acc);
 }
 
 namespace {
@@ -2933,8 +2959,9 @@ int local_dt_planes(Problem &P, int face, const int lo[3], const int hi[3],
   const DtBox bx = dt_box(lo, hi);
   const long n = (long)bx.n[0] * bx.n[1] * bx.n[2];
   unsigned long long *acc = P.acc4;
+  const unsigned blocks = (unsigned)std::min<long>(kLocalDtBlocks, nblocks(n, kLocalDtThreads));
   k_local_dt_init<<<1, 1, 0, P.stream>>>(acc);
-  k_local_dt_planes<<<nblocks(n, 256), 256, 0, P.stream>>>(make_k(P), face, bx, o, acc);
+  k_local_dt_planes<<<blocks, kLocalDtThreads, 0, P.stream>>>(make_k(P), face, bx, o, acc);
   return finish_local_dt(acc, P.stream, dt, invalid);
 }
 
@@ -2944,9 +2971,10 @@ int local_dt_arrays(const double *u, const double *b, const double *b0, const do
                     bool *invalid) {
   const DtBox bx = dt_box(lo, hi);
   const long n = (long)bx.n[0] * bx.n[1] * bx.n[2];
+  const unsigned blocks = (unsigned)std::min<long>(kLocalDtBlocks, nblocks(n, kLocalDtThreads));
   k_local_dt_init<<<1, 1, 0, nullptr>>>(acc);
-  k_local_dt_arrays<<<nblocks(n, 256), 256, 0, nullptr>>>(u, b, b0, len, ext[1], ext[2], bx, o,
-                                                          acc);
+  k_local_dt_arrays<<<blocks, kLocalDtThreads, 0, nullptr>>>(u, b, b0, len, ext[1], ext[2], bx,
+                                                             o, acc);
   return finish_local_dt(acc, nullptr, dt, invalid);
 }
 
