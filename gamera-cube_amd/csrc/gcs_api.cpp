@@ -104,6 +104,7 @@ void free_problem(Problem *P) {
   (void)hipFree(P->cfl_partial);
   (void)hipFree(P->cfl_slots);
   (void)hipFree(P->acc4);
+  (void)hipFree(P->pc_partial);
   (void)hipFree(P->halo_cells);
   (void)hipFree(P->halo_faces);
   (void)hipFree(P->seams);

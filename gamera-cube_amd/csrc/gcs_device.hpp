@@ -143,7 +143,8 @@ struct Problem {
   StepScalars *scalars = nullptr;  // device
   unsigned long long *cfl_partial = nullptr;  // per-block CFL minima
   unsigned long long *cfl_slots = nullptr;    // kCflSlots CFL minima folded by the update
-  unsigned long long *acc4 = nullptr;         // 4 accumulator words (pressure control, div B)
+  unsigned long long *acc4 = nullptr;         // 4 accumulator words (div B, gates, local dt)
+  unsigned long long *pc_partial = nullptr;   // pressure-control block partials (on first use)
   // Driver problems fold the new state's CFL dt into cfl_slots during the
   // update (drop-in mirrors take dt from the caller); cfl_folded: the slots
   // hold the current state's minimum.

@@ -2419,21 +2419,55 @@ __global__ void k_widen(const float *src, double *dst, long n) {
 // max over active cells of |net total (B1 + B0) face flux| / volume, the
 // reference driver's divergence gate (test_cubedsphere_dipole_background.c:
 // 808-845); non-negative doubles order like their bit patterns.
-__global__ void k_div_b(K c, unsigned long long *out) {
+// Reductions over the active cells run grid-stride with at most
+// kReduceBlocks blocks of 256 threads and fold each block in LDS before one
+// atomic per block: atomics of every wave (or thread) on one address
+// serialise at the L2.
+constexpr unsigned kReduceBlocks = 2048;
+
+// Block extremum of u64 values (IEEE bits of non-negative doubles order
+// like the doubles) into *out: want_max selects max, else min. Every thread
+// of the 256-thread block calls it.
+GCS_DEV void block_extremum_atomic(unsigned long long v, bool want_max,
+                                   unsigned long long *out) {
+  __shared__ unsigned long long s[4];
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_down(v, off, 64);
+    v = want_max ? (o > v ? o : v) : (o < v ? o : v);
+  }
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) v = want_max ? (s[w] > v ? s[w] : v) : (s[w] < v ? s[w] : v);
+    if (want_max)
+      atomicMax(out, v);
+    else
+      atomicMin(out, v);
+  }
+  __syncthreads();  // s is reused by a second call in the same kernel
+}
+
+__global__ void __launch_bounds__(256) k_div_b(K c, unsigned long long *out) {
   const Layout &L = c.L;
   const int lo[3] = {L.ghost, L.ghost, L.ghost};
   const int hi[3] = {L.ghost + L.nr, L.ghost + L.n, L.ghost + L.n};
-  int f, i, j, k;
-  if (!box(tid(), lo, hi, f, i, j, k)) return;
-  const long at = L.at(f, i, j, k);
   const long step[3] = {L.di(), (long)L.SK, 1};
-  double net = 0.0;
-  for (int d = 0; d < 3; ++d) {
-    const double *phi = c.p(plane_phi(c.cur, d)), *b0 = c.p(P_B0FLUX + d);
-    net += (phi[at + step[d]] + b0[at + step[d]]) - (phi[at] + b0[at]);
+  const long n = 6L * L.nr * L.n * L.n;
+  unsigned long long best = 0;
+  for (long t = tid(); t < n; t += (long)gridDim.x * blockDim.x) {
+    int f, i, j, k;
+    box(t, lo, hi, f, i, j, k);
+    const long at = L.at(f, i, j, k);
+    double net = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      const double *phi = c.p(plane_phi(c.cur, d)), *b0 = c.p(P_B0FLUX + d);
+      net += (phi[at + step[d]] + b0[at + step[d]]) - (phi[at] + b0[at]);
+    }
+    const double v = fabs(net) / c.p(P_VOL)[at];
+    const unsigned long long bits = __double_as_longlong(isfinite(v) ? v : DBL_MAX);
+    best = bits > best ? bits : best;
   }
-  const double v = fabs(net) / c.p(P_VOL)[at];
-  atomicMax(out, (unsigned long long)__double_as_longlong(isfinite(v) ? v : DBL_MAX));
+  block_extremum_atomic(best, true, out);
 }
 
 int prepare_mixed(Problem &P) {
@@ -2547,51 +2581,9 @@ GCS_DEV void pressure_control_wave(bool low, bool hot, double sound, double pres
   out[3] = p_bits;
 }
 
-__global__ void k_pressure_control(K c, PressureControlArgs a, unsigned long long *acc) {
-  bool low, hot;
-  double sound, pressure;
-  pressure_control_cell(c, a, tid(), low, hot, sound, pressure);
-  unsigned long long r[4];
-  pressure_control_wave(low, hot, sound, pressure, r);
-  if ((threadIdx.x & 63) == 0) {
-    if (r[0]) atomicAdd(&acc[0], r[0]);
-    if (r[1]) atomicAdd(&acc[1], r[1]);
-    atomicMax(&acc[2], r[2]);
-    atomicMax(&acc[3], r[3]);
-  }
-}
 
 __global__ void k_zero4(unsigned long long *acc) { acc[threadIdx.x] = 0ull; }
 
-int pressure_control(Problem &P, const double args[7], unsigned long long counts[2],
-                     double maxima[2]) {
-  const PressureControlArgs a{args[0], args[1], args[2], args[3], args[4], args[5], args[6]};
-  if (!(a.dt > 0.0) || !(a.gamma > 1.0) || !(a.density_floor > 0.0) ||
-      !(a.pressure_floor > 0.0) || !(a.light_speed > 0.0) ||
-      !(a.low_density_threshold > 0.0) || !(a.low_density_sound_speed >= 0.0)) {
-    set_error("pressure control: invalid arguments");
-    return -1;
-  }
-  unsigned long long *acc = P.acc4, host[4] = {0, 0, 0, 0};
-  k_zero4<<<1, 4, 0, P.stream>>>(acc);
-  k_pressure_control<<<nblocks(6L * P.L.nr * P.L.n * P.L.n, 256), 256, 0, P.stream>>>(
-      make_k(P), a, acc);
-  const hipError_t e = hipGetLastError();
-  const hipError_t m = hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, P.stream);
-  const hipError_t w = hipStreamSynchronize(P.stream);
-  if (e != hipSuccess || m != hipSuccess || w != hipSuccess) {
-    set_error("pressure control: device error");
-    return -1;
-  }
-  // The state changed: the next step's dt is no longer the folded one.
-  P.cfl_folded = false;
-  P.host_dirty = true;
-  counts[0] = host[0];
-  counts[1] = host[1];
-  std::memcpy(&maxima[0], &host[2], sizeof(double));
-  std::memcpy(&maxima[1], &host[3], sizeof(double));
-  return 0;
-}
 
 // ---- pressure control without device allocation ---------------------------------
 //
@@ -2723,6 +2715,46 @@ int pressure_control_noalloc(Problem &P, const double args[7], unsigned long lon
   return 0;
 }
 
+// The allocating entry point: the same two kernels as the no-allocation one,
+// with the block partials in a buffer of the problem (allocated on first use).
+int pressure_control(Problem &P, const double args[7], unsigned long long counts[2],
+                     double maxima[2]) {
+  const PressureControlArgs a{args[0], args[1], args[2], args[3], args[4], args[5], args[6]};
+  if (!(a.dt > 0.0) || !(a.gamma > 1.0) || !(a.density_floor > 0.0) ||
+      !(a.pressure_floor > 0.0) || !(a.light_speed > 0.0) ||
+      !(a.low_density_threshold > 0.0) || !(a.low_density_sound_speed >= 0.0)) {
+    set_error("pressure control: invalid arguments");
+    return -1;
+  }
+  const long cells = 6L * P.L.nr * P.L.n * P.L.n;
+  const long blocks = (cells + 255) / 256;
+  if (P.pc_partial == nullptr) {
+    if (hipMalloc(&P.pc_partial, 4 * (blocks + 1) * sizeof(unsigned long long)) != hipSuccess) {
+      P.pc_partial = nullptr;
+      set_error("pressure control: device allocation failed");
+      return -1;
+    }
+  }
+  unsigned long long host[4] = {0, 0, 0, 0}, *total = P.pc_partial + 4 * blocks;
+  k_pressure_control_blocks<<<(unsigned)blocks, 256, 0, P.stream>>>(make_k(P), a, P.pc_partial);
+  k_pressure_control_total<<<1, 256, 0, P.stream>>>(P.pc_partial, blocks, total);
+  const hipError_t e = hipGetLastError();
+  const hipError_t m = hipMemcpyAsync(host, total, sizeof(host), hipMemcpyDeviceToHost, P.stream);
+  const hipError_t w = hipStreamSynchronize(P.stream);
+  if (e != hipSuccess || m != hipSuccess || w != hipSuccess) {
+    set_error("pressure control: device error");
+    return -1;
+  }
+  // The state changed: the next step's dt is no longer the folded one.
+  P.cfl_folded = false;
+  P.host_dirty = true;
+  counts[0] = host[0];
+  counts[1] = host[1];
+  std::memcpy(&maxima[0], &host[2], sizeof(double));
+  std::memcpy(&maxima[1], &host[3], sizeof(double));
+  return 0;
+}
+
 // The Earth driver's end-of-run gate (test_cubedsphere_dipole_background.c:
 // 807-893): minimum density and pressure of the active cells' primitive
 // state (conversion floors 1e-12) and the maximum total div B. acc = {min
@@ -2733,13 +2765,15 @@ __global__ void k_gate_init(unsigned long long *acc) {
   acc[threadIdx.x] = threadIdx.x < 2 ? __double_as_longlong(DBL_MAX) : 0ull;
 }
 
-__global__ void k_gates(K c, double gamma, unsigned long long *acc) {
+__global__ void __launch_bounds__(256) k_gates(K c, double gamma, unsigned long long *acc) {
   const Layout &L = c.L;
   const int lo[3] = {L.ghost, L.ghost, L.ghost};
   const int hi[3] = {L.ghost + L.nr, L.ghost + L.n, L.ghost + L.n};
-  int f, i, j, k;
+  const long n = 6L * L.nr * L.n * L.n;
   double rho = DBL_MAX, p = DBL_MAX;
-  if (box(tid(), lo, hi, f, i, j, k)) {
+  for (long t = tid(); t < n; t += (long)gridDim.x * blockDim.x) {
+    int f, i, j, k;
+    box(t, lo, hi, f, i, j, k);
     const long at = L.at(f, i, j, k);
     double u[5];
     for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
@@ -2747,20 +2781,12 @@ __global__ void k_gates(K c, double gamma, unsigned long long *acc) {
     if (!cons_to_prim(u, gamma, 1e-12, 1e-12, w)) {
       c.fail(91, at);
     } else {
-      rho = w.rho;
-      p = w.p;
+      rho = fmin(rho, w.rho);
+      p = fmin(p, w.p);
     }
   }
-  unsigned long long rb = __double_as_longlong(rho), pb = __double_as_longlong(p);
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long ro = __shfl_down(rb, off, 64), po = __shfl_down(pb, off, 64);
-    rb = ro < rb ? ro : rb;
-    pb = po < pb ? po : pb;
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMin(&acc[0], rb);
-    atomicMin(&acc[1], pb);
-  }
+  block_extremum_atomic(__double_as_longlong(rho), false, &acc[0]);
+  block_extremum_atomic(__double_as_longlong(p), false, &acc[1]);
 }
 
 int mi_point(const MiView &view, int what, const double x[3], const double v[3],
@@ -2785,8 +2811,9 @@ int final_gates(Problem &P, double gamma, double out[3]) {
   K c = make_k(P);
   const long n = 6L * P.L.nr * P.L.n * P.L.n;
   k_gate_init<<<1, 4, 0, P.stream>>>(acc);
-  k_gates<<<nblocks(n, 256), 256, 0, P.stream>>>(c, gamma, acc);
-  k_div_b<<<nblocks(n, 256), 256, 0, P.stream>>>(c, acc + 2);
+  const unsigned blocks = std::min(kReduceBlocks, nblocks(n, 256));
+  k_gates<<<blocks, 256, 0, P.stream>>>(c, gamma, acc);
+  k_div_b<<<blocks, 256, 0, P.stream>>>(c, acc + 2);
   const hipError_t e = hipGetLastError();
   const hipError_t m = hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, P.stream);
   const hipError_t w = hipStreamSynchronize(P.stream);
@@ -2982,7 +3009,8 @@ int max_div_b(Problem &P, double *out) {
   unsigned long long *dev = P.acc4, host = 0;
   k_zero4<<<1, 4, 0, P.stream>>>(dev);
   K c = make_k(P);
-  k_div_b<<<nblocks(6L * P.L.nr * P.L.n * P.L.n, 256), 256, 0, P.stream>>>(c, dev);
+  k_div_b<<<std::min(kReduceBlocks, nblocks(6L * P.L.nr * P.L.n * P.L.n, 256)), 256, 0,
+            P.stream>>>(c, dev);
   const hipError_t e = hipGetLastError();
   const hipError_t m = hipMemcpyAsync(&host, dev, sizeof(host), hipMemcpyDeviceToHost, P.stream);
   const hipError_t w = hipStreamSynchronize(P.stream);
