@@ -59,6 +59,7 @@ struct Callbacks {
   RadialFn radial = nullptr;
   SyncFn flux = nullptr;
   SyncFn emf = nullptr;
+  SyncFn tile_emf = nullptr;  // gamera_cs_earth_tile_emf (cubedsphere_earth_boundary.c:453-464)
   bool registered = false;
 };
 
@@ -151,6 +152,24 @@ void resolve_callbacks() {
       dlsym(RTLD_DEFAULT, "gamera_cs_earth_inner_fluid_flux"));
   g_cb.emf = reinterpret_cast<SyncFn>(
       dlsym(RTLD_DEFAULT, "gamera_cs_earth_no_mi_emf"));
+  g_cb.tile_emf = reinterpret_cast<SyncFn>(
+      dlsym(RTLD_DEFAULT, "gamera_cs_earth_tile_emf"));
+}
+
+// The Earth context of an EMF callback the device runs: the reference's
+// gamera_cs_earth_no_mi_emf's own context, or, for gamera_cs_earth_tile_emf
+// on the only radial tile (which applies both the outer and the inner wall
+// EMF, cubedsphere_earth_boundary.c:453-464), its tile context's Earth
+// context. Null otherwise (the callback then runs on the host).
+const gamera_cs_earth_boundary *device_emf_context(SyncFn fn, void *context) {
+  if (fn == nullptr) return nullptr;
+  if (fn == g_cb.emf) return static_cast<const gamera_cs_earth_boundary *>(context);
+  if (fn == g_cb.tile_emf && context != nullptr) {
+    const auto *t = static_cast<const gamera_cs_earth_tile_emf_context *>(context);
+    if (t->tile == nullptr || t->earth == nullptr) return nullptr;
+    if (t->tile->coord[0] == 0 && t->tile->proc_dims[0] == 1) return t->earth;
+  }
+  return nullptr;
 }
 
 // ---- mirrors --------------------------------------------------------------------
@@ -562,7 +581,7 @@ int advance(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
 
   // Boundary work: device kernels for the reference's Earth callbacks.
   const auto *rb = static_cast<const gamera_cs_earth_boundary *>(boundary_context);
-  const auto *eb = static_cast<const gamera_cs_earth_boundary *>(o->edge_emf_context);
+  const gamera_cs_earth_boundary *eb = device_emf_context(o->edge_emf_sync, o->edge_emf_context);
   StepBoundary &bd = P.bnd;
   bd = StepBoundary();
   bool host_radial = false, host_flux = false, host_emf = false;
@@ -583,7 +602,7 @@ int advance(const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
     else host_flux = true;
   }
   if (o->edge_emf_sync != nullptr) {
-    if (o->edge_emf_sync == g_cb.emf && earth_on_device(*m, eb, bg)) {
+    if (eb != nullptr && earth_on_device(*m, eb, bg)) {
       bd.earth_emf = true;
       if (wall == nullptr) wall = eb;
     } else {

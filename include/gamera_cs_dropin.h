@@ -87,10 +87,11 @@ int gamera_no_local_timestep(const gamera_no_grid *grid, const double *conserved
 
 /* Callbacks that run as device kernels. By default libgcs.so looks up the
  * reference's gamera_cs_earth_fill_radial_ghosts,
- * gamera_cs_earth_inner_fluid_flux and gamera_cs_earth_no_mi_emf
- * (cubedsphere_earth_boundary.h:26-45) in the global symbol table; a caller
- * that loads them privately names them here. Other callbacks run on the
- * host between device phases. */
+ * gamera_cs_earth_inner_fluid_flux, gamera_cs_earth_no_mi_emf and
+ * gamera_cs_earth_tile_emf (the last on a single radial tile only;
+ * cubedsphere_earth_boundary.h:26-63) in the global symbol table; a caller
+ * that loads the first three privately names them here. Other callbacks
+ * run on the host between device phases. */
 void gcs_dropin_register_earth_callbacks(
     gamera_cs_radial_boundary_function radial,
     int (*inner_fluid_flux)(gamera_no_storage *, const gamera_no_grid *,

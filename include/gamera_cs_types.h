@@ -374,6 +374,22 @@ typedef struct {
   int strict_inner_gas_wall;
 } gamera_cs_earth_boundary;
 
+/* cubedsphere_decomposition.h:11-18: one rank's tile of a cube face. */
+typedef struct {
+  int face;
+  int coord[3];
+  int proc_dims[3];
+  size_t global_active[3];
+  size_t local_active[3];
+  size_t global_start[3];
+} gamera_cs_decomposition;
+
+/* cubedsphere_earth_boundary.h:53-56: context of gamera_cs_earth_tile_emf. */
+typedef struct {
+  const gamera_cs_decomposition *tile;
+  gamera_cs_earth_boundary *earth;
+} gamera_cs_earth_tile_emf_context;
+
 #ifdef __cplusplus
 }
 #endif

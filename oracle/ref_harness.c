@@ -92,6 +92,9 @@ typedef struct {
    * its gamera_cs_mi_dipole_maybe_update */
   void *ext_mi;
   int (*ext_update)(void *mi, double time_code);
+  /* the MPI Earth driver's EMF callback on one radial tile (refh_use_tile_emf) */
+  gamera_cs_decomposition tile;
+  gamera_cs_earth_tile_emf_context tile_emf;
 } refh_problem;
 
 /* Wrappers of the reference's Earth callbacks at addresses of their own:
@@ -530,6 +533,28 @@ int refh_local_timestep(refh_problem *p, void *fn, int face, const size_t lower[
   const refh_timestep_fn call = fn != NULL ? (refh_timestep_fn)fn : gamera_no_local_timestep;
   return call(&p->grid[face], p->storage[face].conserved, p->storage[face].cell_magnetic,
               lower, upper, options, p->owned[face].cell_magnetic, dt);
+}
+
+/* 1: the edge EMF callback is gamera_cs_earth_tile_emf
+ * (cubedsphere_earth_boundary.c:453-464) with a decomposition of one tile
+ * per cube face, as the MPI Earth driver sets it
+ * (test_cubedsphere_mpi_earth_tile.c:369-379); 0: gamera_cs_earth_no_mi_emf. */
+void refh_use_tile_emf(refh_problem *p, int on) {
+  if (p == NULL) return;
+  if (on) {
+    memset(&p->tile, 0, sizeof(p->tile));
+    for (int a = 0; a < 3; ++a) {
+      p->tile.proc_dims[a] = 1;
+      p->tile.global_active[a] = p->tile.local_active[a] = p->active[a];
+    }
+    p->tile_emf.tile = &p->tile;
+    p->tile_emf.earth = &p->boundary;
+    p->options.edge_emf_sync = gamera_cs_earth_tile_emf;
+    p->options.edge_emf_context = &p->tile_emf;
+  } else {
+    p->options.edge_emf_sync = gamera_cs_earth_no_mi_emf;
+    p->options.edge_emf_context = &p->boundary;
+  }
 }
 
 /* The local CFL dt function of the driver's dt rule (the prototype of

@@ -103,10 +103,14 @@ def release(gpu_lib, ref_lib, p):
                                ref_lib.refh_storage(p._handle))
 
 
-def run_pair(ref_lib, gpu, name, steps, wrapped, step=None, mhd=True):
+def run_pair(ref_lib, gpu, name, steps, wrapped, step=None, mhd=True, tile_emf=False):
     """`step`: address of a function with the planned entry point's
-    prototype (default: libgcs.so's own); mhd=False: hydrodynamic steps."""
+    prototype (default: libgcs.so's own); mhd=False: hydrodynamic steps;
+    tile_emf: the MPI Earth driver's EMF callback on one radial tile.
+    Returns the host round trips the drop-in made."""
     lib = gpu.library()
+    lib.gcs_dropin_host_round_trips.restype = ctypes.c_long
+    rounds = lib.gcs_dropin_host_round_trips()
     cfg = gcs_config.workload(name)
     a = problem.Problem(ref_lib, "refh", cfg)
     b = problem.Problem(ref_lib, "refh", cfg)
@@ -114,6 +118,10 @@ def run_pair(ref_lib, gpu, name, steps, wrapped, step=None, mhd=True):
         ref_lib.refh_set_mhd.argtypes = [ctypes.c_void_p, ctypes.c_int]
         ref_lib.refh_set_mhd(a._handle, 0)
         ref_lib.refh_set_mhd(b._handle, 0)
+    if tile_emf:
+        ref_lib.refh_use_tile_emf.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        ref_lib.refh_use_tile_emf(a._handle, 1)
+        ref_lib.refh_use_tile_emf(b._handle, 1)
     if step is None:
         step = address(lib, "gamera_cs_advance_six_with_background_planned")
     ref_lib.refh_set_step(b._handle, step)
@@ -135,6 +143,7 @@ def run_pair(ref_lib, gpu, name, steps, wrapped, step=None, mhd=True):
             assert abs(ca["dt"] - cb["dt"]) <= STEP_TOL * (s + 1) * ca["dt"]
     finally:
         release(lib, ref_lib, b)
+    return lib.gcs_dropin_host_round_trips() - rounds
 
 
 @pytest.mark.parametrize("name,steps", [("tiny", 4), ("small", 8), ("ghost5", 4)])
@@ -198,7 +207,15 @@ def test_mpi_entry_points_match_reference_step(ref_global, gpu, entry):
 
 def test_dropin_runs_unknown_callbacks_on_host(ref_global, gpu):
     """Callbacks at other addresses run on the host between device phases."""
-    run_pair(ref_global, gpu, "small", 3, wrapped=True)
+    assert run_pair(ref_global, gpu, "small", 3, wrapped=True) > 0
+
+
+def test_tile_emf_callback_runs_on_device(ref_global, gpu):
+    """gamera_cs_earth_tile_emf (cubedsphere_earth_boundary.c:453-464), the
+    MPI Earth driver's EMF callback, on one radial tile applies both wall
+    EMFs: the device runs it like gamera_cs_earth_no_mi_emf, with the tile
+    context's Earth context, and no state goes to the host."""
+    assert run_pair(ref_global, gpu, "small", 4, wrapped=False, tile_emf=True) == 0
 
 
 def test_plain_entry_point_matches_reference(ref_global, gpu):
