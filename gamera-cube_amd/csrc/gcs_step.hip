@@ -1663,7 +1663,10 @@ GCS_DEV void face_flux_face_fp32(const K &c, int d, int f, long at, long st, con
   for (int e = 0; e < 3; ++e) c.sp(M_MAXWELL + 3 * d + e)[at] = area * mx.m[e];
 }
 
-__global__ void __launch_bounds__(128) k_face_flux_fp32(K c, int d) {
+// Four waves per SIMD: the rare FP64 fallback reconstruction otherwise
+// lifts the kernel to 154 VGPRs and three waves; bounded, it spills 84
+// bytes a lane on that path only (0.75 -> 0.63 ms per direction at config 1).
+__global__ void __launch_bounds__(128, 4) k_face_flux_fp32(K c, int d) {
   const Layout &L = c.L;
   const int lo[3] = {L.ghost, L.ghost, L.ghost};
   const int hi[3] = {L.ghost + L.nr + (d == 0), L.ghost + L.n + (d == 1),
