@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgcs.so")
 SOURCES = ["gcs_setup.hip", "gcs_step.hip", "gcs_mi.hip", "gcs_unet.hip", "gcs_plans.cpp",
-           "gcs_api.cpp", "gcs_transfer.cpp", "gcs_dropin.cpp", "gcs_unet_api.cpp"]
+           "gcs_api.cpp", "gcs_transfer.cpp", "gcs_dropin.cpp", "gcs_unet_api.cpp",
+           "gcs_output.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GCS_OFFLOAD_ARCH", "gfx950")
 

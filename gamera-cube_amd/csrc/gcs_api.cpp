@@ -585,6 +585,17 @@ int gcs_problem_set_wind(Problem *P, size_t count, const double *time, const dou
   return 0;
 }
 
+int gcs_problem_write_snapshot(Problem *P, const char *path) {
+  if (P == nullptr || path == nullptr) return -1;
+  return gcs::write_snapshot(*P, path);
+}
+
+int gcs_problem_write_drive_frame(Problem *P, const char *prefix, long frame, long step,
+                                  double seconds, const char *index_path) {
+  if (P == nullptr) return -1;
+  return gcs::write_drive_frame(*P, prefix, frame, step, seconds, index_path);
+}
+
 int gcs_copy_bandwidth(size_t doubles, int iterations, double *gb_per_s) {
   return gcs::copy_bandwidth(doubles, iterations, gb_per_s);
 }

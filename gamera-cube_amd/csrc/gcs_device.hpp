@@ -314,6 +314,15 @@ int max_div_b(Problem &P, double *out);
 // {min density, min pressure, max total div B} of the current state's
 // active cells (the Earth driver's end-of-run gate).
 int final_gates(Problem &P, double gamma, double out[3]);
+// The drivers' CSV output (gcs_output.cpp): the six-face snapshot
+// (test_cubedsphere_dipole_background.c:120-161), an output frame with its
+// index line (:177-193), and the ionosphere's CSV files
+// (cubedsphere_mi_dipole.c:620-690).
+int write_snapshot(Problem &P, const char *path);
+int write_drive_frame(Problem &P, const char *prefix, long frame, long step, double seconds,
+                      const char *index_path);
+int write_mi_csv(Problem &P, const char *prefix);
+int write_shell_current_csv(Problem &P, const char *prefix);
 // Read + write bytes per second of a plain device copy of `doubles` doubles
 // on the current device, timed with HIP events over `iterations` launches.
 int copy_bandwidth(size_t doubles, int iterations, double *gb_per_s);

@@ -1312,31 +1312,8 @@ int gamera_cs_mi_dipole_write_csv(const gamera_cs_mi_dipole *state, const char *
   using namespace gcs;
   std::lock_guard<std::mutex> guard(g_lock);
   MiHandle *h = find_mi(state);
-  if (!handle_ready(h) || h->m == nullptr || prefix == nullptr || std::strlen(prefix) > 4000U)
-    return -1;
-  std::vector<double> fac, pot;
-  if (polar_arrays(h, fac, pot) != 0) return -1;
-  MiInfo info;
-  mi_info(*h->m->P, info);
-  const size_t np = h->cfg.azimuth_points, nt = h->cfg.colatitude_points;
-  for (int hemi = 0; hemi < 2; ++hemi) {
-    const char *name = hemi == 0 ? "north" : "south";
-    char path[4096];
-    if (std::snprintf(path, sizeof(path), "%s.mi_%s.csv", prefix, name) >= (int)sizeof(path))
-      return -1;
-    FILE *file = std::fopen(path, "w");
-    if (file == nullptr) return -1;
-    std::fputs("time_s,hemisphere,theta_deg,phi_deg,fac_A_m2,potential_V\n", file);
-    for (size_t t = 0; t < nt; ++t)
-      for (size_t p = 0; p < np; ++p) {
-        const size_t q = hemi * np * nt + t * np + p;
-        std::fprintf(file, "%.17g,%s,%.17g,%.17g,%.17g,%.17g\n", info.epoch_s, name,
-                     info.max_colat * (double)t / (double)(nt - 1U) * 180.0 / std::acos(-1.0),
-                     360.0 * (double)p / (double)np, fac[q], pot[q]);
-      }
-    if (std::fclose(file) != 0) return -1;
-  }
-  return 0;
+  if (!handle_ready(h) || h->m == nullptr) return -1;
+  return write_mi_csv(*h->m->P, prefix);
 }
 
 // cubedsphere_mi_dipole.c:650-689
@@ -1345,24 +1322,8 @@ int gamera_cs_mi_dipole_write_shell_current_csv(const gamera_cs_mi_dipole *state
   using namespace gcs;
   std::lock_guard<std::mutex> guard(g_lock);
   MiHandle *h = find_mi(state);
-  if (!handle_ready(h) || h->m == nullptr || prefix == nullptr) return -1;
-  char path[4096];
-  if (std::snprintf(path, sizeof(path), "%s.shell_j.csv", prefix) >= (int)sizeof(path))
-    return -1;
-  std::vector<double> shell;
-  if (mi_shell_current(*h->m->P, shell) != 0) return -1;
-  FILE *file = std::fopen(path, "w");
-  if (file == nullptr) return -1;
-  std::fputs("face,j,k,mlat_deg,phi_deg,jparallel_A_m2\n", file);
-  const size_t nj = h->active[1], nk = h->active[2], per = 6 * nj * nk;
-  for (int face = 0; face < GAMERA_CS_FACE_COUNT; ++face)
-    for (size_t j = 0; j < nj; ++j)
-      for (size_t k = 0; k < nk; ++k) {
-        const size_t q = ((size_t)face * nj + j) * nk + k;
-        std::fprintf(file, "%d,%zu,%zu,%.17g,%.17g,%.17g\n", face, j, k, shell[per + q],
-                     shell[2 * per + q], shell[q]);
-      }
-  return std::fclose(file) == 0 ? 0 : -1;
+  if (!handle_ready(h) || h->m == nullptr) return -1;
+  return write_shell_current_csv(*h->m->P, prefix);
 }
 
 }  // extern "C"

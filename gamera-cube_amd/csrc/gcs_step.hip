@@ -3121,3 +3121,63 @@ int copy_bandwidth(size_t doubles, int iterations, double *gb_per_s) {
 }
 
 }  // namespace gcs
+
+// ---- the drivers' CSV snapshot (the rows' values) ------------------------------------
+
+namespace gcs {
+
+// One cube face's snapshot rows (test_cubedsphere_dipole_background.c:128-150):
+// per active cell in i, j, k order, 11 doubles: centroid, density,
+// pressure and velocity with floors 1e-12, total cell B (background +
+// residual). The index of the first cell whose state does not convert is
+// folded into *first_bad.
+__global__ void __launch_bounds__(256) k_snapshot_face(K c, int f, double gamma, double *rows,
+                                                      unsigned long long *first_bad) {
+  const Layout &L = c.L;
+  const long n = (long)L.nr * L.n * L.n;
+  const long t = tid();
+  if (t >= n) return;
+  const int k = L.ghost + (int)(t % L.n);
+  const int j = L.ghost + (int)((t / L.n) % L.n);
+  const int i = L.ghost + (int)(t / ((long)L.n * L.n));
+  const long at = L.at(f, i, j, k);
+  double u[5];
+  for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
+  Prim w;
+  if (!cons_to_prim(u, gamma, 1e-12, 1e-12, w)) {
+    atomicMin(first_bad, (unsigned long long)t);
+    return;
+  }
+  double *r = rows + 11 * t;
+  for (int m = 0; m < 3; ++m) r[m] = c.p(P_CEN + m)[at];
+  r[3] = w.rho;
+  r[4] = w.p;
+  for (int m = 0; m < 3; ++m) r[5 + m] = w.v[m];
+  for (int m = 0; m < 3; ++m) r[8 + m] = c.p(P_B0C + m)[at] + c.p(plane_b(c.cur, m))[at];
+}
+
+int snapshot_face(Problem &P, int face, double gamma, std::vector<double> &rows,
+                  long *first_bad) {
+  const long n = (long)P.L.nr * P.L.n * P.L.n;
+  rows.resize((size_t)n * 11);
+  double *dev = nullptr;
+  GCS_HIP(hipMalloc(&dev, rows.size() * sizeof(double)));
+  unsigned long long *bad = P.acc4, host_bad = 0;
+  k_gate_init<<<1, 4, 0, P.stream>>>(bad);  // bad[0] = max, as "none"
+  k_snapshot_face<<<nblocks(n, 256), 256, 0, P.stream>>>(make_k(P), face, gamma, dev, bad);
+  const hipError_t e = hipGetLastError();
+  const hipError_t m1 = hipMemcpyAsync(rows.data(), dev, rows.size() * sizeof(double),
+                                       hipMemcpyDeviceToHost, P.stream);
+  const hipError_t m2 = hipMemcpyAsync(&host_bad, bad, sizeof(host_bad), hipMemcpyDeviceToHost,
+                                       P.stream);
+  const hipError_t w = hipStreamSynchronize(P.stream);
+  (void)hipFree(dev);
+  if (e != hipSuccess || m1 != hipSuccess || m2 != hipSuccess || w != hipSuccess) {
+    set_error("snapshot: device error");
+    return -1;
+  }
+  *first_bad = host_bad < (unsigned long long)n ? (long)host_bad : -1;
+  return 0;
+}
+
+}  // namespace gcs

@@ -168,6 +168,25 @@ int gcs_problem_time_steps(gcs_problem *problem, long steps, int profile,
 int gcs_stage_count(void);
 const char *gcs_stage_name(int stage);
 
+/* The serial Earth driver's six-face CSV snapshot of the current state
+ * (cubedsphere/tests/test_cubedsphere_dipole_background.c:120-161,
+ * write_earth_snapshot): header "face,i,j,k,x,y,z,rho,p,vx,vy,vz,bx,by,bz",
+ * then per active cell in face, i, j, k order the active-relative indices,
+ * centroid, primitive state with floors 1e-12 and total cell B, %.17g; with
+ * the MI coupling on, the ionosphere's files path.mi_north.csv,
+ * path.mi_south.csv and path.shell_j.csv follow (cubedsphere_mi_dipole.c:
+ * 620-690). Values are computed on the device, formatted on the host. -1
+ * when a cell's state does not convert (the rows before it are written, as
+ * the reference leaves them) or a file cannot be written. */
+int gcs_problem_write_snapshot(gcs_problem *problem, const char *path);
+
+/* test_cubedsphere_dipole_background.c:177-193 (write_drive_frame): the
+ * snapshot "<prefix>_<frame, 6 digits>.csv" and the line
+ * "frame,step,simulation_seconds,path" appended to index_path, whose header
+ * (:561) is written first when the file is new or empty. */
+int gcs_problem_write_drive_frame(gcs_problem *problem, const char *prefix, long frame,
+                                  long step, double seconds, const char *index_path);
+
 /* Measurement helpers, no reference counterpart: the HBM bandwidth of a
  * plain device-to-device copy of `doubles` doubles on the current device
  * (read + write bytes per second over `iterations` launches, HIP events),

@@ -9,6 +9,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -517,4 +518,44 @@ int orc_pressure_control(orc_problem *p, double dt, double gamma, double density
                                     low_density_threshold, low_density_sound_speed, counts,
                                     maxima) != 0;
   return failed ? -1 : 0;
+}
+
+/* The serial Earth driver's six-face CSV snapshot
+ * (cubedsphere/tests/test_cubedsphere_dipole_background.c:120-156,
+ * write_earth_snapshot, without the MI files it appends): a header, then
+ * per active cell in face, i, j, k order (k fastest) the active-relative
+ * indices, the centroid, the primitive state with floors 1e-12 and the
+ * total cell B (background + residual), %.17g. Returns -1 at the first cell
+ * whose state does not convert, leaving the rows before it, as the
+ * reference does. */
+int orc_write_snapshot(orc_problem *p, const char *path) {
+  if (p == NULL || path == NULL) return -1;
+  FILE *out = fopen(path, "w");
+  if (out == NULL) return -1;
+  fputs("face,i,j,k,x,y,z,rho,p,vx,vy,vz,bx,by,bz\n", out);
+  const double gamma = p->cfg.startup.gamma;
+  const size_t g = p->ghost;
+  for (int face = 0; face < ORC_FACES; ++face)
+    for (size_t i = g; i < g + p->active[0]; ++i)
+      for (size_t j = g; j < g + p->active[1]; ++j)
+        for (size_t k = g; k < g + p->active[2]; ++k) {
+          const size_t cell = orc_idx(p->grid[face].cell_extent, i, j, k);
+          gamera_no_primitive w;
+          if (orc_cons_to_prim(&p->storage[face].conserved[cell * ORC_NQ], gamma, 1e-12,
+                               1e-12, &w) != 0) {
+            fclose(out);
+            return -1;
+          }
+          const gamera_no_vec3 x = p->grid[face].cell[cell].centroid;
+          const gamera_no_vec3 b1 = p->storage[face].cell_magnetic[cell];
+          const gamera_no_vec3 b0 = p->bg[face].cell_magnetic[cell];
+          fprintf(out,
+                  "%d,%zu,%zu,%zu,%.17g,%.17g,%.17g,%.17g,%.17g,"
+                  "%.17g,%.17g,%.17g,%.17g,%.17g,%.17g\n",
+                  face, i - g, j - g, k - g, x.value[0], x.value[1], x.value[2], w.density,
+                  w.pressure, w.velocity.value[0], w.velocity.value[1], w.velocity.value[2],
+                  b0.value[0] + b1.value[0], b0.value[1] + b1.value[1],
+                  b0.value[2] + b1.value[2]);
+        }
+  return fclose(out) == 0 ? 0 : -1;
 }

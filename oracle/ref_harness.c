@@ -708,6 +708,35 @@ void refh_counters(const refh_problem *p, int face, double out[2]) {
 
 int refh_mi_updates(const refh_problem *p) { return p->mi_updates; }
 
+/* The driver's snapshot writers (ref_snapshot.c) on this problem's arrays,
+ * with its own ionosphere state (if any) for the MI files. */
+int refh_driver_write_snapshot(const char *path,
+                               const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+                               const gamera_no_storage storage[GAMERA_CS_FACE_COUNT],
+                               const gamera_no_background_data owned[GAMERA_CS_FACE_COUNT],
+                               const size_t active[3], size_t ghost, double gamma,
+                               const gamera_cs_mi_dipole *mi);
+int refh_driver_write_drive_frame(const char *prefix, long frame, long step, double seconds,
+                                  const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+                                  const gamera_no_storage storage[GAMERA_CS_FACE_COUNT],
+                                  const gamera_no_background_data owned[GAMERA_CS_FACE_COUNT],
+                                  const size_t active[3], size_t ghost, double gamma,
+                                  const gamera_cs_mi_dipole *mi, const char *index_path);
+
+int refh_write_snapshot(refh_problem *p, const char *path) {
+  if (p == NULL || path == NULL) return -1;
+  return refh_driver_write_snapshot(path, p->grid, p->storage, p->owned, p->active, p->ghost,
+                                    p->cfg.startup.gamma, p->mi);
+}
+
+int refh_write_drive_frame(refh_problem *p, const char *prefix, long frame, long step,
+                           double seconds, const char *index_path) {
+  if (p == NULL || prefix == NULL || index_path == NULL) return -1;
+  return refh_driver_write_drive_frame(prefix, frame, step, seconds, p->grid, p->storage,
+                                       p->owned, p->active, p->ghost, p->cfg.startup.gamma,
+                                       p->mi, index_path);
+}
+
 /* The reference's own ionosphere state of the problem (NULL without MI). */
 gamera_cs_mi_dipole *refh_mi(const refh_problem *p) {
   return p != NULL ? p->mi : NULL;

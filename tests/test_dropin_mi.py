@@ -91,7 +91,7 @@ def create_gcs_mi(ref, gcs, p, cfg):
     return h
 
 
-def test_mi_coupled_steps_through_the_reference_interface(libs):
+def test_mi_coupled_steps_through_the_reference_interface(libs, tmp_path):
     """45 driver steps (MI updates at t = 0 and 10 s) of a small Earth
     problem: the drop-in step with libgcs.so's ionosphere against the
     reference's step and ionosphere; no host round trip."""
@@ -147,6 +147,27 @@ def test_mi_coupled_steps_through_the_reference_interface(libs):
         for q in range(3):
             x, y = out["ref"][q], out["gcs"][q]
             assert np.abs(x - y).max() <= 1e-9 * np.abs(x).max(), q
+        # the CSV files (cubedsphere_mi_dipole.c:620-690): same headers,
+        # labels, grid coordinates and clock; values to the solve tolerance
+        files = {}
+        for key, lib, mi in (("ref", ref, ref_mi), ("gcs", gcs, h)):
+            prefix = str(tmp_path / key).encode()
+            lib.gamera_cs_mi_dipole_write_csv.argtypes = [VP, ctypes.c_char_p]
+            lib.gamera_cs_mi_dipole_write_shell_current_csv.argtypes = [VP, ctypes.c_char_p]
+            assert lib.gamera_cs_mi_dipole_write_csv(mi, prefix) == 0
+            assert lib.gamera_cs_mi_dipole_write_shell_current_csv(mi, prefix) == 0
+            files[key] = {ext: (tmp_path / f"{key}.{ext}").read_text().splitlines()
+                          for ext in ("mi_north.csv", "mi_south.csv", "shell_j.csv")}
+        for ext, exact_columns in (("mi_north.csv", 4), ("mi_south.csv", 4), ("shell_j.csv", 3)):
+            fa, fb = files["ref"][ext], files["gcs"][ext]
+            assert fa[0] == fb[0] and len(fa) == len(fb), ext
+            ra = [line.split(",") for line in fa[1:]]
+            rb = [line.split(",") for line in fb[1:]]
+            assert all(x[:exact_columns] == y[:exact_columns] for x, y in zip(ra, rb)), ext
+            va = np.array([[float(v) for v in x[exact_columns:]] for x in ra])
+            vb = np.array([[float(v) for v in x[exact_columns:]] for x in rb])
+            scale = np.maximum(np.abs(va).max(axis=0), 1e-300)
+            assert (np.abs(va - vb).max(axis=0) <= 1e-9 * scale).all(), ext
     finally:
         ref.refh_use_mi(b._handle, None, None)
         gcs.gamera_cs_mi_dipole_destroy(h)
