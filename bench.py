@@ -157,7 +157,17 @@ def dropin_resident(gpu_problem, cfg, steps, warmup=3):
         if gcs.gcs_dropin_sync_to_host(grid, storage) != 0:
             return {"error": "gcs_dropin_sync_to_host failed"}
         sync_s = time.perf_counter() - t1
+        # the default (strict) mode: every call uploads the host state and
+        # downloads every array the step writes, over PCIe
+        gcs.gcs_dropin_set_resident(0)
+        strict_steps = 2
+        t2 = time.perf_counter()
+        o.advance(strict_steps)
+        gcs.gcs_device_synchronize()
+        strict_s = time.perf_counter() - t2
         return {"ms_per_step": seconds * 1e3 / steps,
+                "strict_ms_per_step": strict_s * 1e3 / strict_steps,
+                "strict_cell_updates_per_s": cfg.active_cells * strict_steps / strict_s,
                 "cell_updates_per_s": cfg.active_cells * steps / seconds,
                 "steps": steps, "warmup": warmup,
                 "dt_reductions_on_device": gcs.gcs_dropin_local_timestep_device_calls() - served,

@@ -596,6 +596,26 @@ int gcs_problem_write_drive_frame(Problem *P, const char *prefix, long frame, lo
   return gcs::write_drive_frame(*P, prefix, frame, step, seconds, index_path);
 }
 
+int gcs_problem_load_snapshot(Problem *P, const char *path) {
+  if (P == nullptr || path == nullptr) return -1;
+  return gcs::load_snapshot(*P, path);
+}
+
+int gcs_problem_mi_maybe_update(Problem *P, double time_code, int *updated) {
+  if (P == nullptr || updated == nullptr || P->mi == nullptr) return -1;
+  const int rc = gcs::mi_maybe_update(*P, time_code);
+  if (rc < 0) return -1;
+  *updated = rc;
+  return 0;
+}
+
+int gcs_problem_write_mi_csv(Problem *P, const char *prefix) {
+  if (P == nullptr || prefix == nullptr || P->mi == nullptr) return -1;
+  return gcs::write_mi_csv(*P, prefix) == 0 && gcs::write_shell_current_csv(*P, prefix) == 0
+             ? 0
+             : -1;
+}
+
 int gcs_copy_bandwidth(size_t doubles, int iterations, double *gb_per_s) {
   return gcs::copy_bandwidth(doubles, iterations, gb_per_s);
 }

@@ -323,6 +323,7 @@ int write_drive_frame(Problem &P, const char *prefix, long frame, long step, dou
                       const char *index_path);
 int write_mi_csv(Problem &P, const char *prefix);
 int write_shell_current_csv(Problem &P, const char *prefix);
+int load_snapshot(Problem &P, const char *path);
 // Read + write bytes per second of a plain device copy of `doubles` doubles
 // on the current device, timed with HIP events over `iterations` launches.
 int copy_bandwidth(size_t doubles, int iterations, double *gb_per_s);

@@ -21,6 +21,8 @@ namespace gcs {
 
 int snapshot_face(Problem &P, int face, double gamma, std::vector<double> &rows,
                   long *first_bad);
+int load_snapshot_rows(Problem &P, const std::vector<double> &rows, const std::vector<int> &idx,
+                       double gamma, long *first_bad);
 
 namespace {
 
@@ -161,6 +163,99 @@ int write_drive_frame(Problem &P, const char *prefix, long frame, long step, dou
   ok = ok && std::fprintf(index, "%ld,%ld,%.17g,%s\n", frame, step, seconds, path) >= 0;
   ok = (std::fclose(index) == 0) && ok;
   return ok ? 0 : -1;
+}
+
+// test_cubedsphere_dipole_background.c:61-118 (load_mhd_snapshot): the
+// active cells' conserved state and residual cell B, both time levels, from
+// a snapshot of the format above. Every line must parse as the reference's
+// sscanf reads it, name an active cell and carry that cell's centroid
+// (within 1e-8), and the line count must equal the active cell count.
+// Unlike the reference, which writes as it reads, nothing is changed on
+// the device when any check fails. The face fluxes are not part of a
+// snapshot: the loaded cell B is for the MI coupling (the reference's MI
+// replay, :526-536); a step recovers cell B from the face fluxes again.
+int load_snapshot(Problem &P, const char *path) {
+  if (path == nullptr) return -1;
+  FILE *file = std::fopen(path, "rb");
+  if (file == nullptr) {
+    set_error(std::string("snapshot load: cannot open ") + path);
+    return -1;
+  }
+  std::string text;
+  char buffer[1 << 16];
+  size_t got;
+  while ((got = std::fread(buffer, 1, sizeof(buffer), file)) != 0) text.append(buffer, got);
+  std::fclose(file);
+  const char *header = "face,i,j,k,x,y,z,rho,p,vx,vy,vz,bx,by,bz\n";
+  if (text.compare(0, std::strlen(header), header) != 0) {
+    set_error("snapshot load: header");
+    return -1;
+  }
+  // line starts (fgets reads lines of at most 1023 characters)
+  std::vector<size_t> starts;
+  for (size_t pos = std::strlen(header); pos < text.size();) {
+    const size_t end = text.find('\n', pos);
+    starts.push_back(pos);
+    if (end == std::string::npos) break;
+    pos = end + 1;
+  }
+  const long n = (long)starts.size();
+  const long cells = 6L * P.L.nr * P.L.n * P.L.n;
+  if (n != cells) {
+    set_error("snapshot load: the line count is not the active cell count");
+    return -1;
+  }
+  std::vector<double> rows((size_t)n * 11);
+  std::vector<int> idx((size_t)n * 4);
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int threads = (int)std::max(1u, std::min(16u, hw == 0 ? 1u : hw));
+  std::vector<int> bad((size_t)threads, 0);
+  std::vector<std::thread> pool;
+  const long chunk = (n + threads - 1) / threads;
+  for (int w = 0; w < threads; ++w)
+    pool.emplace_back([&, w] {
+      char line[1024];
+      for (long r = (long)w * chunk; r < std::min(n, (long)(w + 1) * chunk) && !bad[w]; ++r) {
+        const size_t b = starts[(size_t)r];
+        const size_t e = std::min(text.find('\n', b), text.size());
+        const size_t len = std::min(e - b + 1, sizeof(line) - 1);
+        std::memcpy(line, text.data() + b, len);
+        line[len] = '\0';
+        int face;
+        size_t i, j, k;
+        double *x = &rows[11 * (size_t)r];
+        if (std::sscanf(line, "%d,%zu,%zu,%zu,%lf,%lf,%lf,%lf,%lf,%lf,%lf,%lf,%lf,%lf,%lf",
+                        &face, &i, &j, &k, &x[0], &x[1], &x[2], &x[3], &x[4], &x[5], &x[6],
+                        &x[7], &x[8], &x[9], &x[10]) != 15 ||
+            face < 0 || face >= 6 || i >= (size_t)P.L.nr || j >= (size_t)P.L.n ||
+            k >= (size_t)P.L.n) {
+          bad[w] = 1;
+          break;
+        }
+        int *q = &idx[4 * (size_t)r];
+        q[0] = face;
+        q[1] = (int)i;
+        q[2] = (int)j;
+        q[3] = (int)k;
+      }
+    });
+  for (std::thread &t : pool) t.join();
+  for (int b : bad)
+    if (b) {
+      set_error("snapshot load: a line does not parse or names no active cell");
+      return -1;
+    }
+  long first_bad = -1;
+  if (load_snapshot_rows(P, rows, idx, P.cfg.startup.gamma, &first_bad) != 0) return -1;
+  if (first_bad >= 0) {
+    set_error("snapshot load: line " + std::to_string(first_bad + 2) +
+              ": centroid differs by more than 1e-8 or the state does not convert");
+    return -1;
+  }
+  P.host_dirty = true;
+  P.cfl_folded = false;
+  P.b_recovered = false;
+  return 0;
 }
 
 }  // namespace gcs

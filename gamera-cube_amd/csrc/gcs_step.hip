@@ -3181,3 +3181,81 @@ int snapshot_face(Problem &P, int face, double gamma, std::vector<double> &rows,
 }
 
 }  // namespace gcs
+
+// ---- loading the drivers' CSV snapshot ------------------------------------------------
+
+namespace gcs {
+
+// Row r of a parsed snapshot (test_cubedsphere_dipole_background.c:61-118):
+// idx[4r..] = face and active-relative i, j, k; rows[11r..] = centroid,
+// density, pressure, velocity, total cell B. With apply 0 the rows are only
+// checked (centroid within 1e-8 of the cell's, primitives convertible with
+// floors 1e-12), the first bad row folded into *bad; with apply 1 the
+// conserved state and residual cell B of both levels are written.
+__global__ void __launch_bounds__(256) k_load_snapshot(K c, const double *rows, const int *idx,
+                                                      long n, double gamma, int apply,
+                                                      unsigned long long *bad) {
+  const long r = tid();
+  if (r >= n) return;
+  const Layout &L = c.L;
+  const int f = idx[4 * r], i = idx[4 * r + 1], j = idx[4 * r + 2], k = idx[4 * r + 3];
+  const long at = L.at(f, L.ghost + i, L.ghost + j, L.ghost + k);
+  const double *x = rows + 11 * r;
+  Prim w;
+  w.rho = x[3];
+  w.p = x[4];
+  for (int m = 0; m < 3; ++m) w.v[m] = x[5 + m];
+  double u[5];
+  bool ok = prim_to_cons(w, gamma, 1e-12, 1e-12, u);
+  for (int m = 0; m < 3; ++m) ok = ok && !(fabs(c.p(P_CEN + m)[at] - x[m]) > 1e-8);
+  if (!apply) {
+    if (!ok) atomicMin(bad, (unsigned long long)r);
+    return;
+  }
+  for (int q = 0; q < 5; ++q) {
+    c.p(plane_u(c.cur, q))[at] = u[q];
+    c.p(plane_u(c.old, q))[at] = u[q];
+  }
+  for (int m = 0; m < 3; ++m) {
+    const double residual = x[8 + m] - c.p(P_B0C + m)[at];
+    c.p(plane_b(c.cur, m))[at] = residual;
+    c.p(plane_b(c.old, m))[at] = residual;
+  }
+}
+
+int load_snapshot_rows(Problem &P, const std::vector<double> &rows, const std::vector<int> &idx,
+                       double gamma, long *first_bad) {
+  const long n = (long)idx.size() / 4;
+  double *drows = nullptr;
+  int *didx = nullptr;
+  unsigned long long *bad = P.acc4, host_bad = 0;
+  int status = -1;
+  if (hipMalloc(&drows, rows.size() * sizeof(double)) == hipSuccess &&
+      hipMalloc(&didx, idx.size() * sizeof(int)) == hipSuccess &&
+      hipMemcpy(drows, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice) ==
+          hipSuccess &&
+      hipMemcpy(didx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice) ==
+          hipSuccess) {
+    K c = make_k(P);
+    k_gate_init<<<1, 4, 0, P.stream>>>(bad);  // bad[0] = "none"
+    k_load_snapshot<<<nblocks(n, 256), 256, 0, P.stream>>>(c, drows, didx, n, gamma, 0, bad);
+    if (hipMemcpyAsync(&host_bad, bad, sizeof(host_bad), hipMemcpyDeviceToHost, P.stream) ==
+            hipSuccess &&
+        hipStreamSynchronize(P.stream) == hipSuccess) {
+      *first_bad = host_bad < (unsigned long long)n ? (long)host_bad : -1;
+      status = 0;
+      if (*first_bad < 0) {
+        k_load_snapshot<<<nblocks(n, 256), 256, 0, P.stream>>>(c, drows, didx, n, gamma, 1, bad);
+        status = hipGetLastError() == hipSuccess && hipStreamSynchronize(P.stream) == hipSuccess
+                     ? 0
+                     : -1;
+      }
+    }
+  }
+  (void)hipFree(drows);
+  (void)hipFree(didx);
+  if (status != 0) set_error("snapshot load: device error");
+  return status;
+}
+
+}  // namespace gcs

@@ -187,6 +187,31 @@ int gcs_problem_write_snapshot(gcs_problem *problem, const char *path);
 int gcs_problem_write_drive_frame(gcs_problem *problem, const char *prefix, long frame,
                                   long step, double seconds, const char *index_path);
 
+/* test_cubedsphere_dipole_background.c:61-118 (load_mhd_snapshot), the
+ * driver's earth-mi-replay input: every active cell's conserved state
+ * (primitive conversion with floors 1e-12) and residual cell B (total minus
+ * the dipole background) of both time levels from a snapshot written as
+ * above. Same checks as the reference: exact header, every line parsed by
+ * the same format naming an active cell whose centroid is within 1e-8, one
+ * line per active cell. -1 when any check fails; unlike the reference,
+ * which writes the lines before a bad one, the problem is then unchanged.
+ * Face fluxes, the clock and the ionosphere are not touched (a later step
+ * recovers cell B from the face fluxes, as the reference's boundary does). */
+int gcs_problem_load_snapshot(gcs_problem *problem, const char *path);
+
+/* gamera_cs_mi_dipole_maybe_update (cubedsphere_mi_dipole.h:59-60) of a
+ * problem created with mi_enabled, at code time time_code: *updated = 1 when
+ * the cadence called for a solve (which ran), 0 otherwise. With
+ * gcs_problem_load_snapshot it replays the driver's earth-mi-replay
+ * (:526-533). */
+int gcs_problem_mi_maybe_update(gcs_problem *problem, double time_code, int *updated);
+
+/* The ionosphere's CSV files of a problem created with mi_enabled
+ * (gamera_cs_mi_dipole_write_csv and _write_shell_current_csv,
+ * cubedsphere_mi_dipole.c:620-690): prefix.mi_north.csv,
+ * prefix.mi_south.csv and prefix.shell_j.csv. */
+int gcs_problem_write_mi_csv(gcs_problem *problem, const char *prefix);
+
 /* Measurement helpers, no reference counterpart: the HBM bandwidth of a
  * plain device-to-device copy of `doubles` doubles on the current device
  * (read + write bytes per second over `iterations` launches, HIP events),

@@ -723,6 +723,19 @@ int refh_driver_write_drive_frame(const char *prefix, long frame, long step, dou
                                   const size_t active[3], size_t ghost, double gamma,
                                   const gamera_cs_mi_dipole *mi, const char *index_path);
 
+int refh_driver_load_snapshot(const char *path, const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+                              gamera_no_storage storage[GAMERA_CS_FACE_COUNT],
+                              const gamera_no_background_data owned[GAMERA_CS_FACE_COUNT],
+                              const size_t active[3], size_t ghost, double gamma);
+
+/* The driver's earth-mi-replay input (:526-528): conserved state and
+ * residual cell B of both levels from a snapshot. */
+int refh_load_snapshot(refh_problem *p, const char *path) {
+  if (p == NULL || path == NULL) return -1;
+  return refh_driver_load_snapshot(path, p->grid, p->storage, p->owned, p->active, p->ghost,
+                                   p->cfg.startup.gamma);
+}
+
 int refh_write_snapshot(refh_problem *p, const char *path) {
   if (p == NULL || path == NULL) return -1;
   return refh_driver_write_snapshot(path, p->grid, p->storage, p->owned, p->active, p->ghost,

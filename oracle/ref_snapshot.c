@@ -1,7 +1,8 @@
 /*
- * The serial Earth driver's CSV snapshot writers
- * (cubedsphere/tests/test_cubedsphere_dipole_background.c:120-193,
- * write_earth_snapshot and write_drive_frame), compiled from the driver's
+ * The serial Earth driver's CSV snapshot reader and writers
+ * (cubedsphere/tests/test_cubedsphere_dipole_background.c:61-193,
+ * load_mhd_snapshot, write_earth_snapshot and write_drive_frame), compiled
+ * from the driver's
  * source where it lies: the file is included here with its main renamed,
  * so that tests can call these static functions on a harness problem's
  * arrays. Test infrastructure, like ref_harness.c; built into
@@ -36,4 +37,11 @@ int refh_driver_write_drive_frame(const char *prefix, long frame, long step, dou
   ok = ok && write_drive_frame(prefix, frame, step, seconds, grid, storage, owned, active,
                                ghost, gamma, mi, index) == 0;
   return fclose(index) == 0 && ok ? 0 : -1;
+}
+
+int refh_driver_load_snapshot(const char *path, const gamera_no_grid grid[GAMERA_CS_FACE_COUNT],
+                              gamera_no_storage storage[GAMERA_CS_FACE_COUNT],
+                              const gamera_no_background_data owned[GAMERA_CS_FACE_COUNT],
+                              const size_t active[3], size_t ghost, double gamma) {
+  return load_mhd_snapshot(path, grid, storage, owned, active, ghost, gamma);
 }
