@@ -97,7 +97,7 @@ void free_problem(Problem *P) {
   mi_destroy(*P);
   for (auto &g : P->graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
-  (void)hipFree(P->planes);
+  (void)hipFree(P->planes_alloc);
   (void)hipFree(P->planes32);
   (void)hipFree(P->bytes);
   (void)hipFree(P->scalars);
@@ -122,7 +122,8 @@ int allocate_problem(Problem &P) {
   const size_t nbytes = (size_t)PLANES * P.L.total * sizeof(double);
   const size_t nflags = (size_t)BYTE_PLANES * P.L.total;
   if (hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&P.planes, nbytes) != hipSuccess ||
+      hipMalloc(&P.planes_alloc, nbytes + plane_shift(P.L) * sizeof(double)) != hipSuccess ||
+      (P.planes = P.planes_alloc + plane_shift(P.L), false) ||
       hipMalloc(&P.bytes, nflags) != hipSuccess ||
       hipMalloc(&P.scalars, sizeof(StepScalars)) != hipSuccess ||
       hipMalloc(&P.cfl_partial,

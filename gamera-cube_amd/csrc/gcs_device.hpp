@@ -137,7 +137,8 @@ struct Problem {
   StepOptions opt;
   StepBoundary bnd;
   Layout L;
-  double *planes = nullptr;        // PLANES * L.total doubles
+  double *planes = nullptr;        // PLANES * L.total doubles (planes_alloc + plane_shift)
+  double *planes_alloc = nullptr;
   float *planes32 = nullptr;       // PLANES32 * L.total floats (mixed form)
   uint8_t *bytes = nullptr;        // BYTE_PLANES * L.total
   StepScalars *scalars = nullptr;  // device

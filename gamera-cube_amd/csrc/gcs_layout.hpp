@@ -31,6 +31,13 @@ struct Layout {
   __host__ __device__ long di() const { return (long)SJ * SK; }
 };
 
+// Offset, in doubles, of every FP64 plane from the start of its allocation:
+// rows are a multiple of 16 doubles (SK) apart and planes a multiple of
+// L.total, so with this offset the first active cell of every row (k =
+// ghost) starts a 128-byte cache line, and a wave's run of active cells
+// along k touches no partial line at its start.
+inline int plane_shift(const Layout &L) { return (16 - L.ghost % 16) % 16; }
+
 inline Layout make_layout(int nr, int n, int ghost) {
   Layout L;
   L.nr = nr;
