@@ -1154,7 +1154,7 @@ using FluxTileJ = FluxTile<1, 8, 16>;
 //
 // For edge directions e = 0 (t1 = j, t2 = k) and e = 1 (t1 = k, t2 = i) a
 // block owns a tile of E1 x E2 edges in the (t1, t2) plane at one position
-// along e, with the k direction 32 wide so that every LDS fill row is a
+// along e, with the k direction the long one so that every LDS fill row is a
 // contiguous run. It stages in LDS the cells the tile's t1-face velocities
 // need (vol * m / rho summands and volumes, 4 cells beyond the tile on each
 // side), the t1-faces' areas and B.n along t2 and the t2-faces' along t1,
@@ -1168,12 +1168,17 @@ using FluxTileJ = FluxTile<1, 8, 16>;
 template <int E>
 struct EmfTile {
   static constexpr int T1 = (E + 1) % 3, T2 = (E + 2) % 3;
-  static constexpr int E1 = T1 == 2 ? 32 : 8;  // edges along t1
-  static constexpr int E2 = T2 == 2 ? 32 : 8;  // edges along t2
+  // Edges along t1 and t2: 33 along k by 7 along the other axis (231 of the
+  // 256 threads). An axis of n cells has n + 1 edges, so tiles of 32 x 8
+  // left a one-edge remainder tile on each axis at the drivers' sizes, which
+  // cost nearly as much as a full tile; with 33 x 7, 96 + 1 = 33 + 33 + 31
+  // and 192 + 1 = 27 x 7 + 4 leave no nearly empty tile.
+  static constexpr int E1 = E == 1 ? 33 : 7;
+  static constexpr int E2 = E == 1 ? 7 : 33;
   static constexpr int C1 = E1 + 7, C2 = E2 + 7;
   // Row stride of the t2-face arrays: when the edge loop walks t1 (= k)
   // fastest, an odd stride keeps a wave's reads on distinct LDS banks.
-  static constexpr int S2 = T1 == 2 ? E2 + 1 : E2;
+  static constexpr int S2 = T1 == 2 ? (E2 | 1) : E2;
   // (position along e, along t1, along t2) -> (i, j, k)
   GCS_DEV static void ijk(int ce, int a1, int a2, int &i, int &j, int &k) {
     int x[3];
