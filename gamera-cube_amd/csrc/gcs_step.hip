@@ -1150,6 +1150,80 @@ __global__ void __launch_bounds__(TD_ * TT_) k_face_flux_tiled(K c) {
 using FluxTileI = FluxTile<0, 8, 16>;
 using FluxTileJ = FluxTile<1, 8, 16>;
 
+// The k-faces of the FP64 face flux in LDS rows. A block takes kRowFaces
+// consecutive faces of the active box's k-faces in (cube face, i, j, k)
+// order: the tail of one (i, j) row, whole rows and the head of another.
+// It stages the nine stencil variables of those rows' cells (each row's
+// faces plus 7, contiguous in k) in LDS, so each face reads its 8 x 9
+// stencil values from LDS instead of issuing 72 global loads, and no lane
+// idles on a row's end. A block spans at most kRowMax rows when a row has
+// at least 24 faces (n >= 23); smaller problems use k_face_flux. The
+// arithmetic is face_flux_face's, bit for bit.
+constexpr int kRowFaces = 256, kRowMax = 12;
+constexpr int kRowSlots = kRowFaces + 7 * kRowMax;
+constexpr int kRowMinFaces = (kRowFaces - 1 + kRowMax - 2) / (kRowMax - 1);
+
+__global__ void __launch_bounds__(kRowFaces) k_face_flux_rows(K c) {
+  __shared__ double s_cell[9 * kRowSlots];
+  const Layout &L = c.L;
+  const int nf = L.n + 1;  // k-faces per row
+  const long rows = 6L * L.nr * L.n, total = rows * nf;
+  const long q0 = (long)block_xcd() * kRowFaces;
+  if (q0 >= total) return;
+  const int nq = (int)min((long)kRowFaces, total - q0);  // faces of this block
+  const long row0 = q0 / nf;
+  const int kf0 = (int)(q0 - row0 * nf);
+  const int head = min(nf - kf0, nq);  // faces in the first row
+  const int nrows = 1 + (nq - head + nf - 1) / nf;
+  // Slots: the first row's head + 7 cells from k-face kf0 - 4 on, then each
+  // later row's faces + 7 cells from k-face -4 on; face ql of local row r
+  // finds its stencil at slot ql + 7 r.
+  const int nslots = nq + 7 * nrows, s1 = head + 7;
+  auto row_cell = [&](int r, int ck) {
+    const long row = row0 + r;
+    const long fi = row / L.n;
+    const int jj = (int)(row - fi * L.n);
+    const int f = (int)(fi / L.nr), ii = (int)(fi - (long)f * L.nr);
+    return L.at(f, L.ghost + ii, L.ghost + jj, L.ghost + ck);
+  };
+  constexpr int kFill = (kRowSlots + kRowFaces - 1) / kRowFaces;
+  double fill[kFill][9];
+  int slot[kFill];
+#pragma unroll
+  for (int m = 0; m < kFill; ++m) {
+    const int sl = (int)threadIdx.x + m * kRowFaces;
+    slot[m] = -1;
+    if (sl >= nslots) continue;
+    int r, ck;
+    if (sl < s1) {
+      r = 0;
+      ck = kf0 - 4 + sl;
+    } else {
+      r = 1 + (sl - s1) / (nf + 7);
+      ck = (sl - s1) - (r - 1) * (nf + 7) - 4;
+    }
+    const long cell = row_cell(r, ck);
+    slot[m] = sl;
+    for (int v = 0; v < 5; ++v) fill[m][v] = c.p(P_WPRED + v)[cell];
+    fill[m][5] = c.p(P_VOL)[cell];
+    for (int v = 0; v < 3; ++v) fill[m][6 + v] = c.p(P_BPRED + v)[cell];
+  }
+#pragma unroll
+  for (int m = 0; m < kFill; ++m)
+    if (slot[m] >= 0)
+      for (int v = 0; v < 9; ++v) s_cell[v * kRowSlots + slot[m]] = fill[m][v];
+  __syncthreads();
+  const int ql = (int)threadIdx.x;
+  if (ql >= nq) return;
+  const int r = ql < head ? 0 : 1 + (ql - head) / nf;
+  const int kf = r == 0 ? kf0 + ql : (ql - head) % nf;
+  const long row = row0 + r;
+  const int f = (int)(row / ((long)L.nr * L.n));
+  const long at = row_cell(r, kf);
+  const double *base = s_cell + ql + 7 * r;
+  face_flux_face(c, 2, f, at, 1, [&](int v, int n) { return base[v * kRowSlots + n]; });
+}
+
 // ---- fused face velocity + edge EMF, LDS tiles ------------------------------------
 //
 // For edge directions e = 0 (t1 = j, t2 = k) and e = 1 (t1 = k, t2 = i) a
@@ -2261,9 +2335,8 @@ int enqueue_phase(Problem &P, Phase phase) {
     for (int d = 0; d < 3; ++d) {
       const long n = 6L * (L.nr + (d == 0)) * (L.n + (d == 1)) * (L.n + (d == 2));
       mark_stage(P, S_FACE_FLUX0 + d);
-      // FP64: LDS tiles along i and j; the k stencil is contiguous and stays
-      // on the global-load kernel. The FP32 kernels read half the bytes and
-      // are faster without tiles.
+      // FP64: LDS tiles along i and j, LDS rows along k. The FP32 kernels
+      // read half the bytes and are faster without tiles.
       if (P.mixed)
         k_face_flux_fp32<<<nblocks(n, 128), 128, 0, s>>>(c, d);
       else if (P.plain)
@@ -2274,6 +2347,8 @@ int enqueue_phase(Problem &P, Phase phase) {
       else if (d == 1)
         k_face_flux_tiled<1, FluxTileJ::TD, FluxTileJ::TT>
             <<<FluxTileJ::blocks(L.nr, L.n), FluxTileJ::kThreads, 0, s>>>(c);
+      else if (L.n + 1 >= kRowMinFaces)
+        k_face_flux_rows<<<nblocks(n, kRowFaces), kRowFaces, 0, s>>>(c);
       else
         k_face_flux<<<nblocks(n, 128), 128, 0, s>>>(c, d);
     }
