@@ -111,6 +111,7 @@ void free_problem(Problem *P) {
   (void)hipFree(P->edge_members);
   (void)hipFree(P->edge_groups);
   (void)hipFree(P->seam_cells);
+  (void)hipFree(P->seam_geom);
   (void)hipFree(P->wind_buffer);
   if (P->ev_start) (void)hipEventDestroy(P->ev_start);
   if (P->ev_stop) (void)hipEventDestroy(P->ev_stop);

@@ -182,6 +182,7 @@ struct Problem {
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   std::vector<double> kernel_ms;   // per-stage timing (profiling mode)
   int *seam_cells = nullptr;       // active cells with a seam face
+  double *seam_geom = nullptr;     // their static recovery and CFL data
   int n_seam_cells = 0;
   WindSeries wind_series;
   double *wind_buffer = nullptr;   // device: times, then 8 values per sample
@@ -298,6 +299,9 @@ int setup_problem(Problem &P);
 // atan2 like the reference's plans (cubedsphere_mesh.c:77-103).
 bool cube_inverse(const double x[3], int &face, double &a, double &b);
 int build_plans(Problem &P);
+// Gather the seam cells' static recovery and CFL data (after build_plans'
+// seam list and the geometry).
+int gather_seam_geometry(Problem &P);
 // Options and boundary of the serial Earth driver
 // (test_cubedsphere_dipole_background.c:405-449).
 void driver_options(Problem &P);

@@ -456,7 +456,8 @@ Mirror *mirror_for(const gamera_no_grid *grid, gamera_no_storage *storage,
   for (int f = 0; bg_same && background && f < GAMERA_CS_FACE_COUNT; ++f)
     bg_same = m->background_arrays[f] == background[f].cell_magnetic;
   if (!bg_same) {
-    if (upload_background(*m->P, background) != 0 || derive_constants(*m->P) != 0)
+    if (upload_background(*m->P, background) != 0 || derive_constants(*m->P) != 0 ||
+        gather_seam_geometry(*m->P) != 0)
       return nullptr;
     m->background = background;
     m->has_background = background != nullptr;

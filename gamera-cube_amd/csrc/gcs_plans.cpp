@@ -350,7 +350,7 @@ int build_plans(Problem &P) {
   P.n_edge_groups = (int)groups.size() - 1;
   P.n_edge_members = (int)members.size();
   P.n_seam_cells = (int)seam_cells.size();
-  return 0;
+  return gather_seam_geometry(P);
 }
 
 }  // namespace gcs

@@ -152,7 +152,9 @@ constexpr int kCflThreads = 256;
 // Local CFL dt of one active cell from its conserved state u and residual
 // cell B b (nonorthogonal_step.c:449-525); DBL_MAX, with the failure
 // recorded, when the state is invalid.
-GCS_DEV double cfl_dt(const K &c, long at, const double u[5], const double b[3]) {
+// b0: the cell's background B; len: the minimum of its three CFL lengths.
+GCS_DEV double cfl_dt_with(const K &c, long at, const double u[5], const double b[3],
+                           const double b0[3], double len) {
   Prim w;
   if (!cons_to_prim(u, c.o.ts_gamma, c.o.ts_rho_floor, c.o.ts_p_floor, w)) {
     c.fail(1, at);
@@ -160,19 +162,28 @@ GCS_DEV double cfl_dt(const K &c, long at, const double u[5], const double b[3])
   }
   const double cs = sqrt(c.o.ts_gamma * w.p / w.rho);
   double m[3];
-  for (int d = 0; d < 3; ++d) m[d] = b[d] + c.p(P_B0C + d)[at];
+  for (int d = 0; d < 3; ++d) m[d] = b[d] + b0[d];
   double va = sqrt(dot3(m, m)) / sqrt(w.rho);
   if (c.o.ts_boris) va = c.o.ts_c * va / hypot(c.o.ts_c, va);
   const double speed = sqrt(dot3(w.v, w.v)) + hypot(cs, va);
-  double len = c.p(P_CFL)[at];
-  len = fmin(len, c.p(P_CFL + 1)[at]);
-  len = fmin(len, c.p(P_CFL + 2)[at]);
   const double cand = c.o.ts_cfl * len / speed;
   if (!isfinite(cand) || cand <= 0.0) {
     c.fail(2, at);
     return DBL_MAX;
   }
   return cand;
+}
+
+GCS_DEV double cfl_min_length(const K &c, long at) {
+  double len = c.p(P_CFL)[at];
+  len = fmin(len, c.p(P_CFL + 1)[at]);
+  len = fmin(len, c.p(P_CFL + 2)[at]);
+  return len;
+}
+
+GCS_DEV double cfl_dt(const K &c, long at, const double u[5], const double b[3]) {
+  const double b0[3] = {c.p(P_B0C)[at], c.p(P_B0C + 1)[at], c.p(P_B0C + 2)[at]};
+  return cfl_dt_with(c, at, u, b, b0, cfl_min_length(c, at));
 }
 
 // Minimum of v over the wavefront, folded by lane 0 into one of the slots.
@@ -789,29 +800,74 @@ __global__ void k_recover_ghosts(K c, int phi_plane, int b_plane, int inner_copy
   for (int d = 0; d < 3; ++d) c.p(b_plane + d)[at] = b[d];
 }
 
-// Recover a listed set of cells (active cells with a seam face).
-// With `cfl`, the cells are the current level's seam cells after the final
-// seam sync, and their CFL dt is folded into the CFL slots.
-__global__ void k_recover_list(K c, const int *cells, int n, int phi_plane,
+// Static data of the seam cells' recovery and CFL dt, gathered once per
+// problem in the order of the seam cell list, field-major (kSeamFields
+// arrays of n): the six face centroids, the cell centroid, the recovery
+// factors, the volume, the pivot code, the background B and the minimum CFL
+// length. Half of the seam cells lie alone on their row (the k seams), so
+// reading these from the planes would cost a cache line per value.
+constexpr int kSeamFields = 36;
+enum { SG_FC = 0, SG_CEN = 18, SG_REC = 21, SG_VOL = 30, SG_PIV = 31, SG_B0 = 32, SG_LEN = 35 };
+
+__global__ void k_seam_geometry(K c, const int *cells, int n, double *geom) {
+  const long t = tid();
+  if (t >= n) return;
+  const Layout &L = c.L;
+  const long at = cells[t];
+  const long step[3] = {L.di(), (long)L.SK, 1};
+  auto put = [&](int field, double v) { geom[(long)field * n + t] = v; };
+  for (int d = 0; d < 3; ++d)
+    for (int s = 0; s < 2; ++s)
+      for (int e = 0; e < 3; ++e)
+        put(SG_FC + 3 * (2 * d + s) + e, c.p(P_FACE + FACE_PLANES * d + FO_CEN + e)[at + s * step[d]]);
+  for (int e = 0; e < 3; ++e) put(SG_CEN + e, c.p(P_CEN + e)[at]);
+  for (int r = 0; r < 9; ++r) put(SG_REC + r, c.p(P_REC + r)[at]);
+  put(SG_VOL, c.p(P_VOL)[at]);
+  put(SG_PIV, (double)c.b(Q_PIV)[at]);
+  for (int e = 0; e < 3; ++e) put(SG_B0 + e, c.p(P_B0C + e)[at]);
+  put(SG_LEN, cfl_min_length(c, at));
+}
+
+// Recover the seam cells (active cells with a seam face) from the face
+// fluxes of phi_plane, as recover_at does, with their static data from
+// the gathered arrays. With `cfl`, the cells are the current level's seam
+// cells after the final seam sync, and their CFL dt is folded into the CFL
+// slots.
+__global__ void k_recover_list(K c, const int *cells, const double *geom, int n, int phi_plane,
                                int b_plane, unsigned long long *cfl) {
   const long t = tid();
   double dt_cell = DBL_MAX;
   if (t < n) {
     const long at = cells[t];
-    double b[3];
-    if (!recover_at(c, at, phi_plane, b)) {
+    auto get = [&](int field) { return geom[(long)field * n + t]; };
+    const Layout &L = c.L;
+    const long step[3] = {L.di(), (long)L.SK, 1};
+    const int piv = (int)get(SG_PIV);
+    double flux[6], fc[6][3], cen[3], rec[9], b[3];
+    for (int d = 0; d < 3; ++d) {
+      const double *ph = c.p(phi_plane + d);
+      flux[2 * d] = ph[at];
+      flux[2 * d + 1] = ph[at + step[d]];
+    }
+    for (int q = 0; q < 6; ++q)
+      for (int e = 0; e < 3; ++e) fc[q][e] = get(SG_FC + 3 * q + e);
+    for (int e = 0; e < 3; ++e) cen[e] = get(SG_CEN + e);
+    for (int r = 0; r < 9; ++r) rec[r] = get(SG_REC + r);
+    if (piv == 0xff || !recover_cell(flux, fc, cen, get(SG_VOL), rec, piv, b)) {
       c.fail(23, at);
     } else {
       for (int d = 0; d < 3; ++d) c.p(b_plane + d)[at] = b[d];
       if (cfl != nullptr) {
         double u[5];
         for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
-        dt_cell = cfl_dt(c, at, u, b);
+        const double b0[3] = {get(SG_B0), get(SG_B0 + 1), get(SG_B0 + 2)};
+        dt_cell = cfl_dt_with(c, at, u, b, b0, get(SG_LEN));
       }
     }
   }
   if (cfl != nullptr) cfl_fold(dt_cell, cfl);
 }
+
 
 // ---- predictor -----------------------------------------------------------------
 
@@ -2248,6 +2304,21 @@ void driver_options(Problem &P) {
   P.bnd.earth_cells = P.bnd.inner_trap = P.bnd.earth_emf = true;
 }
 
+int gather_seam_geometry(Problem &P) {
+  (void)hipFree(P.seam_geom);
+  P.seam_geom = nullptr;
+  if (P.n_seam_cells == 0) return 0;
+  if (hipMalloc(&P.seam_geom, (size_t)kSeamFields * P.n_seam_cells * sizeof(double)) !=
+      hipSuccess) {
+    set_error("device allocation failed");
+    return -1;
+  }
+  k_seam_geometry<<<nblocks(P.n_seam_cells, 256), 256, 0, P.stream>>>(
+      make_k(P), P.seam_cells, P.n_seam_cells, P.seam_geom);
+  GCS_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_recover_all(Problem &P, int phi_first, int b_first) {
   K c = make_k(P);
   const Layout &L = P.L;
@@ -2444,9 +2515,9 @@ int enqueue_phase(Problem &P, Phase phase) {
       mark_stage(P, S_FINAL_SYNC);
       k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(n, P.seams, P.n_seams, 0);
       mark_stage(P, S_SEAM_RECOVER);
-      k_recover_list<<<nblocks(P.n_seam_cells, T), T, 0, s>>>(n, P.seam_cells, P.n_seam_cells,
-                                                              plane_phi(n.cur, 0),
-                                                              plane_b(n.cur, 0), cfl);
+      k_recover_list<<<nblocks(P.n_seam_cells, T), T, 0, s>>>(
+          n, P.seam_cells, P.seam_geom, P.n_seam_cells, plane_phi(n.cur, 0), plane_b(n.cur, 0),
+          cfl);
     }
     if (P.fidelity) {
       mark_stage(P, S_FIDELITY);
