@@ -115,6 +115,10 @@ struct MiView {
          b_norm = 0.0;
   double moment[3] = {0.0, 0.0, 0.0}; // startup dipole and radial cut
   double cut_radius = 0.0, cut_length = 0.0;
+  // E x B drift of the inner radial ghost cells for `potential`, field-major
+  // [3][6][ghost][NJ][NK] (NaN where it fails), or null: then the step
+  // kernels compute it. Set by make_k from the problem's MI state.
+  const double *drift = nullptr;
 };
 
 struct MiState;  // owner of the MI arrays and solver (gcs_mi.hip)
@@ -299,6 +303,11 @@ int setup_problem(Problem &P);
 // atan2 like the reference's plans (cubedsphere_mesh.c:77-103).
 bool cube_inverse(const double x[3], int &face, double &a, double &b);
 int build_plans(Problem &P);
+// Recompute the inner ghost cells' drift for `potential` into `drift`
+// (the MiView::drift layout), on the problem's stream.
+int refresh_mi_drift(Problem &P, const double *potential, double *drift);
+// The problem's cached drift when the step view reads its own potential.
+const double *mi_drift_cache(const Problem &P);
 // Gather the seam cells' static recovery and CFL data (after build_plans'
 // seam list and the geometry).
 int gather_seam_geometry(Problem &P);

@@ -78,6 +78,9 @@ struct MiState {
   double *shell_current = nullptr;
   double *fac = nullptr;        // [2][nt][np]
   double *potential = nullptr;  // [2][nt][np]
+  // E x B drift of the inner ghost cells for the current potential
+  // (MiView::drift), refreshed by every writer of `potential`
+  double *drift = nullptr;
   double *work = nullptr;       // [2][kVectors][n] + ring scratch
   double *scal = nullptr;       // [2][SL_COUNT]
   double *host_scal = nullptr;  // pinned mirror of scal
@@ -1547,6 +1550,7 @@ int mi_create(Problem &P) {
   GCS_HIP(hipMemset(m.shell_current, 0, 18L * L.n * L.n * sizeof(double)));
   GCS_HIP(hipMalloc(&m.fac, polar * sizeof(double)));
   GCS_HIP(hipMalloc(&m.potential, polar * sizeof(double)));
+  GCS_HIP(hipMalloc(&m.drift, 18L * L.ghost * L.NJ * L.NK * sizeof(double)));
   GCS_HIP(hipMalloc(&m.work, 2L * kVectors * m.n * sizeof(double)));
   {
     // the cooperative BiCGStab needs all 2 kCoopBlocks workgroups resident;
@@ -1599,6 +1603,7 @@ void mi_destroy(Problem &P) {
   (void)hipFree(m->shell_current);
   (void)hipFree(m->fac);
   (void)hipFree(m->potential);
+  (void)hipFree(m->drift);
   (void)hipFree(m->work);
   (void)hipFree(m->scal);
   (void)hipFree(m->bfringe);
@@ -1629,8 +1634,16 @@ int mi_maybe_update(Problem &P, double time_code) {
   do {
     m.next_update += m.cfg.coupling_interval_s;
   } while (m.next_update <= ts + 1e-10);
+  if (refresh_mi_drift(P, m.potential, m.drift) != 0) return -1;
   P.mi_view.potential = m.potential;
   return 1;
+}
+
+const double *mi_drift_cache(const Problem &P) {
+  return P.mi != nullptr && P.mi->drift != nullptr && P.mi_view.potential != nullptr &&
+                 P.mi_view.potential == P.mi->potential
+             ? P.mi->drift
+             : nullptr;
 }
 
 // The field-aligned current of the current state on the polar grids
@@ -1711,6 +1724,7 @@ int mi_import_potential(Problem &P, const double *values, double epoch_s,
   m.epoch = epoch_s;
   m.next_update = next_update_s;
   m.ready = true;
+  if (refresh_mi_drift(P, m.potential, m.drift) != 0) return -1;
   P.mi_view.potential = m.potential;
   return 0;
 }
@@ -2197,6 +2211,8 @@ int mi_solve_current(Problem &P, const double *fac, double *potential, int use_n
   if (!use_net) m.net = nullptr;
   const int rc = solve(P, m.hall > 0.0);
   m.net = keep;
+  // a later step reads this potential
+  if (refresh_mi_drift(P, m.potential, m.drift) != 0) return -1;
   iterations[0] = m.iterations[0];
   iterations[1] = m.iterations[1];
   *ms = m.solve_ms;

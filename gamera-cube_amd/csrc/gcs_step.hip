@@ -562,10 +562,9 @@ GCS_DEV bool mi_potential(const MiView &m, const double x[3], double &out) {
 }
 
 // :505-543 with NO/nonorthogonal_mi.c:20-41: E from the potential by
-// central differences, the E x B drift of the startup dipole without its
-// radial part, and the ghost velocity 2 v_EB - v.
-GCS_DEV bool mi_ghost_velocity(const MiView &m, const double x[3], const double v[3],
-                               double g[3]) {
+// central differences and the E x B drift of the startup dipole without its
+// radial part (v_EB, the same for both time levels of a ghost cell).
+GCS_DEV bool mi_drift(const MiView &m, const double x[3], double d[3]) {
   const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
   if (!(r > 0.0)) return false;
   const double eps = 1e-4 * fmax(1.0, r);
@@ -596,17 +595,29 @@ GCS_DEV bool mi_ghost_velocity(const MiView &m, const double x[3], const double 
   double b2 = 0.0;
   for (int c = 0; c < 3; ++c) b2 += b[c] * b[c];
   if (!(b2 > 0.0) || !isfinite(b2)) return false;
-  double d[3] = {(e[1] * b[2] - e[2] * b[1]) / b2, (e[2] * b[0] - e[0] * b[2]) / b2,
-                 (e[0] * b[1] - e[1] * b[0]) / b2};
+  d[0] = (e[1] * b[2] - e[2] * b[1]) / b2;
+  d[1] = (e[2] * b[0] - e[0] * b[2]) / b2;
+  d[2] = (e[0] * b[1] - e[1] * b[0]) / b2;
   double rad = 0.0;
   for (int c = 0; c < 3; ++c) rad += d[c] * x[c];
   rad /= r * r;
   for (int c = 0; c < 3; ++c) d[c] -= rad * x[c];
+  return true;
+}
+
+// The ghost velocity 2 v_EB - v from the drift.
+GCS_DEV bool mi_ghost_from_drift(const double d[3], const double v[3], double g[3]) {
   for (int c = 0; c < 3; ++c) {
     g[c] = 2.0 * d[c] - v[c];
     if (!isfinite(g[c])) return false;
   }
   return true;
+}
+
+GCS_DEV bool mi_ghost_velocity(const MiView &m, const double x[3], const double v[3],
+                               double g[3]) {
+  double d[3];
+  return mi_drift(m, x, d) && mi_ghost_from_drift(d, v, g);
 }
 
 // One point query of the potential (what 0) or the MI ghost velocity (what 1)
@@ -637,6 +648,16 @@ __global__ void k_earth_cells(K c, WindSeries w) {
   const long t = L.at(f, i, j, k), s = L.at(f, si, j, k);
   const double x[3] = {c.p(P_CEN)[t], c.p(P_CEN + 1)[t], c.p(P_CEN + 2)[t]};
   const double rad = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  double drift[3];
+  const bool coupled = inner && c.mi.potential != nullptr;
+  bool drift_ok = false;
+  if (coupled && c.mi.drift != nullptr) {
+    const long n = 6L * L.ghost * L.NJ * L.NK, q = (((long)f * L.ghost + i) * L.NJ + j) * L.NK + k;
+    for (int e = 0; e < 3; ++e) drift[e] = c.mi.drift[e * n + q];
+    drift_ok = true;  // a failed drift is NaN, which the ghost velocity rejects
+  } else if (coupled) {
+    drift_ok = mi_drift(c.mi, x, drift);
+  }
   for (int lv = 0; lv < 2; ++lv) {
     const int level = lv == 0 ? c.cur : c.old;
     const double time = lv == 0 ? c.sc->time : c.sc->previous_time;
@@ -652,7 +673,7 @@ __global__ void k_earth_cells(K c, WindSeries w) {
       if (c.mi.potential != nullptr) {
         // cubedsphere_earth_boundary.c:58-68: the ionospheric drift
         double g[3];
-        if (!mi_ghost_velocity(c.mi, x, pw.v, g)) {
+        if (!drift_ok || !mi_ghost_from_drift(drift, pw.v, g)) {
           c.fail(14, t);
           return;
         }
@@ -2259,6 +2280,7 @@ static K make_k(Problem &P) {
   c.o = P.opt;
   c.o.rates = P.fidelity ? 1 : 0;
   c.mi = P.mi_view;
+  c.mi.drift = mi_drift_cache(P);
   return c;
 }
 
@@ -2302,6 +2324,33 @@ void driver_options(Problem &P) {
   o.strict_wall = cfg.strict_inner_gas_wall != 0;
   P.bnd.outflow = false;
   P.bnd.earth_cells = P.bnd.inner_trap = P.bnd.earth_emf = true;
+}
+
+// The drift of every inner ghost cell for `potential` (MiView::drift).
+__global__ void k_mi_drift_cache(K c, double *drift) {
+  const Layout &L = c.L;
+  const long n = 6L * L.ghost * L.NJ * L.NK, q = tid();
+  if (q >= n) return;
+  const int k = (int)(q % L.NK);
+  const long r = q / L.NK;
+  const int j = (int)(r % L.NJ);
+  const int i = (int)((r / L.NJ) % L.ghost), f = (int)(r / ((long)L.NJ * L.ghost));
+  const long t = L.at(f, i, j, k);
+  const double x[3] = {c.p(P_CEN)[t], c.p(P_CEN + 1)[t], c.p(P_CEN + 2)[t]};
+  double d[3];
+  if (!mi_drift(c.mi, x, d)) d[0] = d[1] = d[2] = NAN;
+  for (int e = 0; e < 3; ++e) drift[e * n + q] = d[e];
+}
+
+int refresh_mi_drift(Problem &P, const double *potential, double *drift) {
+  if (drift == nullptr) return 0;
+  K c = make_k(P);
+  c.mi.potential = potential;
+  c.mi.drift = nullptr;
+  const long n = 6L * P.L.ghost * P.L.NJ * P.L.NK;
+  k_mi_drift_cache<<<nblocks(n, 256), 256, 0, P.stream>>>(c, drift);
+  GCS_HIP(hipGetLastError());
+  return 0;
 }
 
 int gather_seam_geometry(Problem &P) {
