@@ -1356,7 +1356,9 @@ __global__ void __launch_bounds__(256) k_edge_emf_tiled(K c) {
   const int lo1 = L.ghost, hi1 = L.ghost + act[t1] + 1;
   const int lo2 = L.ghost, hi2 = L.ghost + act[t2] + 1;
   const int nt1 = (hi1 - lo1 + E1 - 1) / E1, nt2 = (hi2 - lo2 + E2 - 1) / E2;
-  long b = blockIdx.x;
+  // consecutive logical blocks, neighbours along t2 that share fill cells,
+  // run on one XCD (one L2)
+  long b = block_xcd();
   const int tile2 = (int)(b % nt2);
   b /= nt2;
   const int tile1 = (int)(b % nt1);
