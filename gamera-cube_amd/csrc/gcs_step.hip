@@ -851,9 +851,10 @@ __global__ void k_seam_geometry(K c, const int *cells, int n, double *geom) {
 
 // Recover the seam cells (active cells with a seam face) from the face
 // fluxes of phi_plane, as recover_at does, with their static data from
-// the gathered arrays. With `cfl`, the cells are the current level's seam
-// cells after the final seam sync, and their CFL dt is folded into the CFL
-// slots.
+// the gathered arrays (from the planes when geom is null: the plain-kernel
+// switch, which tests hold the two to). With `cfl`, the cells are the
+// current level's seam cells after the final seam sync, and their CFL dt is
+// folded into the CFL slots.
 __global__ void k_recover_list(K c, const int *cells, const double *geom, int n, int phi_plane,
                                int b_plane, unsigned long long *cfl) {
   const long t = tid();
@@ -861,28 +862,39 @@ __global__ void k_recover_list(K c, const int *cells, const double *geom, int n,
   if (t < n) {
     const long at = cells[t];
     auto get = [&](int field) { return geom[(long)field * n + t]; };
-    const Layout &L = c.L;
-    const long step[3] = {L.di(), (long)L.SK, 1};
-    const int piv = (int)get(SG_PIV);
-    double flux[6], fc[6][3], cen[3], rec[9], b[3];
-    for (int d = 0; d < 3; ++d) {
-      const double *ph = c.p(phi_plane + d);
-      flux[2 * d] = ph[at];
-      flux[2 * d + 1] = ph[at + step[d]];
+    double b[3];
+    bool ok;
+    if (geom != nullptr) {
+      const Layout &L = c.L;
+      const long step[3] = {L.di(), (long)L.SK, 1};
+      const int piv = (int)get(SG_PIV);
+      double flux[6], fc[6][3], cen[3], rec[9];
+      for (int d = 0; d < 3; ++d) {
+        const double *ph = c.p(phi_plane + d);
+        flux[2 * d] = ph[at];
+        flux[2 * d + 1] = ph[at + step[d]];
+      }
+      for (int q = 0; q < 6; ++q)
+        for (int e = 0; e < 3; ++e) fc[q][e] = get(SG_FC + 3 * q + e);
+      for (int e = 0; e < 3; ++e) cen[e] = get(SG_CEN + e);
+      for (int r = 0; r < 9; ++r) rec[r] = get(SG_REC + r);
+      ok = piv != 0xff && recover_cell(flux, fc, cen, get(SG_VOL), rec, piv, b);
+    } else {
+      ok = recover_at(c, at, phi_plane, b);
     }
-    for (int q = 0; q < 6; ++q)
-      for (int e = 0; e < 3; ++e) fc[q][e] = get(SG_FC + 3 * q + e);
-    for (int e = 0; e < 3; ++e) cen[e] = get(SG_CEN + e);
-    for (int r = 0; r < 9; ++r) rec[r] = get(SG_REC + r);
-    if (piv == 0xff || !recover_cell(flux, fc, cen, get(SG_VOL), rec, piv, b)) {
+    if (!ok) {
       c.fail(23, at);
     } else {
       for (int d = 0; d < 3; ++d) c.p(b_plane + d)[at] = b[d];
       if (cfl != nullptr) {
         double u[5];
         for (int q = 0; q < 5; ++q) u[q] = c.p(plane_u(c.cur, q))[at];
-        const double b0[3] = {get(SG_B0), get(SG_B0 + 1), get(SG_B0 + 2)};
-        dt_cell = cfl_dt_with(c, at, u, b, b0, get(SG_LEN));
+        if (geom != nullptr) {
+          const double b0[3] = {get(SG_B0), get(SG_B0 + 1), get(SG_B0 + 2)};
+          dt_cell = cfl_dt_with(c, at, u, b, b0, get(SG_LEN));
+        } else {
+          dt_cell = cfl_dt(c, at, u, b);
+        }
       }
     }
   }
@@ -2282,7 +2294,8 @@ static K make_k(Problem &P) {
   c.o = P.opt;
   c.o.rates = P.fidelity ? 1 : 0;
   c.mi = P.mi_view;
-  c.mi.drift = mi_drift_cache(P);
+  // the plain-kernel switch computes the drift in the step instead
+  c.mi.drift = P.plain ? nullptr : mi_drift_cache(P);
   return c;
 }
 
@@ -2567,8 +2580,8 @@ int enqueue_phase(Problem &P, Phase phase) {
       k_sync_phi<<<nblocks(P.n_seams, T), T, 0, s>>>(n, P.seams, P.n_seams, 0);
       mark_stage(P, S_SEAM_RECOVER);
       k_recover_list<<<nblocks(P.n_seam_cells, T), T, 0, s>>>(
-          n, P.seam_cells, P.seam_geom, P.n_seam_cells, plane_phi(n.cur, 0), plane_b(n.cur, 0),
-          cfl);
+          n, P.seam_cells, P.plain ? nullptr : P.seam_geom, P.n_seam_cells, plane_phi(n.cur, 0),
+          plane_b(n.cur, 0), cfl);
     }
     if (P.fidelity) {
       mark_stage(P, S_FIDELITY);

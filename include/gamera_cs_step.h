@@ -140,11 +140,14 @@ int gcs_problem_set_wind(gcs_problem *problem, size_t count, const double *time,
                          const double *state, int linear);
 
 /* 1: run every face-flux, face-velocity and edge-EMF sweep with one face or
- * edge per thread (the untiled kernels), in FP64 and in the mixed form.
- * 0 (default): the LDS-tiled and register-march kernels. Both compute each
- * face and edge with the same expressions, so results are bit-identical;
- * the switch lets tests hold the tiled kernels to the plain ones at any
- * size. No reference counterpart. */
+ * edge per thread (the untiled kernels), in FP64 and in the mixed form; the
+ * seam recovery reads its cells' static data from the planes, and the Earth
+ * boundary computes the ionosphere drift of its ghost cells in every step.
+ * 0 (default): the LDS-tiled and register-march kernels, the seam data
+ * gathered once per problem and the drift cached per potential. Both compute
+ * each value with the same expressions, so results are bit-identical; the
+ * switch lets tests hold the fast kernels to the plain ones at any size. No
+ * reference counterpart. */
 int gcs_problem_set_plain_kernels(gcs_problem *problem, int on);
 
 /* Sample the field-aligned current of the current state onto the MI polar

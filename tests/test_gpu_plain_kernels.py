@@ -1,9 +1,11 @@
 """The LDS-tiled face-flux and edge-EMF kernels and the register-march
-face-velocity kernels against the one-face-per-thread kernels: the same
-expressions per face and edge, so every array must be bit-identical, in
-FP64 and in the mixed form, at the ragged parity size (every tile partial),
-at the small size (16 x 12^2 per face) and at config 1 (6 x 96^2 x
-192), the size the tiles were tuned at."""
+face-velocity kernels against the one-face-per-thread kernels, and the
+seam recovery's gathered data and the cached ionosphere drift against
+their reads from the planes and the per-step drift: the same expressions
+per value, so every array must be bit-identical, in FP64 and in the mixed
+form, at the ragged parity size (every tile partial), at the small size
+(16 x 12^2 per face), with the MI coupling across two ionosphere updates,
+and at config 1 (6 x 96^2 x 192), the size the tiles were tuned at."""
 import pytest
 
 from conftest import bits_equal, gcs_config, problem
@@ -20,6 +22,7 @@ CHECKED = (problem.STATE_FIELDS
 @pytest.mark.timeout(600)
 @pytest.mark.parametrize("name,steps,mixed", [("ragged", 6, False), ("ragged", 6, True),
                                               ("small", 4, False),
+                                              ("small_mi", 44, False),
                                               ("config1_nomi", 2, False),
                                               ("config1_nomi", 2, True)])
 def test_tiled_kernels_bitwise_equal_plain_kernels(gpu, name, steps, mixed):
