@@ -1172,8 +1172,11 @@ struct FluxTile {
   }
 };
 
+// Four waves per SIMD: unbounded, the kernel takes 129 VGPRs and runs three;
+// bounded, it spills 8 registers a lane (0.89 / 0.91 -> 0.81 / 0.83 ms at
+// config 1; k_face_flux_rows likewise, 0.97 -> 0.90 ms).
 template <int D, int TD_, int TT_>
-__global__ void __launch_bounds__(TD_ * TT_) k_face_flux_tiled(K c) {
+__global__ void __launch_bounds__(TD_ * TT_, 4) k_face_flux_tiled(K c) {
   using T = FluxTile<D, TD_, TT_>;
   constexpr int TD = T::TD, TT = T::TT, CD = T::CD, TA = T::TA, XA = T::XA;
   constexpr int kSlots = T::kSlots;
@@ -1252,7 +1255,7 @@ constexpr int kRowFaces = 256, kRowMax = 12;
 constexpr int kRowSlots = kRowFaces + 7 * kRowMax;
 constexpr int kRowMinFaces = (kRowFaces - 1 + kRowMax - 2) / (kRowMax - 1);
 
-__global__ void __launch_bounds__(kRowFaces) k_face_flux_rows(K c) {
+__global__ void __launch_bounds__(kRowFaces, 4) k_face_flux_rows(K c) {
   __shared__ double s_cell[9 * kRowSlots];
   const Layout &L = c.L;
   const int nf = L.n + 1;  // k-faces per row
