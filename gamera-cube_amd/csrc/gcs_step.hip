@@ -919,29 +919,54 @@ __global__ void k_predict(K c) {
   const double r = c.sc->ratio;
   const long step[3] = {L.di(), (long)L.SK, 1};
   const bool in[3] = {i < L.NI, j < L.NJ, k < L.NK};
+  const bool cell = in[0] && in[1] && in[2];
+  // Every load is issued before the first store: the planes share one
+  // allocation, so a load after a store could not be moved above it.
+  bool face[3];
+  double cf[3], of[3], area[3];
   for (int d = 0; d < 3; ++d) {
-    const bool face = (d == 0 || in[0]) && (d == 1 || in[1]) && (d == 2 || in[2]);
-    if (!face) continue;
-    const double cf = c.p(plane_phi(c.cur, d))[at];
-    const double of = c.p(plane_phi(c.old, d))[at];
-    const double pred = cf + r * (cf - of);
+    face[d] = (d == 0 || in[0]) && (d == 1 || in[1]) && (d == 2 || in[2]);
+    if (!face[d]) continue;
+    cf[d] = c.p(plane_phi(c.cur, d))[at];
+    of[d] = c.p(plane_phi(c.old, d))[at];
+    area[d] = c.p(P_FACE + FACE_PLANES * d + FO_AREA)[at];
+  }
+  double uo[5], uc[5], vol = 0.0, cu[3], ou[3], fc[6][3], cen[3], rec[9];
+  int piv = 0xff;
+  if (cell) {
+    for (int q = 0; q < 5; ++q) {
+      uo[q] = c.p(plane_u(c.old, q))[at];
+      uc[q] = c.p(plane_u(c.cur, q))[at];
+    }
+    vol = c.p(P_VOL)[at];
+    piv = c.b(Q_PIV)[at];
+    for (int d = 0; d < 3; ++d) {
+      cu[d] = c.p(plane_phi(c.cur, d))[at + step[d]];
+      ou[d] = c.p(plane_phi(c.old, d))[at + step[d]];
+      for (int s = 0; s < 2; ++s)
+        for (int e = 0; e < 3; ++e)
+          fc[2 * d + s][e] = c.p(P_FACE + FACE_PLANES * d + FO_CEN + e)[at + s * step[d]];
+    }
+    for (int e = 0; e < 3; ++e) cen[e] = c.p(P_CEN + e)[at];
+    for (int q = 0; q < 9; ++q) rec[q] = c.p(P_REC + q)[at];
+  }
+  double pred[3];
+  for (int d = 0; d < 3; ++d) {
+    if (!face[d]) continue;
+    pred[d] = cf[d] + r * (cf[d] - of[d]);
     // The predicted flux itself is storage output only (the step reads its
     // B.n below, and the outer wall recomputes it): written with fidelity.
-    if (c.o.rates) c.p(P_PHIPRED + d)[at] = pred;
+    if (c.o.rates) c.p(P_PHIPRED + d)[at] = pred[d];
     // Face-normal B of the predicted flux, read by the face fluxes and the
     // edge EMF (both divide this flux by this area).
-    const double bn = pred / c.p(P_FACE + FACE_PLANES * d + FO_AREA)[at];
+    const double bn = pred[d] / area[d];
     if (Mixed)
       c.sp(M_BN + d)[at] = (float)bn;
     else
       c.p(P_BN + d)[at] = bn;
   }
-  if (!(in[0] && in[1] && in[2])) return;
-  double uo[5], uc[5], up[5];
-  for (int q = 0; q < 5; ++q) {
-    uo[q] = c.p(plane_u(c.old, q))[at];
-    uc[q] = c.p(plane_u(c.cur, q))[at];
-  }
+  if (!cell) return;
+  double up[5];
   const double nan = __longlong_as_double(0x7ff8000000000000ll);
   auto wpred = [&](int q, double v) {
     if (Mixed)
@@ -975,7 +1000,6 @@ __global__ void k_predict(K c) {
   // Volume-weighted velocity with the EMF's density floor, the summand of the
   // face velocity's central8 (nonorthogonal_sweep.c:223-259).
   {
-    const double vol = c.p(P_VOL)[at];
     const double rho = fmax(up[0], c.o.emf_rho_floor);
     for (int e = 0; e < 3; ++e) {
       const double wv = vol * up[1 + e] / rho;
@@ -985,22 +1009,14 @@ __global__ void k_predict(K c) {
         c.p(P_WV + e)[at] = wv;
     }
   }
-  // B of the predicted fluxes; the upper faces' predictions are recomputed
-  // here with the same expression rather than read back.
-  const int piv = c.b(Q_PIV)[at];
-  double flux[6], fc[6][3], cen[3], rec[9], b[3];
+  // B of the predicted fluxes (the lower faces' predictions above, the
+  // upper faces' by the same expression).
+  double flux[6], b[3];
   for (int d = 0; d < 3; ++d) {
-    const double *cp = c.p(plane_phi(c.cur, d)), *op = c.p(plane_phi(c.old, d));
-    for (int s = 0; s < 2; ++s) {
-      const long fa = at + s * step[d];
-      flux[2 * d + s] = cp[fa] + r * (cp[fa] - op[fa]);
-      for (int e = 0; e < 3; ++e)
-        fc[2 * d + s][e] = c.p(P_FACE + FACE_PLANES * d + FO_CEN + e)[fa];
-    }
+    flux[2 * d] = pred[d];
+    flux[2 * d + 1] = cu[d] + r * (cu[d] - ou[d]);
   }
-  for (int e = 0; e < 3; ++e) cen[e] = c.p(P_CEN + e)[at];
-  for (int q = 0; q < 9; ++q) rec[q] = c.p(P_REC + q)[at];
-  if (piv == 0xff || !recover_cell(flux, fc, cen, c.p(P_VOL)[at], rec, piv, b)) {
+  if (piv == 0xff || !recover_cell(flux, fc, cen, vol, rec, piv, b)) {
     c.fail(32, at);
     return;
   }
