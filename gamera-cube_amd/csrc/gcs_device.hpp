@@ -144,6 +144,7 @@ struct Problem {
   double *planes = nullptr;        // PLANES * L.total doubles (planes_alloc + plane_shift)
   double *planes_alloc = nullptr;
   float *planes32 = nullptr;       // PLANES32 * L.total floats (mixed form)
+  float *planes32_alloc = nullptr; // its allocation (planes32 - plane_shift32)
   uint8_t *bytes = nullptr;        // BYTE_PLANES * L.total
   StepScalars *scalars = nullptr;  // device
   unsigned long long *cfl_partial = nullptr;  // per-block CFL minima

@@ -37,6 +37,10 @@ struct Layout {
 // ghost) starts a 128-byte cache line, and a wave's run of active cells
 // along k touches no partial line at its start.
 inline int plane_shift(const Layout &L) { return (16 - L.ghost % 16) % 16; }
+// The same for the FP32 planes (32 floats a line): rows 7 lines of 16
+// doubles apart are 3.5 lines of floats apart, so the first active cell
+// starts a line in every other row.
+inline int plane_shift32(const Layout &L) { return (32 - L.ghost % 32) % 32; }
 
 inline Layout make_layout(int nr, int n, int ghost) {
   Layout L;

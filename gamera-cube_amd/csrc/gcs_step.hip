@@ -2710,12 +2710,14 @@ __global__ void __launch_bounds__(256) k_div_b(K c, unsigned long long *out) {
 int prepare_mixed(Problem &P) {
   if (P.planes32 != nullptr) return 0;
   const size_t bytes = (size_t)PLANES32 * (size_t)P.L.total * sizeof(float);
-  if (hipMalloc((void **)&P.planes32, bytes) != hipSuccess) {
-    P.planes32 = nullptr;
+  if (hipMalloc((void **)&P.planes32_alloc, bytes + plane_shift32(P.L) * sizeof(float)) !=
+      hipSuccess) {
+    P.planes32_alloc = P.planes32 = nullptr;
     set_error("mixed form: cannot allocate " + std::to_string(bytes >> 20) +
               " MiB of FP32 planes");
     return -1;
   }
+  P.planes32 = P.planes32_alloc + plane_shift32(P.L);
   GCS_HIP(hipMemsetAsync(P.planes32, 0, bytes, P.stream));
   k_mixed_geometry<<<nblocks(P.L.total, 256), 256, 0, P.stream>>>(make_k(P));
   GCS_HIP(hipGetLastError());
