@@ -664,11 +664,14 @@ int backward_device(Net &n, int batch) {
     const int H = n.H >> lv, W = n.W >> lv;
     const long M = (long)batch * H * W;
     k_relu_grad<<<nb(M * L.cout), kT, 0, s>>>(n.gact[l + 1], n.act[l + 1], M * L.cout, n.gz);
-    const int chunk = 1024;
+    // Pixel chunks small enough that the sums run in parallel: a bias sum
+    // is one thread's loop over its chunk, and a weight-gradient launch has
+    // (K / 64) (cout / 64) blocks per chunk, as few as five at level 0.
+    const int chunk = 64;
     k_colsum<<<nb(M, chunk), 256, 0, s>>>(n.gz, (int)M, L.cout, n.grads + n.b_off[l], chunk);
     const ConvIn in = layer_input(n, l);
     const int K = 9 * L.cin;
-    const int wchunk = 2048;
+    const int wchunk = 256;
     k_wgrad3<<<dim3((K + 63) / 64, (L.cout + 63) / 64, nb(M, wchunk)), 256, 0, s>>>(
         in, batch, H, W, n.gz, L.cout, n.grads + n.w_off[l], wchunk);
     if (l == 0) continue;  // no gradient for the input features
