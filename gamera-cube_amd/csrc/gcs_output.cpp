@@ -217,10 +217,16 @@ int load_snapshot(Problem &P, const char *path) {
       char line[1024];
       for (long r = (long)w * chunk; r < std::min(n, (long)(w + 1) * chunk) && !bad[w]; ++r) {
         const size_t b = starts[(size_t)r];
-        const size_t e = std::min(text.find('\n', b), text.size());
-        const size_t len = std::min(e - b + 1, sizeof(line) - 1);
-        std::memcpy(line, text.data() + b, len);
-        line[len] = '\0';
+        const size_t nl = text.find('\n', b);
+        const size_t e = nl == std::string::npos ? text.size() : nl + 1;
+        // fgets(line, 1024) would split a longer line, and the reference
+        // then fails on its remainder
+        if (e - b > sizeof(line) - 1) {
+          bad[w] = 1;
+          break;
+        }
+        std::memcpy(line, text.data() + b, e - b);
+        line[e - b] = '\0';
         int face;
         size_t i, j, k;
         double *x = &rows[11 * (size_t)r];

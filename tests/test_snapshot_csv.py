@@ -197,8 +197,9 @@ def test_device_load_is_the_drivers_load(gpu, tmp_path):
 def test_device_load_rejects_what_the_driver_rejects(gpu, tmp_path):
     """Files the driver's loader rejects (wrong header, a missing line, a
     cell index out of range, a centroid moved by 1e-6, a density the
-    conversion refuses) make gcs_problem_load_snapshot fail with the state
-    unchanged; the reference's loader returns -1 on each as well."""
+    conversion refuses, a line longer than its read buffer) make
+    gcs_problem_load_snapshot fail with the state unchanged; the
+    reference's loader returns -1 on each as well."""
     if not os.path.exists(REF_LIB):
         pytest.skip("compiled reference (oracle/_ref) not built")
     ref = ctypes.CDLL(REF_LIB)
@@ -227,6 +228,9 @@ def test_device_load_rejects_what_the_driver_rejects(gpu, tmp_path):
         "i out of range": edit(7, 1, lambda v: str(cfg.radial_cells)),
         "centroid": edit(len(lines) // 2, 5, lambda v: repr(float(v) + 1e-6)),
         "density": edit(len(lines) - 3, 7, lambda v: "nan"),
+        # longer than the driver's 1024-byte fgets buffer: the reference
+        # parses the line's head and fails on its remainder
+        "long line": lines[:9] + [lines[9].rstrip("\n") + " " * 1100 + "\n"] + lines[10:],
     }
     before = [g.field(f, problem.CONSERVED).copy() for f in range(gcs_config.FACE_COUNT)]
     r = problem.Problem(ref, "refh", cfg)
