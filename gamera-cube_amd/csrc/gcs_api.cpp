@@ -98,6 +98,7 @@ void free_problem(Problem *P) {
   for (auto &g : P->graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   (void)hipFree(P->planes_alloc);
+  if (P->pinned != nullptr) (void)hipHostFree(P->pinned);
   (void)hipFree(P->planes32_alloc);
   (void)hipFree(P->bytes);
   (void)hipFree(P->scalars);

@@ -216,7 +216,8 @@ struct Problem {
   std::vector<hipEvent_t> stage_events;
   std::vector<int> stage_ids;      // stage id of each recorded event
   size_t stage_cursor = 0;
-  std::vector<double> staging;     // host buffer for plane transfers
+  double *pinned = nullptr;        // pinned host buffer for plane transfers
+  size_t pinned_count = 0;         // its size in doubles
   MiState *mi = nullptr;           // ionosphere coupling (mi_enabled)
   MiView mi_view;                  // what the step kernels read
 
