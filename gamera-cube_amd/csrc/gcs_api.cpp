@@ -124,9 +124,12 @@ int allocate_problem(Problem &P) {
   const size_t nbytes = (size_t)PLANES * P.L.total * sizeof(double);
   const size_t nflags = (size_t)BYTE_PLANES * P.L.total;
   if (hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&P.planes_alloc, nbytes + plane_shift(P.L) * sizeof(double)) != hipSuccess ||
-      (P.planes = P.planes_alloc + plane_shift(P.L), false) ||
-      hipMalloc(&P.bytes, nflags) != hipSuccess ||
+      hipMalloc(&P.planes_alloc, nbytes + plane_shift(P.L) * sizeof(double)) != hipSuccess) {
+    set_error("device allocation failed");
+    return -1;
+  }
+  P.planes = P.planes_alloc + plane_shift(P.L);
+  if (hipMalloc(&P.bytes, nflags) != hipSuccess ||
       hipMalloc(&P.scalars, sizeof(StepScalars)) != hipSuccess ||
       hipMalloc(&P.cfl_partial,
                 ((6L * P.L.nr * P.L.n * P.L.n + 255) / 256 + 1) * sizeof(unsigned long long)) !=
